@@ -1,0 +1,521 @@
+// fb_api.hip -- C ABI of the swarm log-likelihood library (include/ferebus_hip.h).
+//
+// Host side of the drop-in boundary: validates the kernel specification,
+// expands it into the device KernelProgram, owns every device buffer of a
+// training set, and enqueues the swarm pipeline
+//   hyper + build -> { gemm_step, potrf_diag, trsm_step } x nsteps -> likelihood
+// on one HIP stream.  The batched path replays a captured hipGraph of that
+// pipeline; the plain path launches it eagerly, one particle at a time.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "fb_common.h"
+#include "../../include/ferebus_hip.h"
+
+using fb::KernelProgram;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define FB_HIP(call)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return fail(FB_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_));    \
+    } while (0)
+
+// Expands the postfix program of the kernel expression into a sum of
+// products of leaves (the terms), and assigns feature slots.
+int make_program(const fb_kernel_spec *sp, int d, int optimise_noise, double noise,
+                 KernelProgram &prog, std::vector<int> &slot_feature, std::vector<int> &slot_kind)
+{
+    if (!sp || sp->n_nodes < 1 || sp->n_nodes > 64 || !sp->node_type)
+        return fail(FB_E_SPEC, "kernel spec: empty or too long program");
+    std::memset(&prog, 0, sizeof(prog));
+    std::vector<std::vector<uint32_t>> stack;
+    std::map<std::pair<int, int>, int> slot_of;  // (feature, kind) -> slot
+    int par = 0, doff = 0, ndims = 0;
+    for (int k = 0; k < sp->n_nodes; ++k) {
+        const int t = sp->node_type[k];
+        const int nd = sp->node_ndim ? sp->node_ndim[k] : 0;
+        if (t == FB_OP_SUM || t == FB_OP_PROD) {
+            if (stack.size() < 2) return fail(FB_E_SPEC, "kernel spec: operator without two operands");
+            std::vector<uint32_t> b = stack.back();
+            stack.pop_back();
+            std::vector<uint32_t> a = stack.back();
+            stack.pop_back();
+            std::vector<uint32_t> r;
+            if (t == FB_OP_SUM) {
+                r = a;
+                r.insert(r.end(), b.begin(), b.end());
+            } else {
+                for (uint32_t x : a)
+                    for (uint32_t y : b) r.push_back(x | y);
+            }
+            if ((int)r.size() > fb::kMaxTerms)
+                return fail(FB_E_SPEC, "kernel spec: more than 16 product terms");
+            stack.push_back(r);
+            continue;
+        }
+        const int l = prog.n_leaves;
+        if (l >= fb::kMaxLeaves) return fail(FB_E_SPEC, "kernel spec: more than 8 leaves");
+        int npar = 0;
+        switch (t) {
+        case FB_LEAF_RBF: case FB_LEAF_PERIODIC: case FB_LEAF_LINEAR:
+            if (nd < 1) return fail(FB_E_SPEC, "kernel spec: leaf without active dimensions");
+            npar = nd;
+            break;
+        case FB_LEAF_RBF_CYCLIC:
+            if (nd != d) return fail(FB_E_SPEC, "kernel spec: rbf-cyclic must span every feature");
+            npar = nd;
+            break;
+        case FB_LEAF_CONSTANT:
+            if (nd != 0) return fail(FB_E_SPEC, "kernel spec: constant leaf takes no dimensions");
+            npar = 1;
+            break;
+        default:
+            return fail(FB_E_SPEC, "kernel spec: unknown node type");
+        }
+        prog.leaf_kind[l] = t;
+        prog.leaf_ndim[l] = (t == FB_LEAF_CONSTANT) ? 0 : nd;
+        prog.leaf_par_off[l] = par;
+        prog.leaf_dim_off[l] = ndims;
+        for (int q = 0; q < prog.leaf_ndim[l]; ++q) {
+            if (!sp->active_dims) return fail(FB_E_SPEC, "kernel spec: missing active_dims");
+            const int f = sp->active_dims[doff + q];
+            if (f < 1 || f > d) return fail(FB_E_SPEC, "kernel spec: active dimension out of range");
+            if (t == FB_LEAF_RBF_CYCLIC && f != q + 1)
+                return fail(FB_E_SPEC, "kernel spec: rbf-cyclic dimensions must be 1..d in order");
+            if (ndims >= fb::kMaxDims) return fail(FB_E_SPEC, "kernel spec: too many dimensions");
+            int code = 0;
+            if (t == FB_LEAF_RBF || t == FB_LEAF_RBF_CYCLIC || t == FB_LEAF_PERIODIC) {
+                const int skind = (t == FB_LEAF_PERIODIC) ? 1 : 0;
+                auto key = std::make_pair(f - 1, skind);
+                auto it = slot_of.find(key);
+                int slot;
+                if (it == slot_of.end()) {
+                    slot = (int)slot_feature.size();
+                    slot_of[key] = slot;
+                    slot_feature.push_back(f - 1);
+                    slot_kind.push_back(skind == 1 ? 1 : 0);
+                    if (skind == 1) {  // cos(x/2) right after sin(x/2)
+                        slot_feature.push_back(f - 1);
+                        slot_kind.push_back(2);
+                    }
+                } else {
+                    slot = it->second;
+                }
+                // distance_matrix.f90:119-120 wraps features 3, 6, 9, ...
+                const int wrap = (t == FB_LEAF_RBF_CYCLIC && f % 3 == 0) ? 1 : 0;
+                code = slot | (wrap << 16);
+            }
+            prog.dims[ndims++] = code;
+        }
+        doff += (t == FB_LEAF_CONSTANT) ? 0 : nd;
+        par += npar;
+        prog.n_leaves++;
+        stack.push_back(std::vector<uint32_t>{1u << l});
+    }
+    if (stack.size() != 1) return fail(FB_E_SPEC, "kernel spec: program does not reduce to one kernel");
+    const std::vector<uint32_t> &terms = stack.back();
+    prog.n_terms = (int)terms.size();
+    for (int t = 0; t < prog.n_terms; ++t) prog.term_mask[t] = terms[t];
+    prog.n_dims = ndims;
+    prog.n_par_kernel = par;
+    prog.optimise_noise = optimise_noise ? 1 : 0;
+    prog.n_par = par + prog.optimise_noise;
+    prog.noise = noise;
+    return FB_OK;
+}
+
+struct EventPair {
+    int kind;
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct fb_handle {
+    int n = 0, d = 0, np = 0, nrhs = 0, nsteps = 0, likelihood = 0, max_swarm = 0;
+    int n_slots = 0;
+    KernelProgram prog;
+    hipStream_t own_stream = nullptr, stream = nullptr;
+    int device = 0;
+    // device buffers
+    KernelProgram *d_prog = nullptr;
+    double *d_slots = nullptr, *d_rhs = nullptr, *d_theta = nullptr, *d_hyp = nullptr;
+    double *d_K = nullptr, *d_logdet = nullptr, *d_inv = nullptr, *d_ll = nullptr;
+    double *d_extra = nullptr, *d_z = nullptr, *d_alpha = nullptr;
+    int *d_info = nullptr;
+    // pinned staging
+    double *h_theta = nullptr, *h_ll = nullptr, *h_extra = nullptr;
+    int *h_info = nullptr;
+    // captured pipelines of the batched path, by swarm size
+    std::map<int, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
+    // profiling
+    bool profile = false;
+    std::vector<EventPair> pending;
+    std::vector<hipEvent_t> event_pool;
+    double prof_ms[6] = {0, 0, 0, 0, 0, 0};
+    long long prof_n[6] = {0, 0, 0, 0, 0, 0};
+};
+
+namespace {
+
+fb::SwarmBuffers buffers(fb_handle *h, int S, const double *theta, double *ll, int *info,
+                         size_t k_offset_matrices)
+{
+    fb::SwarmBuffers b;
+    b.n = h->n; b.d = h->d; b.np = h->np; b.nrhs = h->nrhs; b.likelihood = h->likelihood;
+    b.S = S;
+    b.n_slots = h->n_slots; b.n_leaves = h->prog.n_leaves;
+    b.slots = h->d_slots; b.RHS = h->d_rhs; b.hyp = h->d_hyp;
+    b.theta = theta; b.prog = h->d_prog;
+    b.K = h->d_K + k_offset_matrices * (size_t)h->np * h->np;
+    b.info = info; b.ll = ll;
+    b.logdet_part = h->d_logdet; b.inv_diag = h->d_inv;
+    b.extra = h->d_extra; b.zbuf = h->d_z;
+    return b;
+}
+
+hipEvent_t take_event(fb_handle *h)
+{
+    if (!h->event_pool.empty()) {
+        hipEvent_t e = h->event_pool.back();
+        h->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Records an event pair around `fn` when profiling.
+template <class F>
+void timed(fb_handle *h, int kind, hipStream_t st, F fn)
+{
+    if (!h->profile) { fn(); return; }
+    EventPair p{kind, take_event(h), take_event(h)};
+    if (p.a) (void)hipEventRecord(p.a, st);
+    fn();
+    if (p.b) (void)hipEventRecord(p.b, st);
+    if (p.a && p.b) h->pending.push_back(p);
+}
+
+void enqueue_pipeline(fb_handle *h, const fb::SwarmBuffers &b, hipStream_t st)
+{
+    timed(h, 0, st, [&] { fb::launch_build(b, st); });
+    for (int j = 0; j < h->nsteps; ++j) {
+        if (j > 0) timed(h, 1, st, [&] { fb::launch_gemm_step(b, j, st); });
+        timed(h, 2, st, [&] { fb::launch_potrf_diag(b, j, st); });
+        if (j + 1 < h->nsteps) timed(h, 3, st, [&] { fb::launch_trsm_step(b, j, st); });
+    }
+    timed(h, 4, st, [&] { fb::launch_likelihood(b, st); });
+}
+
+int drain_profile(fb_handle *h)
+{
+    if (h->pending.empty()) return FB_OK;
+    FB_HIP(hipStreamSynchronize(h->stream));
+    for (const EventPair &p : h->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            h->prof_ms[p.kind] += ms;
+            h->prof_n[p.kind] += 1;
+        }
+        h->event_pool.push_back(p.a);
+        h->event_pool.push_back(p.b);
+    }
+    h->pending.clear();
+    return FB_OK;
+}
+
+int check_gfx950(int *dev)
+{
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count < 1)
+        return fail(FB_E_NOGPU, "no HIP device visible");
+    FB_HIP(hipGetDevice(dev));
+    hipDeviceProp_t prop;
+    FB_HIP(hipGetDeviceProperties(&prop, *dev));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(FB_E_NOGPU, std::string("device is ") + prop.gcnArchName +
+                                    ", this library is built for gfx950 only");
+    return FB_OK;
+}
+
+void free_all(fb_handle *h)
+{
+    for (auto &g : h->graphs) {
+        (void)hipGraphExecDestroy(g.second.second);
+        (void)hipGraphDestroy(g.second.first);
+    }
+    h->graphs.clear();
+    for (const EventPair &p : h->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+    for (hipEvent_t e : h->event_pool) (void)hipEventDestroy(e);
+    void *dev[] = {h->d_prog, h->d_slots, h->d_rhs, h->d_theta, h->d_hyp, h->d_K, h->d_logdet,
+                   h->d_inv, h->d_ll, h->d_extra, h->d_z, h->d_alpha, h->d_info};
+    for (void *p : dev)
+        if (p) (void)hipFree(p);
+    void *host[] = {h->h_theta, h->h_ll, h->h_extra, h->h_info};
+    for (void *p : host)
+        if (p) (void)hipHostFree(p);
+    if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *fb_last_error(void) { return g_err.c_str(); }
+
+int fb_device_name(char *buf, int len)
+{
+    if (!buf || len < 1) return fail(FB_E_ARG, "fb_device_name: null buffer");
+    int dev = 0, count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count < 1) return fail(FB_E_NOGPU, "no HIP device visible");
+    FB_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    FB_HIP(hipGetDeviceProperties(&prop, dev));
+    std::snprintf(buf, (size_t)len, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+    return FB_OK;
+}
+
+int fb_create(fb_handle **out, int n, int d, const double *x, const double *y, const double *mean,
+              const fb_kernel_spec *spec, double noise, int optimise_noise, int likelihood,
+              int max_swarm)
+{
+    if (!out) return fail(FB_E_ARG, "fb_create: null handle pointer");
+    *out = nullptr;
+    if (n < 2 || d < 1 || !x || !y || max_swarm < 1)
+        return fail(FB_E_ARG, "fb_create: need n >= 2, d >= 1, x, y and max_swarm >= 1");
+    if (likelihood != FB_LIKELIHOOD_MARGINAL && likelihood != FB_LIKELIHOOD_CONCENTRATED)
+        return fail(FB_E_ARG, "fb_create: likelihood must be 0 (marginal) or 1 (concentrated)");
+    fb_handle *h = new fb_handle();
+    std::vector<int> slot_feature, slot_kind;
+    int rc = make_program(spec, d, optimise_noise, noise, h->prog, slot_feature, slot_kind);
+    if (rc) { delete h; return rc; }
+    rc = check_gfx950(&h->device);
+    if (rc) { delete h; return rc; }
+    h->n = n; h->d = d; h->likelihood = likelihood; h->max_swarm = max_swarm;
+    h->nrhs = likelihood == FB_LIKELIHOOD_CONCENTRATED ? 2 : 1;
+    h->np = fb::round_up(n + h->nrhs, fb::kNB);
+    h->nsteps = h->np / fb::kNB;
+    h->n_slots = (int)slot_feature.size();
+
+    // per-point slot values: raw feature, or sin(x/2) / cos(x/2)
+    std::vector<double> slots((size_t)std::max(h->n_slots, 1) * n, 0.0);
+    for (int s = 0; s < h->n_slots; ++s)
+        for (int i = 0; i < n; ++i) {
+            const double v = x[i + (size_t)n * slot_feature[s]];
+            slots[(size_t)s * n + i] = slot_kind[s] == 0 ? v : (slot_kind[s] == 1 ? std::sin(v / 2.0) : std::cos(v / 2.0));
+        }
+    std::vector<double> rhs((size_t)h->nrhs * n);
+    for (int i = 0; i < n; ++i) {
+        if (h->nrhs == 1) {
+            rhs[i] = y[i] - (mean ? mean[i] : 0.0);   // likelihood.f90:178
+        } else {
+            rhs[i] = y[i];
+            rhs[(size_t)n + i] = 1.0;
+        }
+    }
+
+    const size_t S = (size_t)max_swarm, np2 = (size_t)h->np * h->np;
+    const int hs = h->prog.n_par_kernel + fb::kMaxTerms + 1;
+#define FB_ALLOC(ptr, bytes)                                                              \
+    do {                                                                                  \
+        if (hipMalloc((void **)&(ptr), (bytes)) != hipSuccess) {                          \
+            free_all(h);                                                                  \
+            delete h;                                                                     \
+            return fail(FB_E_NOMEM, std::string("fb_create: hipMalloc of ") + #ptr + " failed"); \
+        }                                                                                 \
+    } while (0)
+    FB_ALLOC(h->d_prog, sizeof(KernelProgram));
+    FB_ALLOC(h->d_slots, sizeof(double) * slots.size());
+    FB_ALLOC(h->d_rhs, sizeof(double) * rhs.size());
+    FB_ALLOC(h->d_theta, sizeof(double) * std::max(h->prog.n_par, 1) * S);
+    FB_ALLOC(h->d_hyp, sizeof(double) * hs * S);
+    FB_ALLOC(h->d_K, sizeof(double) * np2 * S);
+    FB_ALLOC(h->d_logdet, sizeof(double) * h->nsteps * S);
+    FB_ALLOC(h->d_inv, sizeof(double) * fb::kNB * fb::kNB * S);
+    FB_ALLOC(h->d_ll, sizeof(double) * S);
+    FB_ALLOC(h->d_extra, sizeof(double) * 4 * S);
+    FB_ALLOC(h->d_z, sizeof(double) * n * S);
+    FB_ALLOC(h->d_alpha, sizeof(double) * n);
+    FB_ALLOC(h->d_info, sizeof(int) * S);
+#undef FB_ALLOC
+    if (hipHostMalloc((void **)&h->h_theta, sizeof(double) * std::max(h->prog.n_par, 1) * S) != hipSuccess ||
+        hipHostMalloc((void **)&h->h_ll, sizeof(double) * S) != hipSuccess ||
+        hipHostMalloc((void **)&h->h_extra, sizeof(double) * 4 * S) != hipSuccess ||
+        hipHostMalloc((void **)&h->h_info, sizeof(int) * S) != hipSuccess) {
+        free_all(h);
+        delete h;
+        return fail(FB_E_NOMEM, "fb_create: pinned host allocation failed");
+    }
+    if (hipStreamCreateWithFlags(&h->own_stream, hipStreamNonBlocking) != hipSuccess) {
+        free_all(h);
+        delete h;
+        return fail(FB_E_HIP, "fb_create: stream creation failed");
+    }
+    h->stream = h->own_stream;
+    hipError_t e = hipMemcpy(h->d_prog, &h->prog, sizeof(KernelProgram), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(h->d_slots, slots.data(), sizeof(double) * slots.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(h->d_rhs, rhs.data(), sizeof(double) * rhs.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(h->d_info, 0, sizeof(int) * S);
+    if (e != hipSuccess) {
+        free_all(h);
+        delete h;
+        return fail(FB_E_HIP, std::string("fb_create: upload failed: ") + hipGetErrorString(e));
+    }
+    *out = h;
+    return FB_OK;
+}
+
+int fb_destroy(fb_handle *h)
+{
+    if (!h) return FB_OK;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    free_all(h);
+    delete h;
+    return FB_OK;
+}
+
+int fb_nparams(const fb_handle *h, int *p)
+{
+    if (!h || !p) return fail(FB_E_ARG, "fb_nparams: null argument");
+    *p = h->prog.n_par;
+    return FB_OK;
+}
+
+int fb_set_stream(fb_handle *h, void *stream)
+{
+    if (!h) return fail(FB_E_ARG, "fb_set_stream: null handle");
+    FB_HIP(hipStreamSynchronize(h->stream));
+    h->stream = stream ? (hipStream_t)stream : h->own_stream;
+    return FB_OK;
+}
+
+int fb_swarm_loglik_async(fb_handle *h, int s, const double *d_theta, double *d_ll, int *d_info)
+{
+    if (!h || !d_theta || !d_ll || !d_info) return fail(FB_E_ARG, "fb_swarm_loglik_async: null argument");
+    if (s < 1 || s > h->max_swarm) return fail(FB_E_CAPACITY, "fb_swarm_loglik_async: swarm size out of range");
+    fb::SwarmBuffers b = buffers(h, s, d_theta, d_ll, d_info, 0);
+    enqueue_pipeline(h, b, h->stream);
+    FB_HIP(hipGetLastError());
+    return FB_OK;
+}
+
+int fb_swarm_loglik(fb_handle *h, int s, const double *theta, double *ll, int *info)
+{
+    if (!h || !theta || !ll || !info) return fail(FB_E_ARG, "fb_swarm_loglik: null argument");
+    if (s < 1 || s > h->max_swarm) return fail(FB_E_CAPACITY, "fb_swarm_loglik: swarm size out of range");
+    const int P = h->prog.n_par;
+    hipStream_t st = h->stream;
+    std::memcpy(h->h_theta, theta, sizeof(double) * P * s);
+    FB_HIP(hipMemcpyAsync(h->d_theta, h->h_theta, sizeof(double) * P * s, hipMemcpyHostToDevice, st));
+    fb::SwarmBuffers b = buffers(h, s, h->d_theta, h->d_ll, h->d_info, 0);
+    if (h->profile) {
+        timed(h, 5, st, [&] { enqueue_pipeline(h, b, st); });
+    } else {
+        auto it = h->graphs.find(s);
+        if (it == h->graphs.end()) {
+            hipGraph_t g = nullptr;
+            hipGraphExec_t ge = nullptr;
+            FB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            enqueue_pipeline(h, b, st);
+            FB_HIP(hipStreamEndCapture(st, &g));
+            FB_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+            it = h->graphs.emplace(s, std::make_pair(g, ge)).first;
+        }
+        FB_HIP(hipGraphLaunch(it->second.second, st));
+    }
+    FB_HIP(hipMemcpyAsync(h->h_ll, h->d_ll, sizeof(double) * s, hipMemcpyDeviceToHost, st));
+    FB_HIP(hipMemcpyAsync(h->h_info, h->d_info, sizeof(int) * s, hipMemcpyDeviceToHost, st));
+    FB_HIP(hipStreamSynchronize(st));
+    FB_HIP(hipGetLastError());
+    std::memcpy(ll, h->h_ll, sizeof(double) * s);
+    std::memcpy(info, h->h_info, sizeof(int) * s);
+    return h->profile ? drain_profile(h) : FB_OK;
+}
+
+int fb_particle_loglik(fb_handle *h, int s, const double *theta, double *ll, int *info)
+{
+    if (!h || !theta || !ll || !info) return fail(FB_E_ARG, "fb_particle_loglik: null argument");
+    if (s < 1) return fail(FB_E_ARG, "fb_particle_loglik: empty swarm");
+    const int P = h->prog.n_par;
+    hipStream_t st = h->stream;
+    for (int q = 0; q < s; ++q) {
+        std::memcpy(h->h_theta, theta + (size_t)P * q, sizeof(double) * P);
+        FB_HIP(hipMemcpyAsync(h->d_theta, h->h_theta, sizeof(double) * P, hipMemcpyHostToDevice, st));
+        fb::SwarmBuffers b = buffers(h, 1, h->d_theta, h->d_ll, h->d_info, 0);
+        enqueue_pipeline(h, b, st);
+        FB_HIP(hipMemcpyAsync(h->h_ll, h->d_ll, sizeof(double), hipMemcpyDeviceToHost, st));
+        FB_HIP(hipMemcpyAsync(h->h_info, h->d_info, sizeof(int), hipMemcpyDeviceToHost, st));
+        FB_HIP(hipStreamSynchronize(st));
+        FB_HIP(hipGetLastError());
+        ll[q] = h->h_ll[0];
+        info[q] = h->h_info[0];
+    }
+    return h->profile ? drain_profile(h) : FB_OK;
+}
+
+int fb_get_alpha(fb_handle *h, const double *theta, double *alpha, double *out4, int *info)
+{
+    if (!h || !theta || !alpha || !out4 || !info) return fail(FB_E_ARG, "fb_get_alpha: null argument");
+    const int P = h->prog.n_par;
+    hipStream_t st = h->stream;
+    std::memcpy(h->h_theta, theta, sizeof(double) * P);
+    FB_HIP(hipMemcpyAsync(h->d_theta, h->h_theta, sizeof(double) * P, hipMemcpyHostToDevice, st));
+    fb::SwarmBuffers b = buffers(h, 1, h->d_theta, h->d_ll, h->d_info, 0);
+    enqueue_pipeline(h, b, st);
+    FB_HIP(hipMemcpyAsync(h->h_info, h->d_info, sizeof(int), hipMemcpyDeviceToHost, st));
+    FB_HIP(hipStreamSynchronize(st));
+    *info = h->h_info[0];
+    if (*info != 0) return h->profile ? drain_profile(h) : FB_OK;
+    fb::launch_backsolve(b, 0, h->d_alpha, st);
+    FB_HIP(hipMemcpyAsync(alpha, h->d_alpha, sizeof(double) * h->n, hipMemcpyDeviceToHost, st));
+    FB_HIP(hipMemcpyAsync(h->h_extra, h->d_extra, sizeof(double) * 4, hipMemcpyDeviceToHost, st));
+    FB_HIP(hipStreamSynchronize(st));
+    FB_HIP(hipGetLastError());
+    std::memcpy(out4, h->h_extra, sizeof(double) * 4);
+    return h->profile ? drain_profile(h) : FB_OK;
+}
+
+int fb_swarm_loglik_grad(fb_handle *h, int s, const double *theta, double *ll, double *grad, int *info)
+{
+    (void)s; (void)theta; (void)ll; (void)grad; (void)info;
+    if (!h) return fail(FB_E_ARG, "fb_swarm_loglik_grad: null handle");
+    return fail(FB_E_ARG, "fb_swarm_loglik_grad: adjoint gradient not built yet");
+}
+
+int fb_profile_enable(fb_handle *h, int enable)
+{
+    if (!h) return fail(FB_E_ARG, "fb_profile_enable: null handle");
+    int rc = drain_profile(h);
+    h->profile = enable != 0;
+    return rc;
+}
+
+int fb_profile_read(fb_handle *h, int kind, double *total_ms, long long *launches)
+{
+    if (!h || kind < 0 || kind > 5 || !total_ms || !launches) return fail(FB_E_ARG, "fb_profile_read: bad argument");
+    int rc = drain_profile(h);
+    *total_ms = h->prof_ms[kind];
+    *launches = h->prof_n[kind];
+    h->prof_ms[kind] = 0.0;
+    h->prof_n[kind] = 0;
+    return rc;
+}
+
+}  // extern "C"

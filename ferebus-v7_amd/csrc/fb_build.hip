@@ -1,0 +1,237 @@
+// fb_build.hip -- covariance build for every particle of the swarm.
+//
+// Replaces, per particle, compute_R (src/model/gpr/gpr.f90:185-189) with the
+// kernel builders it calls: R_stationary (src/kernels/stationary_kernel.f90:101-122),
+// R_per (src/kernels/periodic_kernel.f90:181-206), the cyclic R_stationary
+// (src/kernels/stationary_cyclic_kernel.f90:104-127), R_interface for linear and
+// constant kernels (src/kernels/kernel_module.f90:208-219), R_sum / R_prod
+// (kernel_module.f90:309-325, 359-375) and the nugget K += nugget*I.
+//
+// One work-group owns a TB x TB tile of the lower triangle of the padded
+// matrix and SP particles.  The tile's per-point features (raw x, or
+// sin(x/2) / cos(x/2) for periodic dimensions) are staged once in LDS and
+// shared by the SP particles; the per-particle hyperparameters are read with
+// uniform (scalar) loads.  Lanes of a wave own consecutive rows, so every
+// store of a column segment is one contiguous 8*TB-byte write.
+#include "fb_common.h"
+
+namespace fb {
+
+// Per-particle hyperparameter record, written by hyper_kernel:
+//   [0, n_par_kernel)            exponent weights of stationary leaves
+//                                 rbf / rbf-cyclic: -1/2 exp(theta)
+//                                 periodic: -2 (1/exp(theta))^2
+//   [n_par_kernel, +kMaxTerms)   coefficient of each sum-of-products term
+//   [n_par_kernel + kMaxTerms]   nugget
+__host__ __device__ inline int hyper_stride(const KernelProgram &p) {
+    return p.n_par_kernel + kMaxTerms + 1;
+}
+
+__global__ void __launch_bounds__(64) hyper_kernel(const KernelProgram *__restrict__ prog,
+                                                  const double *__restrict__ theta, int S,
+                                                  double *__restrict__ hyp)
+{
+    const int s = blockIdx.x;
+    if (s >= S) return;
+    const KernelProgram &p = *prog;
+    const double *th = theta + (size_t)s * p.n_par;
+    double *h = hyp + (size_t)s * hyper_stride(p);
+    for (int l = 0; l < p.n_leaves; ++l) {
+        const int kind = p.leaf_kind[l];
+        const int off = p.leaf_par_off[l];
+        const int nd = p.leaf_ndim[l];
+        if (kind == kRbf || kind == kRbfCyclic) {
+            // rbf_kernel.f90:113-119, rbf_cyclic_kernel.f90:109-115: l = exp(theta)
+            for (int q = threadIdx.x; q < nd; q += blockDim.x) h[off + q] = -0.5 * exp(th[off + q]);
+        } else if (kind == kPeriodic) {
+            // periodic_kernel.f90:111-118: l = 1/exp(theta); l = l*l
+            for (int q = threadIdx.x; q < nd; q += blockDim.x) {
+                double r = 1.0 / exp(th[off + q]);
+                h[off + q] = -2.0 * (r * r);
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double leafval[kMaxLeaves];
+        for (int l = 0; l < kMaxLeaves; ++l) leafval[l] = 1.0;
+        for (int l = 0; l < p.n_leaves; ++l) {
+            const int kind = p.leaf_kind[l];
+            const int off = p.leaf_par_off[l];
+            if (kind == kLinear) {
+                // linear_kernel.f90:70-76, order 0: sum(1/c), c = exp(-theta)
+                double t = 0.0;
+                for (int q = 0; q < p.leaf_ndim[l]; ++q) t += 1.0 / exp(-th[off + q]);
+                leafval[l] = t;
+            } else if (kind == kConstant) {
+                leafval[l] = th[off];  // constant.f90:86-90
+            }
+        }
+        for (int t = 0; t < kMaxTerms; ++t) {
+            double c = 1.0;
+            if (t < p.n_terms)
+                for (int l = 0; l < p.n_leaves; ++l)
+                    if (((p.term_mask[t] >> l) & 1u) && (p.leaf_kind[l] == kLinear || p.leaf_kind[l] == kConstant))
+                        c *= leafval[l];
+            h[p.n_par_kernel + t] = c;
+        }
+        // gpr.f90:191-202: nugget from theta when optimise_noise
+        h[p.n_par_kernel + kMaxTerms] = p.optimise_noise ? th[p.n_par_kernel] : p.noise;
+    }
+}
+
+// Wrap of distance_matrix.f90:119-120; only its square is used, so the sign
+// factor of the reference drops out.
+__device__ inline double cyclic_wrap_abs(double diff) {
+    return fmod(fabs(diff) + kPi, kTwoPi) - kPi;
+}
+
+struct BuildArgs {
+    int n, np, nrhs, S, n_slots;
+    const double *slots;   // n_slots x n: feature value of point i at slots[slot*n + i]
+    const double *rhs;     // nrhs x n
+    const double *hyp;
+    const KernelProgram *prog;
+    double *K;
+    int *info;
+};
+
+// TB: tile order; MAXL: leaves held in registers (>= n_leaves); SP: particles
+// per work-group.
+template <int TB, int MAXL, int SP>
+__global__ void __launch_bounds__(kThreads) build_kernel(BuildArgs a)
+{
+    constexpr int CG = kThreads / TB;   // column groups
+    constexpr int CPT = TB / CG;        // columns per thread
+    extern __shared__ double lds[];
+    double *row_f = lds;                        // [n_slots][TB]
+    double *col_f = lds + (size_t)a.n_slots * TB;
+
+    // lower-triangular tile index: b = ti*(ti+1)/2 + tj, tj <= ti
+    const int b = blockIdx.x;
+    int ti = (int)((sqrt(8.0 * (double)b + 1.0) - 1.0) * 0.5);
+    while ((ti + 1) * (ti + 2) / 2 <= b) ++ti;
+    while (ti * (ti + 1) / 2 > b) --ti;
+    const int tj = b - ti * (ti + 1) / 2;
+    const int s0 = blockIdx.y * SP;
+    const int nsp = min(SP, a.S - s0);
+    const KernelProgram &p = *a.prog;
+
+    for (int e = threadIdx.x; e < a.n_slots * TB; e += kThreads) {
+        const int slot = e / TB, r = e % TB;
+        const int gi = ti * TB + r, gj = tj * TB + r;
+        row_f[e] = gi < a.n ? a.slots[(size_t)slot * a.n + gi] : 0.0;
+        col_f[e] = gj < a.n ? a.slots[(size_t)slot * a.n + gj] : 0.0;
+    }
+    if (b == 0 && threadIdx.x < nsp) a.info[s0 + threadIdx.x] = 0;
+    __syncthreads();
+
+    const int r = threadIdx.x % TB;
+    const int cg = threadIdx.x / TB;
+    const int i = ti * TB + r;
+    const int hs = hyper_stride(p);
+    const size_t mstride = (size_t)a.np * a.np;
+
+    for (int m = 0; m < CPT; ++m) {
+        const int jc = cg + CG * m;
+        const int j = tj * TB + jc;
+        if (i < j) continue;  // strictly upper part of a diagonal tile
+        double *kcol = a.K + (size_t)j * a.np + i;
+        if (i < a.n && j < a.n) {
+            double e[SP][MAXL];
+#pragma unroll
+            for (int q = 0; q < SP; ++q)
+#pragma unroll
+                for (int l = 0; l < MAXL; ++l) e[q][l] = 0.0;
+#pragma unroll
+            for (int l = 0; l < MAXL; ++l) {
+                if (l >= p.n_leaves) break;
+                const int kind = p.leaf_kind[l];
+                if (kind == kLinear || kind == kConstant) continue;
+                const int nd = p.leaf_ndim[l];
+                const int doff = p.leaf_dim_off[l];
+                const int poff = p.leaf_par_off[l];
+                for (int q = 0; q < nd; ++q) {
+                    const int code = p.dims[doff + q];
+                    const int slot = code & 0xffff;
+                    double f;
+                    if (kind == kPeriodic) {
+                        // sin((xi-xj)/2) = sin(xi/2)cos(xj/2) - cos(xi/2)sin(xj/2)
+                        const double sh_i = row_f[slot * TB + r], ch_i = row_f[(slot + 1) * TB + r];
+                        const double sh_j = col_f[slot * TB + jc], ch_j = col_f[(slot + 1) * TB + jc];
+                        const double sn = sh_i * ch_j - ch_i * sh_j;
+                        f = sn * sn;
+                    } else {
+                        double diff = row_f[slot * TB + r] - col_f[slot * TB + jc];
+                        if (code >> 16) diff = cyclic_wrap_abs(diff);
+                        f = diff * diff;
+                    }
+#pragma unroll
+                    for (int s = 0; s < SP; ++s)
+                        if (s < nsp) e[s][l] = fma(a.hyp[(size_t)(s0 + s) * hs + poff + q], f, e[s][l]);
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < SP; ++s) {
+                if (s >= nsp) break;
+                const double *h = a.hyp + (size_t)(s0 + s) * hs;
+                double v = 0.0;
+                for (int t = 0; t < p.n_terms; ++t) {
+                    const uint32_t mask = p.term_mask[t];
+                    double ex = 0.0;
+                    bool any = false;
+#pragma unroll
+                    for (int l = 0; l < MAXL; ++l)
+                        if ((mask >> l) & 1u) { ex += e[s][l]; any = true; }
+                    const double c = h[p.n_par_kernel + t];
+                    v += any ? c * exp(ex) : c;
+                }
+                if (i == j) v += h[p.n_par_kernel + kMaxTerms];
+                kcol[(size_t)(s0 + s) * mstride] = v;
+            }
+        } else {
+            // bordered rows (right-hand sides) and the identity padding block
+            double v = 0.0;
+            if (j < a.n) {
+                if (i < a.n + a.nrhs) v = a.rhs[(size_t)(i - a.n) * a.n + j];
+            } else if (i == j) {
+                v = 1.0;
+            }
+#pragma unroll
+            for (int s = 0; s < SP; ++s)
+                if (s < nsp) kcol[(size_t)(s0 + s) * mstride] = v;
+        }
+    }
+}
+
+template <int TB, int MAXL>
+static void launch_tb(const BuildArgs &a, int n_slots, hipStream_t st)
+{
+    constexpr int SP = 8;
+    const int nt = a.np / TB;
+    dim3 grid(nt * (nt + 1) / 2, ceil_div(a.S, SP));
+    const size_t shm = (size_t)2 * n_slots * TB * sizeof(double);
+    hipLaunchKernelGGL((build_kernel<TB, MAXL, SP>), grid, dim3(kThreads), shm, st, a);
+}
+
+template <int TB>
+static void launch_maxl(const BuildArgs &a, int n_leaves, hipStream_t st)
+{
+    if (n_leaves <= 1) launch_tb<TB, 1>(a, a.n_slots, st);
+    else if (n_leaves <= 2) launch_tb<TB, 2>(a, a.n_slots, st);
+    else if (n_leaves <= 4) launch_tb<TB, 4>(a, a.n_slots, st);
+    else launch_tb<TB, 8>(a, a.n_slots, st);
+}
+
+void launch_build(const SwarmBuffers &b, hipStream_t st)
+{
+    hipLaunchKernelGGL(hyper_kernel, dim3(b.S), dim3(64), 0, st, b.prog, b.theta, b.S, b.hyp);
+    BuildArgs a;
+    a.n = b.n; a.np = b.np; a.nrhs = b.nrhs; a.S = b.S; a.n_slots = b.n_slots;
+    a.slots = b.slots; a.rhs = b.RHS; a.hyp = b.hyp; a.prog = b.prog; a.K = b.K; a.info = b.info;
+    // 64 x 64 tiles while the staged features fit 64 KB of LDS, else 32 x 32
+    if ((size_t)2 * b.n_slots * 64 * sizeof(double) <= 65536) launch_maxl<64>(a, b.n_leaves, st);
+    else launch_maxl<32>(a, b.n_leaves, st);
+}
+
+}  // namespace fb

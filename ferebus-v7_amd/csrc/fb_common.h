@@ -1,0 +1,90 @@
+// fb_common.h -- shared definitions of the gfx950 swarm log-likelihood library.
+//
+// Data layout in HBM (one training set = one handle):
+//   slots    n_slots x n FP64 per-point feature values used by the build:
+//            the raw feature for rbf / rbf-cyclic dimensions, and the pair
+//            sin(x/2), cos(x/2) for periodic ones, so that
+//            sin((xi - xj)/2) = sin(xi/2)cos(xj/2) - cos(xi/2)sin(xj/2)
+//            needs no per-pair transcendental in the covariance build.
+//   RHS      nrhs x n: the right-hand sides the likelihood needs
+//            (y - m for the marginal likelihood, y and 1 for the concentrated).
+//   K        S matrices of np x np FP64, column major, ld = np, matrix s at
+//            K + s*np*np.  np = roundup(n + nrhs, NB).  Rows n .. n+nrhs-1 hold
+//            the right-hand sides ("bordered" matrix), so that the Cholesky
+//            factorisation of the leading (n+nrhs) block writes
+//            z = L^{-1} rhs into those rows; the remaining padding rows and
+//            columns form an identity block that factors to itself.
+//            Only the lower triangle is ever read or written.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fb {
+
+constexpr int kMaxLeaves = 8;     // stationary / constant leaves of the kernel tree
+constexpr int kMaxTerms = 16;     // sum-of-products terms after expansion
+constexpr int kMaxDims = 512;     // total active dimensions over all leaves
+
+// Leaf kinds on the device (same numbering as FB_LEAF_*).
+enum LeafKind : int { kRbf = 1, kPeriodic = 2, kRbfCyclic = 3, kLinear = 4, kConstant = 5 };
+
+// The kernel expression expanded into a sum of products of leaves:
+//   K_ij = sum_t  coef_t(s) * exp( sum_{leaf in t} e_leaf(i, j, s) )
+// where stationary leaves contribute exponents e_leaf and linear / constant
+// leaves contribute the factor coef_t (their values do not depend on x).
+struct KernelProgram {
+    int n_leaves;
+    int n_terms;
+    int n_par_kernel;                 // kernel parameters (without the nugget)
+    int n_par;                        // theta entries per particle
+    int optimise_noise;
+    double noise;
+    int leaf_kind[kMaxLeaves];
+    int leaf_ndim[kMaxLeaves];
+    int leaf_dim_off[kMaxLeaves];     // offset into dims[]
+    int leaf_par_off[kMaxLeaves];     // offset into theta
+    uint32_t term_mask[kMaxTerms];    // bit l set: leaf l is a factor of the term
+    int n_dims;
+    int dims[kMaxDims];               // 0-based feature indices, leaf after leaf
+};
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kTwoPi = 2.0 * kPi;
+
+// Tile sizes of the batched factorisation.
+constexpr int kNB = 64;       // panel width: diagonal block order
+constexpr int kMB = 64;       // rows of one update / solve tile
+constexpr int kKC = 16;       // depth of one LDS stage of the update GEMM
+constexpr int kThreads = 256; // work-group size of every kernel (4 wave64)
+
+inline __host__ __device__ int ceil_div(int a, int b) { return (a + b - 1) / b; }
+inline __host__ __device__ int round_up(int a, int b) { return ceil_div(a, b) * b; }
+
+// Launchers (each defined next to its kernels).  All enqueue on `stream`.
+struct SwarmBuffers {
+    int n, d, np, nrhs, likelihood;
+    int S;                 // particles of this launch
+    int n_slots, n_leaves;
+    const double *slots;   // n_slots x n
+    const double *RHS;     // nrhs x n
+    double *hyp;           // S x hyper record (see fb_build.hip)
+    const double *theta;   // n_par x S
+    const KernelProgram *prog;   // device copy
+    double *K;             // S x np x np
+    int *info;             // S
+    double *logdet_part;   // S x (np / kNB)
+    double *inv_diag;      // S x kNB x kNB
+    double *ll;            // S
+    double *extra;         // S x 4: ll, logdet, mu_hat, sigma2_hat
+    double *zbuf;          // S x n: z = L^{-1} rhs_0 (or the concentrated residual)
+};
+
+void launch_build(const SwarmBuffers &b, hipStream_t st);
+void launch_gemm_step(const SwarmBuffers &b, int step, hipStream_t st);
+void launch_potrf_diag(const SwarmBuffers &b, int step, hipStream_t st);
+void launch_trsm_step(const SwarmBuffers &b, int step, hipStream_t st);
+void launch_likelihood(const SwarmBuffers &b, hipStream_t st);
+void launch_backsolve(const SwarmBuffers &b, int particle, double *alpha, hipStream_t st);
+
+}  // namespace fb

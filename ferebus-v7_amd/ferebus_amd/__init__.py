@@ -1,0 +1,14 @@
+"""MI355X (gfx950) swarm log-likelihood evaluation for FEREBUS-v7 models.
+
+Python host mirror of the C ABI in include/ferebus_hip.h.  The arithmetic
+runs in libferebus_hip.so (hand-written HIP for gfx950); this package builds
+inputs (kernel programs, mean vectors) and calls it.
+"""
+from .kernel_expr import (KernelConfig, KernelExpressionError, KernelProgram, compile_kernel,
+                          parse)
+from .mean import MeanFit, fit_mean
+from .gpr import SwarmLikelihood
+from ._abi import FbError, device_name, library_path, load
+
+__all__ = ["KernelConfig", "KernelExpressionError", "KernelProgram", "compile_kernel", "parse",
+           "MeanFit", "fit_mean", "SwarmLikelihood", "FbError", "device_name", "library_path", "load"]

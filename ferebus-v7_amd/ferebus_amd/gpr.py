@@ -1,0 +1,174 @@
+"""Swarm log-likelihood operator over the gfx950 library.
+
+Host mirror of the reference's GaussianProcessRegressor as the PSO sees it
+(src/model/gpr/gpr.f90): construct once per training set, then evaluate the
+log-likelihood of whole swarms of theta vectors.  Every evaluation runs in
+libferebus_hip.so; nothing here computes a likelihood.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Dict, Optional, Sequence, Tuple, Union
+
+import numpy as np
+
+from . import _abi
+from .kernel_expr import KernelConfig, KernelProgram, compile_kernel
+from .mean import MeanFit, fit_mean
+
+LIKELIHOODS = {"marginal": 0, "concentrated": 1}
+PROFILE_KINDS = {"build": 0, "gemm": 1, "potrf": 2, "trsm": 3, "likelihood": 4, "swarm": 5}
+
+
+class SwarmLikelihood:
+    """Log-likelihood of every particle of a swarm for one training set.
+
+    x        (N, d) training inputs (ALF features)
+    y        (N,) training outputs
+    kernel   kernel expression (``"k1+k2"``) with ``kernels`` configs, or a
+             compiled :class:`KernelProgram`
+    mean     mean kind (``zero|constant|linear|quadratic``), a fitted
+             :class:`MeanFit`, or an explicit (N,) mean vector
+    noise    nugget (gpr.f90:185-189); with ``optimise_noise`` the nugget is
+             the last theta entry instead (gpr.f90:191-202)
+    """
+
+    def __init__(self, x: np.ndarray, y: np.ndarray, kernel: Union[str, KernelProgram],
+                 kernels: Optional[Dict[str, KernelConfig]] = None,
+                 mean: Union[str, MeanFit, np.ndarray, None] = "constant",
+                 noise: float = 1e-10, optimise_noise: bool = False,
+                 likelihood: str = "marginal", max_swarm: int = 64):
+        self._lib = _abi.load()
+        self._h = ctypes.c_void_p()
+        x = np.asarray(x, dtype=np.float64)
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        if x.ndim != 2 or y.shape != (x.shape[0],):
+            raise ValueError("x must be (N, d) and y (N,)")
+        self.n, self.d = x.shape
+        if isinstance(kernel, str):
+            kernel = compile_kernel(kernel, kernels or {}, self.d)
+        self.program = kernel
+        if isinstance(mean, str):
+            mean = fit_mean(mean, x, y)
+        if isinstance(mean, MeanFit):
+            self.mean_fit = mean
+            mvec = mean.value
+        elif mean is None:
+            self.mean_fit = None
+            mvec = None
+        else:
+            self.mean_fit = None
+            mvec = np.asarray(mean, dtype=np.float64)
+        if likelihood not in LIKELIHOODS:
+            raise ValueError(f"Unknown likelihood function: {likelihood}")
+        self.likelihood = likelihood
+        self.noise = float(noise)
+        self.optimise_noise = bool(optimise_noise)
+        self.max_swarm = int(max_swarm)
+        self._x = np.asfortranarray(x)
+        self._y = y
+        self._m = None if mvec is None else np.ascontiguousarray(mvec, dtype=np.float64)
+        nt = np.ascontiguousarray(kernel.node_type, dtype=np.int32)
+        nd = np.ascontiguousarray(kernel.node_ndim, dtype=np.int32)
+        ad = np.ascontiguousarray(kernel.active_dims if kernel.active_dims else [0], dtype=np.int32)
+        self._spec_arrays = (nt, nd, ad)
+        spec = _abi.KernelSpec(len(nt), _abi.iptr(nt), _abi.iptr(nd), _abi.iptr(ad))
+        rc = self._lib.fb_create(ctypes.byref(self._h), self.n, self.d, _abi.dptr(self._x),
+                                 _abi.dptr(self._y), _abi.dptr(self._m) if self._m is not None else None,
+                                 ctypes.byref(spec), self.noise, int(self.optimise_noise),
+                                 LIKELIHOODS[likelihood], self.max_swarm)
+        _abi.check(rc, "fb_create")
+        p = ctypes.c_int()
+        _abi.check(self._lib.fb_nparams(self._h, ctypes.byref(p)), "fb_nparams")
+        self.nparams = p.value
+
+    # -- evaluation -------------------------------------------------------
+    def _theta(self, theta) -> np.ndarray:
+        t = np.ascontiguousarray(np.atleast_2d(np.asarray(theta, dtype=np.float64)))
+        if t.shape[1] != self.nparams:
+            raise ValueError(f"theta must have {self.nparams} entries per particle, got {t.shape[1]}")
+        return t
+
+    def swarm_loglik(self, theta) -> Tuple[np.ndarray, np.ndarray]:
+        """Batched path: (ll[S], info[S]) for theta of shape (S, P)."""
+        t = self._theta(theta)
+        s = t.shape[0]
+        ll = np.empty(s)
+        info = np.empty(s, dtype=np.int32)
+        _abi.check(self._lib.fb_swarm_loglik(self._h, s, _abi.dptr(t), _abi.dptr(ll), _abi.iptr(info)),
+                   "fb_swarm_loglik")
+        return ll, info
+
+    def particle_loglik(self, theta) -> Tuple[np.ndarray, np.ndarray]:
+        """Plain path: one launch sequence per particle."""
+        t = self._theta(theta)
+        s = t.shape[0]
+        ll = np.empty(s)
+        info = np.empty(s, dtype=np.int32)
+        _abi.check(self._lib.fb_particle_loglik(self._h, s, _abi.dptr(t), _abi.dptr(ll), _abi.iptr(info)),
+                   "fb_particle_loglik")
+        return ll, info
+
+    def swarm_loglik_async(self, d_theta: int, d_ll: int, d_info: int, s: int) -> None:
+        """Device-pointer variant: enqueue without synchronising."""
+        _abi.check(self._lib.fb_swarm_loglik_async(self._h, s, ctypes.c_void_p(d_theta),
+                                                   ctypes.c_void_p(d_ll), ctypes.c_void_p(d_info)),
+                   "fb_swarm_loglik_async")
+
+    def alpha(self, theta):
+        """Weights alpha and (ll, logdet, mu_hat, sigma2_hat) for one theta."""
+        t = self._theta(theta)[0].copy()
+        alpha = np.empty(self.n)
+        out4 = np.empty(4)
+        info = ctypes.c_int()
+        _abi.check(self._lib.fb_get_alpha(self._h, _abi.dptr(t), _abi.dptr(alpha), _abi.dptr(out4),
+                                          ctypes.byref(info)), "fb_get_alpha")
+        return alpha, out4, info.value
+
+    def swarm_grad(self, theta):
+        """(ll[S], grad[S, P], info[S]) of the adjoint gradient path."""
+        t = self._theta(theta)
+        s = t.shape[0]
+        ll = np.empty(s)
+        grad = np.empty((s, self.nparams))
+        info = np.empty(s, dtype=np.int32)
+        _abi.check(self._lib.fb_swarm_loglik_grad(self._h, s, _abi.dptr(t), _abi.dptr(ll),
+                                                  _abi.dptr(grad), _abi.iptr(info)),
+                   "fb_swarm_loglik_grad")
+        return ll, grad, info
+
+    def cost_function(self, theta) -> Tuple[float, bool]:
+        """negative_log_likelihood (gpr.f90:258-289): (-ll, success)."""
+        ll, info = self.swarm_loglik(np.atleast_2d(theta))
+        return -float(ll[0]), bool(info[0] == 0)
+
+    # -- plumbing ---------------------------------------------------------
+    def set_stream(self, stream_handle: Optional[int]) -> None:
+        _abi.check(self._lib.fb_set_stream(self._h, ctypes.c_void_p(stream_handle or 0)), "fb_set_stream")
+
+    def profile(self, enable: bool) -> None:
+        _abi.check(self._lib.fb_profile_enable(self._h, int(enable)), "fb_profile_enable")
+
+    def profile_read(self, kind: str) -> Tuple[float, int]:
+        ms = ctypes.c_double()
+        n = ctypes.c_longlong()
+        _abi.check(self._lib.fb_profile_read(self._h, PROFILE_KINDS[kind], ctypes.byref(ms), ctypes.byref(n)),
+                   "fb_profile_read")
+        return ms.value, n.value
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.fb_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

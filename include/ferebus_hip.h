@@ -1,0 +1,157 @@
+/*
+ * ferebus_hip.h -- C ABI of the MI355X (gfx950) swarm log-likelihood library.
+ *
+ * One call evaluates the Gaussian-process log-likelihood of every particle of a
+ * PSO swarm: covariance build K(theta_s), Cholesky factorisation, log-det and
+ * the quadratic form, for all s at once on one GPU, in FP64.
+ *
+ * Plain C: pointers, sizes and int return codes.  No HIP or torch types appear
+ * in the signatures; streams are passed as `void*` (a hipStream_t, NULL = the
+ * library's own stream).  Host arrays are caller owned and only read during the
+ * call.  Every device buffer is owned by the handle and released by fb_destroy.
+ *
+ * Return codes: 0 = success, < 0 = error (see FB_E_*).  Per-particle Cholesky
+ * status uses LAPACK INFO semantics: info[s] = 0 on success, k > 0 when the
+ * leading minor of order k is not positive definite (pivot <= 0 or NaN).
+ * ll[s] is unspecified when info[s] != 0.
+ *
+ * Reference seams replaced (popelier-group/FEREBUS-v7, Fortran):
+ *   GaussianProcessRegressor%cost_function   src/model/gpr/gpr.f90:258-289
+ *   GaussianProcessRegressor%R (compute_R)    src/model/gpr/gpr.f90:185-189
+ *   Kernel%R and its overrides                src/kernels/kernel_module.f90:208-219,309-325,359-375
+ *   Likelihood%ll (marginal / concentrated)   src/model/gpr/likelihood.f90:161-253
+ *   potrf / potrs wrappers                    src/utils/linalg.f90:73-98,135-168
+ * called per particle from Particle%get_cost  src/optimiser/pso/particle.f90:149-162
+ * inside the OpenMP swarm loop                src/optimiser/pso/pso.f90:419-452
+ */
+#ifndef FEREBUS_HIP_H
+#define FEREBUS_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ------------------------------------------------------- */
+#define FB_OK              0
+#define FB_E_ARG          -1   /* invalid argument                          */
+#define FB_E_SPEC         -2   /* malformed kernel specification            */
+#define FB_E_HIP          -3   /* a HIP runtime call failed                 */
+#define FB_E_NOMEM        -4   /* device allocation failed                  */
+#define FB_E_NOGPU        -5   /* no usable gfx950 device                   */
+#define FB_E_CAPACITY     -6   /* swarm larger than the handle was sized for */
+
+/* ---- kernel composition ------------------------------------------------- *
+ * The kernel expression of ferebus.toml (model.kernel, e.g. "k1+k2",
+ * "k1*k2", "(k1+k2)*k3", "2*k1") is passed as a postfix program.  Leaves are
+ * listed in the order the reference's parser meets them, which is also the
+ * order of their parameters in theta (kernel_module.f90:227-245).
+ *
+ *   leaf type          parameters      hyperparameter transform (reference)
+ *   FB_LEAF_RBF        ndim            l_d = exp(theta_d)      rbf_kernel.f90:113-119
+ *   FB_LEAF_PERIODIC   ndim            l_d = exp(-2 theta_d)   periodic_kernel.f90:111-118
+ *   FB_LEAF_RBF_CYCLIC ndim (= d)      l_d = exp(theta_d)      rbf_cyclic_kernel.f90:109-115
+ *   FB_LEAF_LINEAR     ndim            c_d = exp(-theta_d)     linear_kernel.f90:94-100
+ *   FB_LEAF_CONSTANT   1               value = theta           constant.f90:86-90
+ * Operators: FB_OP_SUM (kernel_module.f90:309-325), FB_OP_PROD (:359-375).
+ * With optimise_noise the nugget is the last theta entry (gpr.f90:191-202).
+ */
+#define FB_LEAF_RBF         1
+#define FB_LEAF_PERIODIC    2
+#define FB_LEAF_RBF_CYCLIC  3
+#define FB_LEAF_LINEAR      4
+#define FB_LEAF_CONSTANT    5
+#define FB_OP_SUM          16
+#define FB_OP_PROD         17
+
+typedef struct fb_kernel_spec {
+    int        n_nodes;      /* length of the postfix program                   */
+    const int *node_type;    /* [n_nodes] FB_LEAF_* or FB_OP_*                  */
+    const int *node_ndim;    /* [n_nodes] active dimensions per leaf (0 for ops) */
+    const int *active_dims;  /* leaves' 1-based feature indices, concatenated in
+                                program order (sum of node_ndim entries)        */
+} fb_kernel_spec;
+
+#define FB_LIKELIHOOD_MARGINAL      0   /* likelihood.f90:161-201 */
+#define FB_LIKELIHOOD_CONCENTRATED  1   /* likelihood.f90:203-253 */
+
+typedef struct fb_handle fb_handle;
+
+/* Create a handle for one training set.
+ *   x      n*d FP64, column major (x[i + n*f] = feature f of point i), as the
+ *          reference stores training inputs (src/csv.f90:37-177).
+ *   y      n FP64 targets.
+ *   mean   n FP64 prior mean m(x_i) evaluated once on the host
+ *          (gpr.f90:168-171); NULL means a zero mean.  Ignored by the
+ *          concentrated likelihood, as in the reference (likelihood.f90:220-243).
+ *   noise  nugget added to the diagonal (gpr.f90:185-189) unless
+ *          optimise_noise != 0, in which case theta's last entry is used.
+ *   max_swarm  largest swarm any later call will pass (sizes the K buffers).
+ * Replaces GaussianProcessRegressor%init (gpr.f90:124-183) for the device side. */
+int fb_create(fb_handle **out, int n, int d, const double *x, const double *y,
+              const double *mean, const fb_kernel_spec *spec, double noise,
+              int optimise_noise, int likelihood, int max_swarm);
+
+/* Release every device buffer of the handle.  NULL is accepted. */
+int fb_destroy(fb_handle *h);
+
+/* Number of theta entries P per particle (GaussianProcessRegressor%nparams,
+ * gpr.f90:248-256). */
+int fb_nparams(const fb_handle *h, int *p);
+
+/* Bind the handle to a HIP stream (hipStream_t passed as void*; NULL restores
+ * the handle's own stream).  All later launches of the handle go there. */
+int fb_set_stream(fb_handle *h, void *stream);
+
+/* Swarm log-likelihood, batched path: every particle's build, factorisation and
+ * likelihood in one pass over the whole swarm.
+ *   theta  P*s FP64 host array, column s = particle s.
+ *   ll     s log-likelihoods (not negated; cost = -ll as in gpr.f90:284-288).
+ *   info   s LAPACK-style Cholesky INFO values.
+ * Replaces s calls of cost_function (gpr.f90:258-289) from pso.f90:419-452. */
+int fb_swarm_loglik(fb_handle *h, int s, const double *theta, double *ll, int *info);
+
+/* Same as fb_swarm_loglik with device-resident theta / ll / info, enqueued on
+ * the handle's stream without host synchronisation (for callers that keep the
+ * swarm on the GPU or capture the iteration in a graph). */
+int fb_swarm_loglik_async(fb_handle *h, int s, const double *d_theta,
+                          double *d_ll, int *d_info);
+
+/* Plain per-particle path: one launch sequence per particle, particles one after
+ * the other, exactly what a line-by-line offload of get_cost would do.  Same
+ * arguments and results as fb_swarm_loglik.  Kept as the comparison path. */
+int fb_particle_loglik(fb_handle *h, int s, const double *theta, double *ll, int *info);
+
+/* Model weights of one parameter vector, for the .model writer
+ * (write_gpr gpr.f90:329-398, write_likelihood likelihood.f90:255-264).
+ *   alpha  n FP64: K^{-1}(y - m)  (marginal) or K^{-1}(y - mu_hat) (concentrated)
+ *   out4   [ll, logdet, mu_hat, sigma2_hat]; mu_hat and sigma2_hat are only
+ *          meaningful for the concentrated likelihood.
+ *   info   LAPACK INFO of the factorisation. */
+int fb_get_alpha(fb_handle *h, const double *theta, double *alpha, double *out4, int *info);
+
+/* Gradient of the log-likelihood with respect to theta for s particles:
+ * d logL / d theta_k = 1/2 tr((alpha alpha^T - K^{-1}) dK/dtheta_k), evaluated
+ * without forming dK/dtheta.  The reference parses a gradient flag but never
+ * computes one (gpr.f90:272-282); this entry has no reference counterpart and
+ * serves gradient polishing of the swarm's best particle.
+ *   grad  P*s FP64, column s = gradient of particle s (of ll, not of cost). */
+int fb_swarm_loglik_grad(fb_handle *h, int s, const double *theta, double *ll,
+                         double *grad, int *info);
+
+/* Kernel timing: when enabled, each launch of the handle is bracketed by HIP
+ * events and the per-kernel totals accumulate until read.
+ * kind: 0 covariance build, 1 trailing-update GEMM, 2 diagonal potrf,
+ *       3 panel trsm, 4 likelihood reduction, 5 whole swarm evaluation. */
+int fb_profile_enable(fb_handle *h, int enable);
+int fb_profile_read(fb_handle *h, int kind, double *total_ms, long long *launches);
+
+/* Library-wide information: device name into buf (len bytes), 0 on success. */
+int fb_device_name(char *buf, int len);
+
+/* Human-readable message for the last error on this thread. */
+const char *fb_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FEREBUS_HIP_H */
