@@ -20,8 +20,9 @@ namespace fb {
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 // One 16x16x4 FP64 matrix-core product: D = A B + C where lane l supplies
-// A[l % 16][l / 16] and B[l / 16][l % 16] and holds D[4 (l / 16) + v][l % 16]
-// in component v.
+// A[l % 16][l / 16] and B[l / 16][l % 16] and holds D[l / 16 + 4 v][l % 16]
+// in component v (rows interleaved across the four 16-lane groups; checked
+// on gfx950 with a one-wave probe).
 __device__ inline d4 mfma_f64(double a, double b, d4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
@@ -142,7 +143,7 @@ __device__ inline void acc_to_lds(double *Cs, d4 (&acc)[TileShape<MB, NB>::BM][T
         for (int j = 0; j < T::BN; ++j)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int row = wrow + i * 16 + 4 * (lane >> 4) + v;
+                const int row = wrow + i * 16 + (lane >> 4) + 4 * v;
                 const int col = wcol + j * 16 + (lane & 15);
                 Cs[col * T::LDC + row] = acc[i][j][v];
             }

@@ -1,0 +1,149 @@
+!> ISO_C_BINDING interface to libferebus_hip.so (include/ferebus_hip.h).
+!>
+!> The Fortran host side (model, optimiser) calls the gfx950 library through
+!> these interfaces only: plain pointers, sizes and integer status codes.
+module ferebus_hip
+  use iso_c_binding, only: c_int, c_double, c_ptr, c_null_ptr, c_char, c_long_long, &
+                           c_associated, c_loc, c_f_pointer
+  implicit none
+  private
+
+  integer(c_int), parameter, public :: FB_OK = 0
+  integer(c_int), parameter, public :: FB_E_ARG = -1, FB_E_SPEC = -2, FB_E_HIP = -3, &
+                                       FB_E_NOMEM = -4, FB_E_NOGPU = -5, FB_E_CAPACITY = -6
+  integer(c_int), parameter, public :: FB_LEAF_RBF = 1, FB_LEAF_PERIODIC = 2, &
+                                       FB_LEAF_RBF_CYCLIC = 3, FB_LEAF_LINEAR = 4, &
+                                       FB_LEAF_CONSTANT = 5, FB_OP_SUM = 16, FB_OP_PROD = 17
+  integer(c_int), parameter, public :: FB_LIKELIHOOD_MARGINAL = 0, FB_LIKELIHOOD_CONCENTRATED = 1
+
+  !> Postfix kernel program (fb_kernel_spec).
+  type, bind(c), public :: fb_kernel_spec
+    integer(c_int) :: n_nodes
+    type(c_ptr) :: node_type
+    type(c_ptr) :: node_ndim
+    type(c_ptr) :: active_dims
+  end type fb_kernel_spec
+
+  public :: fb_create, fb_destroy, fb_nparams, fb_set_stream, fb_swarm_loglik, &
+            fb_swarm_loglik_async, fb_particle_loglik, fb_get_alpha, fb_swarm_loglik_grad, &
+            fb_profile_enable, fb_profile_read, fb_device_name, fb_last_error, fb_error_message
+
+  interface
+    integer(c_int) function fb_create(h, n, d, x, y, mean, spec, noise, optimise_noise, &
+                                      likelihood, max_swarm) bind(c, name="fb_create")
+      import :: c_int, c_double, c_ptr, fb_kernel_spec
+      type(c_ptr), intent(out) :: h
+      integer(c_int), value :: n, d
+      real(c_double), intent(in) :: x(*), y(*)
+      type(c_ptr), value :: mean
+      type(fb_kernel_spec), intent(in) :: spec
+      real(c_double), value :: noise
+      integer(c_int), value :: optimise_noise, likelihood, max_swarm
+    end function fb_create
+
+    integer(c_int) function fb_destroy(h) bind(c, name="fb_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function fb_destroy
+
+    integer(c_int) function fb_nparams(h, p) bind(c, name="fb_nparams")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), intent(out) :: p
+    end function fb_nparams
+
+    integer(c_int) function fb_set_stream(h, stream) bind(c, name="fb_set_stream")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h, stream
+    end function fb_set_stream
+
+    integer(c_int) function fb_swarm_loglik(h, s, theta, ll, info) bind(c, name="fb_swarm_loglik")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: s
+      real(c_double), intent(in) :: theta(*)
+      real(c_double), intent(out) :: ll(*)
+      integer(c_int), intent(out) :: info(*)
+    end function fb_swarm_loglik
+
+    integer(c_int) function fb_swarm_loglik_async(h, s, d_theta, d_ll, d_info) &
+        bind(c, name="fb_swarm_loglik_async")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: s
+      type(c_ptr), value :: d_theta, d_ll, d_info
+    end function fb_swarm_loglik_async
+
+    integer(c_int) function fb_particle_loglik(h, s, theta, ll, info) bind(c, name="fb_particle_loglik")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: s
+      real(c_double), intent(in) :: theta(*)
+      real(c_double), intent(out) :: ll(*)
+      integer(c_int), intent(out) :: info(*)
+    end function fb_particle_loglik
+
+    integer(c_int) function fb_get_alpha(h, theta, alpha, out4, info) bind(c, name="fb_get_alpha")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: theta(*)
+      real(c_double), intent(out) :: alpha(*), out4(4)
+      integer(c_int), intent(out) :: info
+    end function fb_get_alpha
+
+    integer(c_int) function fb_swarm_loglik_grad(h, s, theta, ll, grad, info) &
+        bind(c, name="fb_swarm_loglik_grad")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: s
+      real(c_double), intent(in) :: theta(*)
+      real(c_double), intent(out) :: ll(*), grad(*)
+      integer(c_int), intent(out) :: info(*)
+    end function fb_swarm_loglik_grad
+
+    integer(c_int) function fb_profile_enable(h, enable) bind(c, name="fb_profile_enable")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: enable
+    end function fb_profile_enable
+
+    integer(c_int) function fb_profile_read(h, kind, total_ms, launches) bind(c, name="fb_profile_read")
+      import :: c_int, c_double, c_ptr, c_long_long
+      type(c_ptr), value :: h
+      integer(c_int), value :: kind
+      real(c_double), intent(out) :: total_ms
+      integer(c_long_long), intent(out) :: launches
+    end function fb_profile_read
+
+    integer(c_int) function fb_device_name(buf, len) bind(c, name="fb_device_name")
+      import :: c_int, c_char
+      character(kind=c_char), intent(out) :: buf(*)
+      integer(c_int), value :: len
+    end function fb_device_name
+
+    type(c_ptr) function fb_last_error() bind(c, name="fb_last_error")
+      import :: c_ptr
+    end function fb_last_error
+  end interface
+
+contains
+
+  !> Last error message of the library as a Fortran string.
+  function fb_error_message() result(msg)
+    character(len=:), allocatable :: msg
+    character(kind=c_char), dimension(:), pointer :: chars
+    type(c_ptr) :: p
+    integer :: n
+    p = fb_last_error()
+    msg = ""
+    if (.not. c_associated(p)) return
+    call c_f_pointer(p, chars, [1024])
+    n = 0
+    do while (n < 1024)
+      if (chars(n + 1) == achar(0)) exit
+      n = n + 1
+    end do
+    allocate(character(len=n) :: msg)
+    msg = transfer(chars(1:n), msg)
+  end function fb_error_message
+end module ferebus_hip
