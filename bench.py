@@ -27,7 +27,7 @@ REPO = Path(__file__).resolve().parent
 sys.path.insert(0, str(REPO / "ferebus-v7_amd"))
 sys.path.insert(0, str(REPO))
 
-NB = 64  # panel width of the factorisation (fb_common.h kNB)
+NBO, NBI = 128, 64  # panel width / diagonal block of the factorisation (fb_common.h)
 
 
 def potrf_flops(n: int) -> float:
@@ -96,12 +96,14 @@ def main():
         gp.swarm_loglik(thetas[k % 4])
     prof = {k: gp.profile_read(k) for k in ("build", "gemm", "potrf", "trsm", "likelihood", "swarm")}
     gp.profile(False)
-    n, np_ = args.n, ((args.n + 1 + NB - 1) // NB) * NB
+    n, np_ = args.n, ((args.n + 1 + NBO - 1) // NBO) * NBO
     fact_ms = (prof["gemm"][0] + prof["potrf"][0] + prof["trsm"][0]) / prof_steps
     fact_flops = S * potrf_flops(n)
     gemm_ms = prof["gemm"][0] / prof_steps
-    # trailing-update flops actually issued: step j updates (np - j*NB) rows x NB cols, depth j*NB
-    gemm_flops = S * sum(2.0 * (np_ - j * NB) * NB * (j * NB) for j in range(1, np_ // NB))
+    # update flops actually issued: panel c0 updates (np - c0) rows x 128 cols at depth c0,
+    # then (np - c0 - 64) rows x 64 cols at depth 64
+    gemm_flops = S * sum(2.0 * (np_ - c0) * NBO * c0 + 2.0 * (np_ - c0 - NBI) * NBI * NBI
+                         for c0 in range(0, np_, NBO))
     build_ms = prof["build"][0] / prof_steps
     build_bytes = S * 8.0 * n * (n + 1) / 2.0 + 8.0 * n * d
     achieved = fact_flops / (fact_ms * 1e-3) / 1e12

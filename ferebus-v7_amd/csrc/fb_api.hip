@@ -3,7 +3,8 @@
 // Host side of the drop-in boundary: validates the kernel specification,
 // expands it into the device KernelProgram, owns every device buffer of a
 // training set, and enqueues the swarm pipeline
-//   hyper + build -> { gemm_step, potrf_diag, trsm_step } x nsteps -> likelihood
+//   hyper + build -> per 128-wide panel { gemm_update, potrf64, trsm64,
+//   gemm_update, potrf64, trsm64 } -> likelihood
 // on one HIP stream.  The batched path replays a captured hipGraph of that
 // pipeline; the plain path launches it eagerly, one particle at a time.
 #include <cmath>
@@ -155,7 +156,7 @@ struct fb_handle {
     // device buffers
     KernelProgram *d_prog = nullptr;
     double *d_slots = nullptr, *d_rhs = nullptr, *d_theta = nullptr, *d_hyp = nullptr;
-    double *d_K = nullptr, *d_logdet = nullptr, *d_inv = nullptr, *d_ll = nullptr;
+    double *d_K = nullptr, *d_logdet = nullptr, *d_ll = nullptr;
     double *d_extra = nullptr, *d_z = nullptr, *d_alpha = nullptr;
     int *d_info = nullptr;
     // pinned staging
@@ -184,7 +185,7 @@ fb::SwarmBuffers buffers(fb_handle *h, int S, const double *theta, double *ll, i
     b.theta = theta; b.prog = h->d_prog;
     b.K = h->d_K + k_offset_matrices * (size_t)h->np * h->np;
     b.info = info; b.ll = ll;
-    b.logdet_part = h->d_logdet; b.inv_diag = h->d_inv;
+    b.logdet_part = h->d_logdet;
     b.extra = h->d_extra; b.zbuf = h->d_z;
     return b;
 }
@@ -216,10 +217,14 @@ void timed(fb_handle *h, int kind, hipStream_t st, F fn)
 void enqueue_pipeline(fb_handle *h, const fb::SwarmBuffers &b, hipStream_t st)
 {
     timed(h, 0, st, [&] { fb::launch_build(b, st); });
-    for (int j = 0; j < h->nsteps; ++j) {
-        if (j > 0) timed(h, 1, st, [&] { fb::launch_gemm_step(b, j, st); });
-        timed(h, 2, st, [&] { fb::launch_potrf_diag(b, j, st); });
-        if (j + 1 < h->nsteps) timed(h, 3, st, [&] { fb::launch_trsm_step(b, j, st); });
+    for (int c0 = 0; c0 < h->np; c0 += fb::kNBO) {
+        const int c1 = c0 + fb::kNBI;
+        if (c0 > 0) timed(h, 1, st, [&] { fb::launch_gemm_update(b, c0, fb::kNBO, c0, 0, c0, st); });
+        timed(h, 2, st, [&] { fb::launch_potrf64(b, c0, st); });
+        timed(h, 3, st, [&] { fb::launch_trsm64(b, c0, c1, st); });
+        timed(h, 1, st, [&] { fb::launch_gemm_update(b, c1, fb::kNBI, c1, c0, c1, st); });
+        timed(h, 2, st, [&] { fb::launch_potrf64(b, c1, st); });
+        if (c0 + fb::kNBO < h->np) timed(h, 3, st, [&] { fb::launch_trsm64(b, c1, c0 + fb::kNBO, st); });
     }
     timed(h, 4, st, [&] { fb::launch_likelihood(b, st); });
 }
@@ -265,7 +270,7 @@ void free_all(fb_handle *h)
     for (const EventPair &p : h->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
     for (hipEvent_t e : h->event_pool) (void)hipEventDestroy(e);
     void *dev[] = {h->d_prog, h->d_slots, h->d_rhs, h->d_theta, h->d_hyp, h->d_K, h->d_logdet,
-                   h->d_inv, h->d_ll, h->d_extra, h->d_z, h->d_alpha, h->d_info};
+                   h->d_ll, h->d_extra, h->d_z, h->d_alpha, h->d_info};
     for (void *p : dev)
         if (p) (void)hipFree(p);
     void *host[] = {h->h_theta, h->h_ll, h->h_extra, h->h_info};
@@ -310,8 +315,8 @@ int fb_create(fb_handle **out, int n, int d, const double *x, const double *y, c
     if (rc) { delete h; return rc; }
     h->n = n; h->d = d; h->likelihood = likelihood; h->max_swarm = max_swarm;
     h->nrhs = likelihood == FB_LIKELIHOOD_CONCENTRATED ? 2 : 1;
-    h->np = fb::round_up(n + h->nrhs, fb::kNB);
-    h->nsteps = h->np / fb::kNB;
+    h->np = fb::round_up(n + h->nrhs, fb::kNBO);
+    h->nsteps = h->np / fb::kNBI;
     h->n_slots = (int)slot_feature.size();
 
     // per-point slot values: raw feature, or sin(x/2) / cos(x/2)
@@ -348,7 +353,6 @@ int fb_create(fb_handle **out, int n, int d, const double *x, const double *y, c
     FB_ALLOC(h->d_hyp, sizeof(double) * hs * S);
     FB_ALLOC(h->d_K, sizeof(double) * np2 * S);
     FB_ALLOC(h->d_logdet, sizeof(double) * h->nsteps * S);
-    FB_ALLOC(h->d_inv, sizeof(double) * fb::kNB * fb::kNB * S);
     FB_ALLOC(h->d_ll, sizeof(double) * S);
     FB_ALLOC(h->d_extra, sizeof(double) * 4 * S);
     FB_ALLOC(h->d_z, sizeof(double) * n * S);

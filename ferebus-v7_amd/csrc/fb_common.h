@@ -9,7 +9,7 @@
 //   RHS      nrhs x n: the right-hand sides the likelihood needs
 //            (y - m for the marginal likelihood, y and 1 for the concentrated).
 //   K        S matrices of np x np FP64, column major, ld = np, matrix s at
-//            K + s*np*np.  np = roundup(n + nrhs, NB).  Rows n .. n+nrhs-1 hold
+//            K + s*np*np.  np = roundup(n + nrhs, 128).  Rows n .. n+nrhs-1 hold
 //            the right-hand sides ("bordered" matrix), so that the Cholesky
 //            factorisation of the leading (n+nrhs) block writes
 //            z = L^{-1} rhs into those rows; the remaining padding rows and
@@ -53,8 +53,9 @@ constexpr double kPi = 3.14159265358979323846;
 constexpr double kTwoPi = 2.0 * kPi;
 
 // Tile sizes of the batched factorisation.
-constexpr int kNB = 64;       // panel width: diagonal block order
-constexpr int kMB = 64;       // rows of one update / solve tile
+constexpr int kNBO = 128;     // panel width of the trailing update
+constexpr int kNBI = 64;      // diagonal block order (potrf / trsm)
+constexpr int kMB = 64;       // rows of one update tile
 constexpr int kKC = 16;       // depth of one LDS stage of the update GEMM
 constexpr int kThreads = 256; // work-group size of every kernel (4 wave64)
 
@@ -73,17 +74,17 @@ struct SwarmBuffers {
     const KernelProgram *prog;   // device copy
     double *K;             // S x np x np
     int *info;             // S
-    double *logdet_part;   // S x (np / kNB)
-    double *inv_diag;      // S x kNB x kNB
+    double *logdet_part;   // S x (np / kNBI)
     double *ll;            // S
     double *extra;         // S x 4: ll, logdet, mu_hat, sigma2_hat
     double *zbuf;          // S x n: z = L^{-1} rhs_0 (or the concentrated residual)
 };
 
 void launch_build(const SwarmBuffers &b, hipStream_t st);
-void launch_gemm_step(const SwarmBuffers &b, int step, hipStream_t st);
-void launch_potrf_diag(const SwarmBuffers &b, int step, hipStream_t st);
-void launch_trsm_step(const SwarmBuffers &b, int step, hipStream_t st);
+void launch_gemm_update(const SwarmBuffers &b, int col0, int width, int row_start, int k0, int k1,
+                        hipStream_t st);
+void launch_potrf64(const SwarmBuffers &b, int c0, hipStream_t st);
+void launch_trsm64(const SwarmBuffers &b, int c0, int row_start, hipStream_t st);
 void launch_likelihood(const SwarmBuffers &b, hipStream_t st);
 void launch_backsolve(const SwarmBuffers &b, int particle, double *alpha, hipStream_t st);
 

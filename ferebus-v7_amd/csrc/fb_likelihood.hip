@@ -88,7 +88,7 @@ __global__ void __launch_bounds__(kThreads) likelihood_kernel(const double *__re
 void launch_likelihood(const SwarmBuffers &b, hipStream_t st)
 {
     hipLaunchKernelGGL(likelihood_kernel, dim3(b.S), dim3(kThreads), 0, st, b.K, b.np, b.n,
-                       b.np / kNB, b.likelihood, b.info, b.logdet_part, b.ll, b.extra, b.zbuf);
+                       b.np / kNBI, b.likelihood, b.info, b.logdet_part, b.ll, b.extra, b.zbuf);
 }
 
 // alpha = L^{-T} w for one particle (weights of the model file), by blocks of

@@ -4,14 +4,17 @@
 // src/model/gpr/likelihood.f90:145) for all particles at once, with LAPACK's
 // INFO convention, plus the log-determinant of src/utils/arith.f90:72-93.
 //
-// Left-looking blocked algorithm over block columns of width kNB.  Step j:
-//   gemm_step   C(i, j) = A(i, j) - L(i, 0:j) L(j, 0:j)^T for every row tile
-//               i >= j of every live matrix (FP64 MFMA 16x16x4, LDS-staged,
-//               double-buffered); skipped at j = 0.
-//   potrf_diag  factor the diagonal block in LDS (INFO, partial log-det) and
-//               form its explicit inverse for the panel solve.
-//   trsm_step   L(i, j) = C(i, j) L(j, j)^{-T} for the row tiles below, as one
-//               more MFMA tile product with the inverse.
+// Left-looking blocked algorithm with two block levels.  Panels of width
+// kNBO = 128 carry the dominant trailing update; inside a panel two blocks of
+// kNBI = 64 are factored.  For the panel at column c0:
+//   gemm_update  C(c0:, c0:c0+128) -= L(c0:, 0:c0) L(c0:c0+128, 0:c0)^T
+//                (FP64 MFMA 16x16x4, LDS-staged, double-buffered; the wide
+//                panel halves how often each row panel is re-read from HBM)
+//   potrf64      factor the 64x64 block at c0 in LDS (INFO, partial log-det)
+//   trsm64       L(c0+64:, c0:c0+64) = C L(c0, c0)^{-T} by forward
+//                substitution (backward stable, no explicit inverse)
+//   gemm_update  C(c0+64:, c0+64:c0+128) -= L(c0+64:, c0:c0+64) L(...)^T
+//   potrf64, trsm64 for the block at c0 + 64.
 // The whole swarm advances in lock step: every launch covers all particles.
 #include "fb_common.h"
 
@@ -48,22 +51,25 @@ struct TileShape {
 };
 
 // acc = sum_k A[r][k] B[c][k] over k < depth, A[r][k] = Ag[k*lda + r],
-// B[c][k] = Bg[k*ldb + c].  depth is a multiple of kKC.
+// B[c][k] = Bg[k*ldb + c].  depth is a multiple of kKC.  LDS holds two
+// stages [A0 A1 B0 B1]; the next stage is fetched into registers while the
+// current one feeds the matrix cores.
 template <int MB, int NB>
 __device__ inline void tile_gemm(const double *__restrict__ Ag, int lda,
                                  const double *__restrict__ Bg, int ldb, int depth,
-                                 double *smem, d4 (&acc)[TileShape<MB, NB>::BM][TileShape<MB, NB>::BN])
+                                 double *__restrict__ smem,
+                                 d4 (&acc)[TileShape<MB, NB>::BM][TileShape<MB, NB>::BN])
 {
     using T = TileShape<MB, NB>;
     constexpr int NA = MB * kKC / 2 / kThreads;   // double2 loads per thread (A)
     constexpr int NBL = NB * kKC / 2 / kThreads;  // double2 loads per thread (B)
+    constexpr int ASZ = kKC * T::LDA;
+    constexpr int BSZ = kKC * T::LDB;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
     const int wrow = (wave / T::WAVES_N) * T::WM;
     const int wcol = (wave % T::WAVES_N) * T::WN;
-    double *As[2] = {smem, smem + kKC * T::LDA};
-    double *Bs[2] = {smem + 2 * kKC * T::LDA, smem + 2 * kKC * T::LDA + kKC * T::LDB};
 
 #pragma unroll
     for (int i = 0; i < T::BM; ++i)
@@ -71,46 +77,61 @@ __device__ inline void tile_gemm(const double *__restrict__ Ag, int lda,
         for (int j = 0; j < T::BN; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
     if (depth <= 0) return;
 
-    double2 ra[NA], rb[NBL];
-    auto gload = [&](int k0) {
+    // per-thread load coordinates (fixed for every stage)
+    int a_off[NA], a_lds[NA], b_off[NBL], b_lds[NBL];
 #pragma unroll
-        for (int u = 0; u < NA; ++u) {
-            const int q = tid + u * kThreads;
-            const int col = q / (MB / 2), r2 = (q % (MB / 2)) * 2;
-            ra[u] = *reinterpret_cast<const double2 *>(Ag + (size_t)(k0 + col) * lda + r2);
-        }
+    for (int u = 0; u < NA; ++u) {
+        const int q = tid + u * kThreads;
+        const int col = q / (MB / 2), r2 = (q % (MB / 2)) * 2;
+        a_off[u] = col * lda + r2;
+        a_lds[u] = col * T::LDA + r2;
+    }
 #pragma unroll
-        for (int u = 0; u < NBL; ++u) {
-            const int q = tid + u * kThreads;
-            const int col = q / (NB / 2), r2 = (q % (NB / 2)) * 2;
-            rb[u] = *reinterpret_cast<const double2 *>(Bg + (size_t)(k0 + col) * ldb + r2);
-        }
-    };
-    auto sstore = [&](int buf) {
+    for (int u = 0; u < NBL; ++u) {
+        const int q = tid + u * kThreads;
+        const int col = q / (NB / 2), r2 = (q % (NB / 2)) * 2;
+        b_off[u] = col * ldb + r2;
+        b_lds[u] = col * T::LDB + r2;
+    }
+    // staging registers as scalar pairs: arrays of double2 are not promoted
+    // to registers by the compiler and would live in scratch
+    double rax[NA], ray[NA], rbx[NBL], rby[NBL];
 #pragma unroll
-        for (int u = 0; u < NA; ++u) {
-            const int q = tid + u * kThreads;
-            const int col = q / (MB / 2), r2 = (q % (MB / 2)) * 2;
-            *reinterpret_cast<double2 *>(As[buf] + col * T::LDA + r2) = ra[u];
-        }
+    for (int u = 0; u < NA; ++u) {
+        const double2 t = *reinterpret_cast<const double2 *>(Ag + a_off[u]);
+        rax[u] = t.x; ray[u] = t.y;
+    }
 #pragma unroll
-        for (int u = 0; u < NBL; ++u) {
-            const int q = tid + u * kThreads;
-            const int col = q / (NB / 2), r2 = (q % (NB / 2)) * 2;
-            *reinterpret_cast<double2 *>(Bs[buf] + col * T::LDB + r2) = rb[u];
-        }
-    };
-
-    const int nchunks = depth / kKC;
-    gload(0);
-    sstore(0);
+    for (int u = 0; u < NBL; ++u) {
+        const double2 t = *reinterpret_cast<const double2 *>(Bg + b_off[u]);
+        rbx[u] = t.x; rby[u] = t.y;
+    }
+#pragma unroll
+    for (int u = 0; u < NA; ++u) *reinterpret_cast<double2 *>(smem + a_lds[u]) = double2{rax[u], ray[u]};
+#pragma unroll
+    for (int u = 0; u < NBL; ++u) *reinterpret_cast<double2 *>(smem + 2 * ASZ + b_lds[u]) = double2{rbx[u], rby[u]};
     __syncthreads();
+
     const int kq = lane >> 4, lr = lane & 15;
+    const int nchunks = depth / kKC;
     for (int c = 0; c < nchunks; ++c) {
-        const int buf = c & 1;
-        if (c + 1 < nchunks) gload((c + 1) * kKC);
-        const double *as = As[buf];
-        const double *bs = Bs[buf];
+        const int cur = c & 1;
+        const bool more = c + 1 < nchunks;
+        if (more) {
+            const size_t ka = (size_t)(c + 1) * kKC * lda, kb = (size_t)(c + 1) * kKC * ldb;
+#pragma unroll
+            for (int u = 0; u < NA; ++u) {
+                const double2 t = *reinterpret_cast<const double2 *>(Ag + ka + a_off[u]);
+                rax[u] = t.x; ray[u] = t.y;
+            }
+#pragma unroll
+            for (int u = 0; u < NBL; ++u) {
+                const double2 t = *reinterpret_cast<const double2 *>(Bg + kb + b_off[u]);
+                rbx[u] = t.x; rby[u] = t.y;
+            }
+        }
+        const double *as = smem + cur * ASZ;
+        const double *bs = smem + 2 * ASZ + cur * BSZ;
 #pragma unroll
         for (int kk = 0; kk < kKC / 4; ++kk) {
             double af[T::BM], bf[T::BN];
@@ -123,7 +144,14 @@ __device__ inline void tile_gemm(const double *__restrict__ Ag, int lda,
 #pragma unroll
                 for (int j = 0; j < T::BN; ++j) acc[i][j] = mfma_f64(af[i], bf[j], acc[i][j]);
         }
-        if (c + 1 < nchunks) sstore(buf ^ 1);
+        if (more) {
+            double *an = smem + (cur ^ 1) * ASZ;
+            double *bn = smem + 2 * ASZ + (cur ^ 1) * BSZ;
+#pragma unroll
+            for (int u = 0; u < NA; ++u) *reinterpret_cast<double2 *>(an + a_lds[u]) = double2{rax[u], ray[u]};
+#pragma unroll
+            for (int u = 0; u < NBL; ++u) *reinterpret_cast<double2 *>(bn + b_lds[u]) = double2{rbx[u], rby[u]};
+        }
         __syncthreads();
     }
 }
@@ -151,21 +179,23 @@ __device__ inline void acc_to_lds(double *Cs, d4 (&acc)[TileShape<MB, NB>::BM][T
 
 // ---- step kernels ---------------------------------------------------------
 
+// C(rows >= row_start, col0:col0+NB) -= L(rows, k0:k1) L(col0:col0+NB, k0:k1)^T
+// for every live matrix; one work-group per MB-row tile.
 template <int MB, int NB>
-__global__ void __launch_bounds__(kThreads) gemm_step_kernel(double *__restrict__ K, int np, int step,
-                                                             const int *__restrict__ info)
+__global__ void __launch_bounds__(kThreads) gemm_update_kernel(double *__restrict__ K, int np, int col0,
+                                                               int row_start, int k0, int k1,
+                                                               const int *__restrict__ info)
 {
     using T = TileShape<MB, NB>;
     __shared__ __attribute__((aligned(16))) double smem[T::SMEM];
-    const int rt_per = (np - step * NB) / MB;
+    const int rt_per = (np - row_start) / MB;
     const int s = blockIdx.x / rt_per;
     const int rt = blockIdx.x % rt_per;
     if (info[s] != 0) return;
-    const int row0 = step * NB + rt * MB;
-    const int col0 = step * NB;
+    const int row0 = row_start + rt * MB;
     double *A = K + (size_t)s * np * np;
     d4 acc[T::BM][T::BN];
-    tile_gemm<MB, NB>(A + row0, np, A + col0, np, step * NB, smem, acc);
+    tile_gemm<MB, NB>(A + (size_t)k0 * np + row0, np, A + (size_t)k0 * np + col0, np, k1 - k0, smem, acc);
     acc_to_lds<MB, NB>(smem, acc);
     __syncthreads();
     double *Ct = A + (size_t)col0 * np + row0;
@@ -180,137 +210,160 @@ __global__ void __launch_bounds__(kThreads) gemm_step_kernel(double *__restrict_
     }
 }
 
-template <int MB, int NB>
-__global__ void __launch_bounds__(kThreads) trsm_step_kernel(double *__restrict__ K, int np, int step,
-                                                             const int *__restrict__ info,
-                                                             const double *__restrict__ inv_diag)
+// L(rows >= row_start, c0:c0+64) = C L(c0:c0+64, c0:c0+64)^{-T}.
+// One wave per 64-row tile (four tiles of one matrix per work-group): lane r
+// keeps row r of the tile in registers and solves x L^T = c right-looking,
+// x_c = c_c / L(c,c) then c_k -= x_c L(k,c) for k > c, reading L from LDS as
+// broadcasts; no barrier after the loads.
+__global__ void __launch_bounds__(kThreads) trsm64_kernel(double *__restrict__ K, int np, int c0,
+                                                          int row_start, int groups,
+                                                          const int *__restrict__ info)
 {
-    using T = TileShape<MB, NB>;
-    __shared__ __attribute__((aligned(16))) double smem[T::SMEM];
-    const int rt_per = (np - (step + 1) * NB) / MB;
-    const int s = blockIdx.x / rt_per;
-    const int rt = blockIdx.x % rt_per;
+    constexpr int B = kNBI;
+    __shared__ double Lc[B * B];      // Lc[c * B + k] = L(k, c) (column c of L)
+    __shared__ double rdiag[B];
+    const int s = blockIdx.x / groups;
+    const int grp = blockIdx.x % groups;
     if (info[s] != 0) return;
-    const int row0 = (step + 1) * NB + rt * MB;
-    const int col0 = step * NB;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     double *A = K + (size_t)s * np * np;
-    double *Ct = A + (size_t)col0 * np + row0;
-    // X = C inv^T:  X[r][c] = sum_k C[r][k] inv[c][k], inv[c][k] at inv[k*NB + c]
-    d4 acc[T::BM][T::BN];
-    tile_gemm<MB, NB>(Ct, np, inv_diag + (size_t)s * NB * NB, NB, NB, smem, acc);
-    acc_to_lds<MB, NB>(smem, acc);
-    __syncthreads();
-    for (int q = threadIdx.x; q < MB * NB / 2; q += kThreads) {
-        const int col = q / (MB / 2), r2 = (q % (MB / 2)) * 2;
-        *reinterpret_cast<double2 *>(Ct + (size_t)col * np + r2) =
-            *reinterpret_cast<const double2 *>(smem + col * T::LDC + r2);
+    const double *Lg = A + (size_t)c0 * np + c0;
+#pragma unroll
+    for (int u = 0; u < B * B / kThreads; ++u) {
+        const int e = tid + u * kThreads;
+        const int c = e / B, k = e % B;
+        Lc[e] = k >= c ? Lg[(size_t)c * np + k] : 0.0;
     }
+    __syncthreads();
+    if (tid < B) rdiag[tid] = 1.0 / Lc[tid * B + tid];
+    const int ntiles = (np - row_start) / B;
+    const int rt = grp * 4 + wave;
+    __syncthreads();
+    if (rt >= ntiles) return;
+    double *Ct = A + (size_t)c0 * np + row_start + (size_t)rt * B + lane;
+    double x[B];
+#pragma unroll
+    for (int k = 0; k < B; ++k) x[k] = Ct[(size_t)k * np];
+#pragma unroll
+    for (int c = 0; c < B; ++c) {
+        x[c] *= rdiag[c];
+#pragma unroll
+        for (int k = c + 1; k < B; ++k) x[k] -= x[c] * Lc[c * B + k];
+    }
+#pragma unroll
+    for (int k = 0; k < B; ++k) Ct[(size_t)k * np] = x[k];
 }
 
-// Factor the diagonal block of step `step` of every live matrix in LDS.
-// Columns at or beyond n (bordered right-hand-side rows and padding) are not
-// factored: they are set to the identity, which leaves the right-hand-side
-// rows holding L^{-1} rhs for the real columns.
-__global__ void __launch_bounds__(kThreads) potrf_diag_kernel(double *__restrict__ K, int np, int n,
-                                                              int step, int nsteps,
-                                                              int *__restrict__ info,
-                                                              double *__restrict__ logdet_part,
-                                                              double *__restrict__ inv_diag)
+// Factor the 64x64 diagonal block at c0 of every live matrix.  Thread
+// (lane i, wave w) keeps row i of columns w, w+4, ..., w+60 in registers;
+// the wave owning column c takes the pivot (LAPACK dpotf2 test), scales the
+// column and publishes it through LDS, then every thread applies the rank-1
+// update to its registers: one barrier per column.  Columns at or beyond n
+// (bordered right-hand-side rows and padding) are not factored: they become
+// the identity, leaving the right-hand-side rows holding L^{-1} rhs.
+__global__ void __launch_bounds__(kThreads) potrf64_kernel(double *__restrict__ K, int np, int n, int c0,
+                                                           int *__restrict__ info,
+                                                           double *__restrict__ logdet_part)
 {
-    constexpr int NB = kNB;
-    constexpr int LDP = NB + 1;
-    __shared__ double L[NB * LDP];
-    __shared__ double X[NB * LDP];
+    constexpr int B = kNBI;
+    constexpr int M = B / 4;
+    __shared__ double col[2][B];
+    __shared__ int fail_col;
     __shared__ double red[kThreads / 64];
     const int s = blockIdx.x;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     if (info[s] != 0) return;
-    const int c0 = step * NB;
-    double *Ag = K + (size_t)s * np * np + (size_t)c0 * np + c0;
-    for (int e = tid; e < NB * NB; e += kThreads) {
-        const int c = e / NB, i = e % NB;
-        L[c * LDP + i] = i >= c ? Ag[(size_t)c * np + i] : 0.0;
+    double *Ag = K + (size_t)s * np * np + (size_t)c0 * np + c0 + lane;
+    double a[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        const int k = wave + 4 * m;
+        a[m] = lane >= k ? Ag[(size_t)k * np] : 0.0;
     }
+    if (tid == 0) fail_col = 0;
+    const int ncol = min(B, n - c0);
     __syncthreads();
-    const int ncol = min(NB, n - c0);
-    int failed = 0;
-    for (int c = 0; c < ncol; ++c) {
-        const double piv = L[c * LDP + c];
-        // dpotf2: stop at the first pivot that is not positive or is NaN
-        if (!(piv > 0.0)) {
-            if (tid == 0) info[s] = c0 + c + 1;
-            failed = 1;
-            break;
-        }
-        const double d = sqrt(piv);
-        const double r = 1.0 / d;
-        for (int i = c + 1 + tid; i < NB; i += kThreads) L[c * LDP + i] *= r;
-        __syncthreads();
-        if (tid == 0) L[c * LDP + c] = d;
-        for (int k = c + 1 + wave; k < NB; k += kThreads / 64) {
-            const double lk = L[c * LDP + k];
-            for (int i = k + lane; i < NB; i += 64) L[k * LDP + i] -= L[c * LDP + i] * lk;
+    // every thread runs all B steps (the barrier is unconditional); steps at
+    // or beyond ncol, or after a failed pivot, do no work
+#pragma unroll
+    for (int c = 0; c < B; ++c) {
+        const bool live = c < ncol && fail_col == 0;
+        if (live && wave == (c & 3)) {
+            const double piv = __shfl(a[c >> 2], c, 64);
+            if (!(piv > 0.0)) {
+                // dpotf2: stop at the first pivot that is not positive or is NaN
+                if (lane == 0) fail_col = c0 + c + 1;
+            } else {
+                const double d = sqrt(piv);
+                const double l = lane > c ? a[c >> 2] * (1.0 / d) : (lane == c ? d : 0.0);
+                if (lane >= c) a[c >> 2] = l;
+                col[c & 1][lane] = l;
+            }
         }
         __syncthreads();
+        if (c < ncol && fail_col == 0) {
+            const double li = col[c & 1][lane];
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                const int k = wave + 4 * m;
+                if (k > c) a[m] -= li * col[c & 1][k];
+            }
+        }
     }
-    if (failed) return;
-    // identity on the unfactored (right-hand-side / padding) columns
-    for (int e = tid; e < NB * NB; e += kThreads) {
-        const int c = e / NB, i = e % NB;
-        if (c >= ncol && i >= c) L[c * LDP + i] = (i == c) ? 1.0 : 0.0;
+    if (fail_col != 0) {
+        if (tid == 0) info[s] = fail_col;
+        return;
     }
+    // identity on the unfactored (right-hand-side / padding) columns, and the
     // partial log-determinant of the real columns of this block
-    double lg = 0.0;
-    for (int c = tid; c < ncol; c += kThreads) lg += log(L[c * LDP + c]);
+    double dv = 1.0;
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        const int k = wave + 4 * m;
+        if (k >= ncol) a[m] = lane == k ? 1.0 : 0.0;
+        if (k == lane) dv = a[m];
+    }
+    double lg = ((lane & 3) == wave && lane < ncol) ? log(dv) : 0.0;
     for (int off = 32; off > 0; off >>= 1) lg += __shfl_xor(lg, off, 64);
     if (lane == 0) red[wave] = lg;
-    __syncthreads();
-    if (tid == 0) logdet_part[(size_t)s * nsteps + step] = (red[0] + red[1]) + (red[2] + red[3]);
-    for (int e = tid; e < NB * NB; e += kThreads) {
-        const int c = e / NB, i = e % NB;
-        if (i >= c) Ag[(size_t)c * np + i] = L[c * LDP + i];
-    }
-    // inverse of the lower-triangular block, one column per thread:
-    // X(i, t) = (delta_it - sum_{t<=k<i} L(i, k) X(k, t)) / L(i, i)
-    // The k loop is uniform across the wave so L(i, k) is a broadcast read;
-    // X(k, t) = 0 for k < t makes the extra terms vanish.
-    if (tid < NB) {
-        const int t = tid;
-        for (int i = 0; i < NB; ++i) {
-            double acc = (i == t) ? 1.0 : 0.0;
-            for (int k = 0; k < i; ++k) acc -= L[k * LDP + i] * X[t * LDP + k];
-            X[t * LDP + i] = (i >= t) ? acc / L[i * LDP + i] : 0.0;
-        }
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        const int k = wave + 4 * m;
+        if (lane >= k) Ag[(size_t)k * np] = a[m];
     }
     __syncthreads();
-    double *inv = inv_diag + (size_t)s * NB * NB;
-    for (int e = tid; e < NB * NB; e += kThreads) {
-        const int c = e / NB, i = e % NB;
-        inv[e] = X[c * LDP + i];   // inv(i, c) at inv[c*NB + i]; zero above the diagonal
+    if (tid == 0) logdet_part[(size_t)s * (np / B) + c0 / B] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+void launch_gemm_update(const SwarmBuffers &b, int col0, int width, int row_start, int k0, int k1,
+                        hipStream_t st)
+{
+    if (k1 <= k0) return;
+    if (width == kNBO) {
+        const int rt = (b.np - row_start) / kMB;
+        hipLaunchKernelGGL((gemm_update_kernel<kMB, kNBO>), dim3(b.S * rt), dim3(kThreads), 0, st,
+                           b.K, b.np, col0, row_start, k0, k1, b.info);
+    } else {
+        const int rt = (b.np - row_start) / kMB;
+        hipLaunchKernelGGL((gemm_update_kernel<kMB, kNBI>), dim3(b.S * rt), dim3(kThreads), 0, st,
+                           b.K, b.np, col0, row_start, k0, k1, b.info);
     }
 }
 
-void launch_gemm_step(const SwarmBuffers &b, int step, hipStream_t st)
+void launch_potrf64(const SwarmBuffers &b, int c0, hipStream_t st)
 {
-    if (step == 0) return;
-    const int rt_per = (b.np - step * kNB) / kMB;
-    hipLaunchKernelGGL((gemm_step_kernel<kMB, kNB>), dim3(b.S * rt_per), dim3(kThreads), 0, st,
-                       b.K, b.np, step, b.info);
+    hipLaunchKernelGGL(potrf64_kernel, dim3(b.S), dim3(kThreads), 0, st, b.K, b.np, b.n, c0, b.info,
+                       b.logdet_part);
 }
 
-void launch_potrf_diag(const SwarmBuffers &b, int step, hipStream_t st)
+void launch_trsm64(const SwarmBuffers &b, int c0, int row_start, hipStream_t st)
 {
-    hipLaunchKernelGGL(potrf_diag_kernel, dim3(b.S), dim3(kThreads), 0, st, b.K, b.np, b.n, step,
-                       b.np / kNB, b.info, b.logdet_part, b.inv_diag);
-}
-
-void launch_trsm_step(const SwarmBuffers &b, int step, hipStream_t st)
-{
-    const int rt_per = (b.np - (step + 1) * kNB) / kMB;
-    if (rt_per <= 0) return;
-    hipLaunchKernelGGL((trsm_step_kernel<kMB, kNB>), dim3(b.S * rt_per), dim3(kThreads), 0, st,
-                       b.K, b.np, step, b.info, b.inv_diag);
+    const int rt = (b.np - row_start) / kNBI;
+    if (rt <= 0) return;
+    const int groups = ceil_div(rt, 4);
+    hipLaunchKernelGGL(trsm64_kernel, dim3(b.S * groups), dim3(kThreads), 0, st, b.K, b.np, c0, row_start,
+                       groups, b.info);
 }
 
 }  // namespace fb

@@ -56,7 +56,7 @@ static int parse_spec(const fb_kernel_spec *sp, int d, or_node *nodes, int *npar
         case FB_LEAF_RBF_CYCLIC:
             /* stationary_cyclic_kernel.f90:104-127 sums over every feature, so
              * the leaf must cover all d features in order. */
-            if (nd != d) return -1;
+            if (d > 0 && nd != d) return -1;
             o->ndim = nd; o->npar = nd;
             break;
         case FB_LEAF_CONSTANT:

@@ -1,16 +1,19 @@
 """GPU parity: the gfx950 library against the CPU oracle, through the C ABI.
 
-Tolerance: relative log-likelihood error <= 1e-9 against the FP64 oracle for
-the conditioned cases, identical Cholesky INFO for every
-particle.  Cases with the reference's 1e-10 nugget on clustered inputs are
-ill-conditioned (cond(K) ~ 1e10+): there two CPU orderings of the same
-factorisation (the oracle's dpotf2 order and LAPACK's blocked dpotrf) already
-differ by ~1e-9, so those cases are held to the spread of the CPU orderings
-(RTOL_ILL, asserted below against the measured oracle/LAPACK gap).
+Tolerance, per particle: relative log-likelihood error against the FP64
+oracle <= max(1e-9, 10 x g), where g is the relative gap between the oracle
+and LAPACK's own dpotrf/dpotrs (scipy) on the same K.  On conditioned cases
+g ~ 1e-13 and the bar is the 1e-9 of the target; on ill-conditioned ones
+(1e-10 nugget on clustered inputs, cond(K) ~ 1e10+) two CPU orderings of the
+same factorisation already differ by g, and the GPU path must agree with the
+oracle as well as the reference's LAPACK does.  Cholesky INFO must be
+identical wherever the oracle and LAPACK agree (a pivot within rounding of
+zero can flip between any two orderings, LAPACK's included).
 """
 import numpy as np
 import pytest
 import scipy.linalg as sl
+from scipy.linalg import lapack
 
 import oracle
 from ferebus_amd import KernelConfig, SwarmLikelihood, compile_kernel, fit_mean
@@ -18,8 +21,7 @@ from ferebus_amd import synthetic
 
 pytestmark = pytest.mark.gpu
 
-RTOL = 1e-9        # conditioned cases
-RTOL_ILL = 1e-7    # 1e-10 nugget on clustered data
+RTOL = 1e-9        # floor of the relative ll tolerance
 
 
 def _case(n, d, seed, kernel="k1+k2", kernels=None, likelihood="marginal", mean="constant",
@@ -43,14 +45,55 @@ def _oracle(x, y, m, prog, th, noise, optimise_noise, likelihood):
                                likelihood=likelihood)
 
 
-def _check(ll, info, oll, oinfo, rtol, label):
+def _lapack(x, y, m, prog, th, noise, optimise_noise, likelihood):
+    """(ll, info) from LAPACK dpotrf / dpotrs on the oracle's K."""
+    spec = oracle.Spec.from_program(prog)
+    n = len(y)
+    lls, infos = [], []
+    for t in np.atleast_2d(th):
+        k = oracle.kernel_matrix(x, spec, t)
+        k = k + np.tril(k, -1).T + (t[-1] if optimise_noise else noise) * np.eye(n)
+        c, info = lapack.dpotrf(k, lower=1)
+        infos.append(int(info))
+        if info:
+            lls.append(np.nan)
+            continue
+        logdet = 2.0 * np.sum(np.log(np.diag(c)))
+        if likelihood == "marginal":
+            r = y - (m if m is not None else 0.0)
+            a, _ = lapack.dpotrs(c, r, lower=1)
+            lls.append(-0.5 * r @ a - 0.5 * logdet - 0.5 * n * np.log(2 * np.pi))
+        else:
+            ry, _ = lapack.dpotrs(c, y, lower=1)
+            r1, _ = lapack.dpotrs(c, np.ones(n), lower=1)
+            r = y - ry.sum() / r1.sum()
+            a, _ = lapack.dpotrs(c, r, lower=1)
+            lls.append(-0.5 * n * np.log(r @ a / n) - 0.5 * logdet)
+    return np.array(lls), np.array(infos)
+
+
+def _check(ll, info, oll, oinfo, rtol, label, lref=None, swarm_gap=False):
+    """rtol: floor of the relative tolerance; lref = (ll, info) of LAPACK.
+    swarm_gap: use the largest oracle/LAPACK gap of the swarm for every
+    particle (particles of one workload share its conditioning, and a single
+    particle's gap can be small by chance)."""
     print(f"{label}: info gpu={info.tolist()} oracle={oinfo.tolist()}")
-    assert np.array_equal(info, oinfo), f"{label}: INFO differs"
-    ok = info == 0
+    tol = np.full(len(ll), rtol)
+    agree = np.ones(len(ll), dtype=bool)
+    if lref is not None:
+        lll, linfo = lref
+        agree = linfo == oinfo
+        both = (oinfo == 0) & (linfo == 0)
+        gap = np.zeros(len(ll))
+        gap[both] = np.abs(lll[both] - oll[both]) / np.abs(oll[both])
+        tol = np.maximum(rtol, 10.0 * (np.full(len(ll), gap.max()) if swarm_gap else gap))
+        print(f"{label}: oracle vs LAPACK max gap {gap.max():.3e}, INFO disagreements {int((~agree).sum())}")
+    assert np.array_equal(info[agree], oinfo[agree]), f"{label}: INFO differs"
+    ok = (info == 0) & (oinfo == 0)
     if ok.any():
         rel = np.abs(ll[ok] - oll[ok]) / np.abs(oll[ok])
-        print(f"{label}: max relative ll error {rel.max():.3e}")
-        assert rel.max() <= rtol, f"{label}: relative error {rel.max():.3e} > {rtol}"
+        print(f"{label}: max relative ll error {rel.max():.3e} (tolerance {tol[ok].min():.1e}..{tol[ok].max():.1e})")
+        assert (rel <= tol[ok]).all(), f"{label}: relative error {rel.max():.3e} above tolerance"
 
 
 @pytest.mark.parametrize("n", [2, 5, 63, 64, 65, 126, 127, 128, 129, 200, 333])
@@ -58,7 +101,7 @@ def test_marginal_sizes(gpu, n):
     x, y, prog, m, gp, th = _case(n, 6, seed=n)
     ll, info = gp.swarm_loglik(th)
     oll, oinfo = _oracle(x, y, m, prog, th, 1e-6, False, "marginal")
-    _check(ll, info, oll, oinfo, RTOL, f"marginal n={n}")
+    _check(ll, info, oll, oinfo, RTOL, f"marginal n={n}", _lapack(x, y, m, prog, th, 1e-6, False, "marginal"))
 
 
 @pytest.mark.parametrize("n", [3, 62, 63, 64, 126, 127, 190, 300])
@@ -66,7 +109,8 @@ def test_concentrated_sizes(gpu, n):
     x, y, prog, m, gp, th = _case(n, 6, seed=100 + n, likelihood="concentrated")
     ll, info = gp.swarm_loglik(th)
     oll, oinfo = _oracle(x, y, m, prog, th, 1e-6, False, "concentrated")
-    _check(ll, info, oll, oinfo, RTOL, f"concentrated n={n}")
+    _check(ll, info, oll, oinfo, RTOL, f"concentrated n={n}",
+           _lapack(x, y, m, prog, th, 1e-6, False, "concentrated"))
 
 
 KERNEL_CASES = {
@@ -91,7 +135,7 @@ def test_kernel_compositions(gpu, name):
         th[:, 0] = np.abs(th[:, 0]) + 0.5   # the literal's constant is a positive parameter
     ll, info = gp.swarm_loglik(th)
     oll, oinfo = _oracle(x, y, m, prog, th, 1e-6, False, "marginal")
-    _check(ll, info, oll, oinfo, RTOL, f"kernel {name}")
+    _check(ll, info, oll, oinfo, RTOL, f"kernel {name}", _lapack(x, y, m, prog, th, 1e-6, False, "marginal"))
 
 
 @pytest.mark.parametrize("mean", ["zero", "constant", "linear", "quadratic"])
@@ -121,18 +165,20 @@ def test_non_pd_rejections(gpu):
     oll, oinfo = _oracle(x, y, m, prog, th, 1e-6, True, "marginal")
     assert (info[[0, 3]] == 1).all()
     _check(ll, info, oll, oinfo, RTOL, "negative nugget")
-    # duplicated training point -> singular K with zero noise
+    # duplicated training point with a small negative nugget: the pivot of
+    # the duplicate's column is -1e-8 + O(eps), robustly negative, so every
+    # ordering reports INFO = 51 (a zero nugget would leave the sign to rounding)
     rng = np.random.default_rng(5)
     x = synthetic.alf_features(rng, 80, 6)
     x[50] = x[17]
     y = synthetic.bump_target(rng, x, nbumps=4)
     prog = compile_kernel("k1", {"k1": KernelConfig("k1", "rbf")}, 6)
-    gp = SwarmLikelihood(x, y, prog, mean="constant", noise=0.0, max_swarm=4)
+    gp = SwarmLikelihood(x, y, prog, mean="constant", noise=-1e-8, max_swarm=4)
     th = rng.uniform(0.0, 1.0, (4, gp.nparams))
     ll, info = gp.swarm_loglik(th)
-    oll, oinfo = _oracle(x, y, fit_mean("constant", x, y).value, prog, th, 0.0, False, "marginal")
+    oll, oinfo = _oracle(x, y, fit_mean("constant", x, y).value, prog, th, -1e-8, False, "marginal")
     print("duplicate-row INFO", info, oinfo)
-    assert np.array_equal(info, oinfo) and (info > 0).all()
+    assert np.array_equal(info, oinfo) and (info == 51).all()
 
 
 def test_alpha_and_extras(gpu):
@@ -176,18 +222,8 @@ def test_water_reference_nugget(gpu):
     th[:, :2] += 3.0     # short lengthscales keep most particles positive definite
     ll, info = gp.swarm_loglik(th)
     oll, oinfo = _oracle(w.x, w.y, mfit.value, prog, th, 1e-10, False, "marginal")
-    # spread of two CPU orderings on the same inputs
-    spec = oracle.Spec.from_program(prog)
-    gaps = []
-    for q in np.flatnonzero(oinfo == 0)[:4]:
-        k = oracle.kernel_matrix(w.x, spec, th[q])
-        k = k + np.tril(k, -1).T + 1e-10 * np.eye(len(w.y))
-        c, low = sl.cho_factor(k, lower=True)
-        r = w.y - mfit.value
-        lref = -0.5 * r @ sl.cho_solve((c, low), r) - np.sum(np.log(np.diag(c))) - 0.5 * len(r) * np.log(2 * np.pi)
-        gaps.append(abs(lref - oll[q]) / abs(oll[q]))
-    print("oracle vs LAPACK relative gap", gaps)
-    _check(ll, info, oll, oinfo, RTOL_ILL, "water 1e-10 nugget")
+    _check(ll, info, oll, oinfo, RTOL, "water 1e-10 nugget",
+           _lapack(w.x, w.y, mfit.value, prog, th, 1e-10, False, "marginal"), swarm_gap=True)
 
 
 def test_methanol_full_size(gpu):
@@ -207,4 +243,5 @@ def test_methanol_full_size(gpu):
     assert np.array_equal(pinfo, info[:3]) and np.array_equal(pl[pinfo == 0], ll[:3][pinfo == 0])
     assert np.isfinite(ll[info == 0]).all()
     oll, oinfo = _oracle(w.x, w.y, mfit.value, prog, th[:2], 1e-10, False, "marginal")
-    _check(ll[:2], info[:2], oll, oinfo, RTOL_ILL, "methanol N=2000")
+    _check(ll[:2], info[:2], oll, oinfo, RTOL, "methanol N=2000",
+           _lapack(w.x, w.y, mfit.value, prog, th[:2], 1e-10, False, "marginal"))
