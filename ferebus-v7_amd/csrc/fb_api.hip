@@ -7,7 +7,9 @@
 //   gemm_update, potrf64, trsm64 } -> likelihood
 // on one HIP stream.  The batched path replays a captured hipGraph of that
 // pipeline; the plain path launches it eagerly, one particle at a time.
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -152,11 +154,24 @@ struct fb_handle {
     int n_slots = 0;
     KernelProgram prog;
     hipStream_t own_stream = nullptr, stream = nullptr;
+    // extra streams: the swarm is split into groups whose block pipelines run
+    // concurrently, so one group's latency-bound potrf / trsm steps overlap the
+    // other groups' trailing updates
+    static constexpr int kMaxGroups = 8;
+    int n_groups = 1;
+    bool use_graph = true;
+    bool use_dag = true;             // persistent factorisation for the batched path
+    int dag_grid = 0, n_xcd = 1;
+    int *d_cnt = nullptr, *d_ctl = nullptr, *h_err = nullptr;
+    // swarm size -> device task table and its queue offsets
+    std::map<int, std::pair<fb::DagTask *, std::vector<int>>> dag_tables;
+    hipStream_t group_stream[kMaxGroups] = {};
+    hipEvent_t fork_ev = nullptr, join_ev[kMaxGroups] = {};
     int device = 0;
     // device buffers
     KernelProgram *d_prog = nullptr;
     double *d_slots = nullptr, *d_rhs = nullptr, *d_theta = nullptr, *d_hyp = nullptr;
-    double *d_K = nullptr, *d_logdet = nullptr, *d_ll = nullptr;
+    double *d_K = nullptr, *d_logdet = nullptr, *d_dinv = nullptr, *d_ll = nullptr;
     double *d_extra = nullptr, *d_z = nullptr, *d_alpha = nullptr;
     int *d_info = nullptr;
     // pinned staging
@@ -168,25 +183,28 @@ struct fb_handle {
     bool profile = false;
     std::vector<EventPair> pending;
     std::vector<hipEvent_t> event_pool;
-    double prof_ms[6] = {0, 0, 0, 0, 0, 0};
-    long long prof_n[6] = {0, 0, 0, 0, 0, 0};
+    double prof_ms[7] = {0, 0, 0, 0, 0, 0, 0};
+    long long prof_n[7] = {0, 0, 0, 0, 0, 0, 0};
 };
 
 namespace {
 
-fb::SwarmBuffers buffers(fb_handle *h, int S, const double *theta, double *ll, int *info,
-                         size_t k_offset_matrices)
+// Buffers of particles [g0, g0 + S) of a swarm whose theta / ll / info
+// arrays start at particle 0.
+fb::SwarmBuffers buffers(fb_handle *h, int g0, int S, const double *theta, double *ll, int *info)
 {
     fb::SwarmBuffers b;
     b.n = h->n; b.d = h->d; b.np = h->np; b.nrhs = h->nrhs; b.likelihood = h->likelihood;
     b.S = S;
     b.n_slots = h->n_slots; b.n_leaves = h->prog.n_leaves;
-    b.slots = h->d_slots; b.RHS = h->d_rhs; b.hyp = h->d_hyp;
-    b.theta = theta; b.prog = h->d_prog;
-    b.K = h->d_K + k_offset_matrices * (size_t)h->np * h->np;
-    b.info = info; b.ll = ll;
-    b.logdet_part = h->d_logdet;
-    b.extra = h->d_extra; b.zbuf = h->d_z;
+    b.slots = h->d_slots; b.RHS = h->d_rhs;
+    b.hyp = h->d_hyp + (size_t)g0 * (h->prog.n_par_kernel + fb::kMaxTerms + 1);
+    b.theta = theta + (size_t)g0 * h->prog.n_par; b.prog = h->d_prog;
+    b.K = h->d_K + (size_t)g0 * h->np * h->np;
+    b.info = info + g0; b.ll = ll + g0;
+    b.logdet_part = h->d_logdet + (size_t)g0 * h->nsteps;
+    b.dinv = h->d_dinv + (size_t)g0 * h->nsteps * fb::kDinv;
+    b.extra = h->d_extra + (size_t)g0 * 4; b.zbuf = h->d_z + (size_t)g0 * h->n;
     return b;
 }
 
@@ -214,7 +232,8 @@ void timed(fb_handle *h, int kind, hipStream_t st, F fn)
     if (p.a && p.b) h->pending.push_back(p);
 }
 
-void enqueue_pipeline(fb_handle *h, const fb::SwarmBuffers &b, hipStream_t st)
+// Lock-step pipeline: one launch per factorisation step (the plain path).
+void enqueue_group(fb_handle *h, const fb::SwarmBuffers &b, hipStream_t st)
 {
     timed(h, 0, st, [&] { fb::launch_build(b, st); });
     for (int c0 = 0; c0 < h->np; c0 += fb::kNBO) {
@@ -227,6 +246,63 @@ void enqueue_pipeline(fb_handle *h, const fb::SwarmBuffers &b, hipStream_t st)
         if (c0 + fb::kNBO < h->np) timed(h, 3, st, [&] { fb::launch_trsm64(b, c1, c0 + fb::kNBO, st); });
     }
     timed(h, 4, st, [&] { fb::launch_likelihood(b, st); });
+}
+
+// Persistent pipeline: build, the whole factorisation as one task-graph
+// launch, likelihood.  The task table for S must exist (dag_table).
+void enqueue_dag(fb_handle *h, const fb::SwarmBuffers &b, hipStream_t st)
+{
+    const auto &tab = h->dag_tables.at(b.S);
+    timed(h, 0, st, [&] { fb::launch_build(b, st); });
+    timed(h, 6, st, [&] { fb::launch_dag(b, tab.first, tab.second, h->d_cnt, h->d_ctl, h->dag_grid, st); });
+    timed(h, 4, st, [&] { fb::launch_likelihood(b, st); });
+}
+
+// Uploads the task table of a swarm of S (outside any graph capture).
+int dag_table(fb_handle *h, int S)
+{
+    if (h->dag_tables.count(S)) return FB_OK;
+    fb::DagSchedule sch = fb::dag_schedule(S, h->np, h->n_xcd);
+    const size_t bytes = sizeof(fb::DagTask) * sch.tasks.size();
+    fb::DagTask *d = nullptr;
+    FB_HIP(hipMalloc((void **)&d, bytes));
+    hipError_t e = hipMemcpy(d, sch.tasks.data(), bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(d);
+        return fail(FB_E_HIP, std::string("task table upload: ") + hipGetErrorString(e));
+    }
+    h->dag_tables[S] = std::make_pair(d, std::move(sch.qoff));
+    return FB_OK;
+}
+
+// The whole swarm [0, S): the persistent path, or groups of particles on
+// concurrent streams forked from and joined back into `st` (also inside a
+// graph capture).
+void enqueue_pipeline(fb_handle *h, int S, const double *theta, double *ll, int *info, hipStream_t st)
+{
+    if (h->use_dag) {
+        enqueue_dag(h, buffers(h, 0, S, theta, ll, info), st);
+        return;
+    }
+    const int G = std::max(1, std::min(h->n_groups, S));
+    if (G == 1) {
+        enqueue_group(h, buffers(h, 0, S, theta, ll, info), st);
+        return;
+    }
+    (void)hipEventRecord(h->fork_ev, st);
+    const int per = (S + G - 1) / G;
+    for (int g = 0; g < G; ++g) {
+        const int g0 = g * per;
+        const int cnt = std::min(per, S - g0);
+        if (cnt <= 0) break;
+        hipStream_t gs = g == 0 ? st : h->group_stream[g];
+        if (g > 0) (void)hipStreamWaitEvent(gs, h->fork_ev, 0);
+        enqueue_group(h, buffers(h, g0, cnt, theta, ll, info), gs);
+        if (g > 0) {
+            (void)hipEventRecord(h->join_ev[g], gs);
+            (void)hipStreamWaitEvent(st, h->join_ev[g], 0);
+        }
+    }
 }
 
 int drain_profile(fb_handle *h)
@@ -260,22 +336,37 @@ int check_gfx950(int *dev)
     return FB_OK;
 }
 
-void free_all(fb_handle *h)
+void drop_graphs(fb_handle *h)
 {
     for (auto &g : h->graphs) {
         (void)hipGraphExecDestroy(g.second.second);
         (void)hipGraphDestroy(g.second.first);
     }
     h->graphs.clear();
+}
+
+void free_all(fb_handle *h)
+{
+    drop_graphs(h);
+    for (auto &t : h->dag_tables) (void)hipFree(t.second.first);
+    h->dag_tables.clear();
+    if (h->d_cnt) (void)hipFree(h->d_cnt);
+    if (h->d_ctl) (void)hipFree(h->d_ctl);
+    if (h->h_err) (void)hipHostFree(h->h_err);
     for (const EventPair &p : h->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
     for (hipEvent_t e : h->event_pool) (void)hipEventDestroy(e);
-    void *dev[] = {h->d_prog, h->d_slots, h->d_rhs, h->d_theta, h->d_hyp, h->d_K, h->d_logdet,
+    void *dev[] = {h->d_prog, h->d_slots, h->d_rhs, h->d_theta, h->d_hyp, h->d_K, h->d_logdet, h->d_dinv,
                    h->d_ll, h->d_extra, h->d_z, h->d_alpha, h->d_info};
     for (void *p : dev)
         if (p) (void)hipFree(p);
     void *host[] = {h->h_theta, h->h_ll, h->h_extra, h->h_info};
     for (void *p : host)
         if (p) (void)hipHostFree(p);
+    for (int g = 0; g < fb_handle::kMaxGroups; ++g) {
+        if (h->group_stream[g]) (void)hipStreamDestroy(h->group_stream[g]);
+        if (h->join_ev[g]) (void)hipEventDestroy(h->join_ev[g]);
+    }
+    if (h->fork_ev) (void)hipEventDestroy(h->fork_ev);
     if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
 }
 
@@ -353,16 +444,20 @@ int fb_create(fb_handle **out, int n, int d, const double *x, const double *y, c
     FB_ALLOC(h->d_hyp, sizeof(double) * hs * S);
     FB_ALLOC(h->d_K, sizeof(double) * np2 * S);
     FB_ALLOC(h->d_logdet, sizeof(double) * h->nsteps * S);
+    FB_ALLOC(h->d_dinv, sizeof(double) * h->nsteps * S * fb::kDinv);
     FB_ALLOC(h->d_ll, sizeof(double) * S);
     FB_ALLOC(h->d_extra, sizeof(double) * 4 * S);
     FB_ALLOC(h->d_z, sizeof(double) * n * S);
     FB_ALLOC(h->d_alpha, sizeof(double) * n);
     FB_ALLOC(h->d_info, sizeof(int) * S);
+    FB_ALLOC(h->d_cnt, sizeof(int) * S * (3 * (h->np / fb::kNBI) + 1));
+    FB_ALLOC(h->d_ctl, sizeof(int) * fb::kDagCtl);
 #undef FB_ALLOC
     if (hipHostMalloc((void **)&h->h_theta, sizeof(double) * std::max(h->prog.n_par, 1) * S) != hipSuccess ||
         hipHostMalloc((void **)&h->h_ll, sizeof(double) * S) != hipSuccess ||
         hipHostMalloc((void **)&h->h_extra, sizeof(double) * 4 * S) != hipSuccess ||
-        hipHostMalloc((void **)&h->h_info, sizeof(int) * S) != hipSuccess) {
+        hipHostMalloc((void **)&h->h_info, sizeof(int) * S) != hipSuccess ||
+        hipHostMalloc((void **)&h->h_err, sizeof(int)) != hipSuccess) {
         free_all(h);
         delete h;
         return fail(FB_E_NOMEM, "fb_create: pinned host allocation failed");
@@ -373,10 +468,26 @@ int fb_create(fb_handle **out, int n, int d, const double *x, const double *y, c
         return fail(FB_E_HIP, "fb_create: stream creation failed");
     }
     h->stream = h->own_stream;
+    if (const char *env = std::getenv("FB_STREAMS")) h->n_groups = std::atoi(env);
+    if (const char *env = std::getenv("FB_GRAPH")) h->use_graph = std::atoi(env) != 0;
+    if (const char *env = std::getenv("FB_PERSISTENT")) h->use_dag = std::atoi(env) != 0;
+    h->dag_grid = fb::dag_grid_size(&h->n_xcd);
+    if (h->dag_grid <= 0) h->use_dag = false;
+    h->n_groups = std::max(1, std::min(h->n_groups, fb_handle::kMaxGroups));
+    bool ev_ok = hipEventCreateWithFlags(&h->fork_ev, hipEventDisableTiming) == hipSuccess;
+    for (int g = 1; g < h->n_groups && ev_ok; ++g)
+        ev_ok = hipStreamCreateWithFlags(&h->group_stream[g], hipStreamNonBlocking) == hipSuccess &&
+                hipEventCreateWithFlags(&h->join_ev[g], hipEventDisableTiming) == hipSuccess;
+    if (!ev_ok) {
+        free_all(h);
+        delete h;
+        return fail(FB_E_HIP, "fb_create: stream / event creation failed");
+    }
     hipError_t e = hipMemcpy(h->d_prog, &h->prog, sizeof(KernelProgram), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(h->d_slots, slots.data(), sizeof(double) * slots.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(h->d_rhs, rhs.data(), sizeof(double) * rhs.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(h->d_info, 0, sizeof(int) * S);
+    if (e == hipSuccess) e = hipMemset(h->d_ctl, 0, sizeof(int) * fb::kDagCtl);
     if (e != hipSuccess) {
         free_all(h);
         delete h;
@@ -414,30 +525,36 @@ int fb_swarm_loglik_async(fb_handle *h, int s, const double *d_theta, double *d_
 {
     if (!h || !d_theta || !d_ll || !d_info) return fail(FB_E_ARG, "fb_swarm_loglik_async: null argument");
     if (s < 1 || s > h->max_swarm) return fail(FB_E_CAPACITY, "fb_swarm_loglik_async: swarm size out of range");
-    fb::SwarmBuffers b = buffers(h, s, d_theta, d_ll, d_info, 0);
-    enqueue_pipeline(h, b, h->stream);
+    if (h->use_dag) {
+        const int rc = dag_table(h, s);
+        if (rc) return rc;
+    }
+    enqueue_pipeline(h, s, d_theta, d_ll, d_info, h->stream);
     FB_HIP(hipGetLastError());
     return FB_OK;
 }
 
-int fb_swarm_loglik(fb_handle *h, int s, const double *theta, double *ll, int *info)
+// One evaluation of the swarm already staged in d_theta; leaves ll / INFO in
+// the pinned host buffers.  *dag_error: the persistent factorisation's error
+// word (0 ok, 1 a wait timed out, 2 a queue was not drained).
+static int run_swarm(fb_handle *h, int s, int *dag_error)
 {
-    if (!h || !theta || !ll || !info) return fail(FB_E_ARG, "fb_swarm_loglik: null argument");
-    if (s < 1 || s > h->max_swarm) return fail(FB_E_CAPACITY, "fb_swarm_loglik: swarm size out of range");
-    const int P = h->prog.n_par;
     hipStream_t st = h->stream;
-    std::memcpy(h->h_theta, theta, sizeof(double) * P * s);
-    FB_HIP(hipMemcpyAsync(h->d_theta, h->h_theta, sizeof(double) * P * s, hipMemcpyHostToDevice, st));
-    fb::SwarmBuffers b = buffers(h, s, h->d_theta, h->d_ll, h->d_info, 0);
     if (h->profile) {
-        timed(h, 5, st, [&] { enqueue_pipeline(h, b, st); });
+        // one stream, so the per-kernel event pairs time each kernel alone
+        const int g = h->n_groups;
+        h->n_groups = 1;
+        timed(h, 5, st, [&] { enqueue_pipeline(h, s, h->d_theta, h->d_ll, h->d_info, st); });
+        h->n_groups = g;
+    } else if (!h->use_graph) {
+        enqueue_pipeline(h, s, h->d_theta, h->d_ll, h->d_info, st);
     } else {
         auto it = h->graphs.find(s);
         if (it == h->graphs.end()) {
             hipGraph_t g = nullptr;
             hipGraphExec_t ge = nullptr;
             FB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            enqueue_pipeline(h, b, st);
+            enqueue_pipeline(h, s, h->d_theta, h->d_ll, h->d_info, st);
             FB_HIP(hipStreamEndCapture(st, &g));
             FB_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
             it = h->graphs.emplace(s, std::make_pair(g, ge)).first;
@@ -446,8 +563,38 @@ int fb_swarm_loglik(fb_handle *h, int s, const double *theta, double *ll, int *i
     }
     FB_HIP(hipMemcpyAsync(h->h_ll, h->d_ll, sizeof(double) * s, hipMemcpyDeviceToHost, st));
     FB_HIP(hipMemcpyAsync(h->h_info, h->d_info, sizeof(int) * s, hipMemcpyDeviceToHost, st));
+    *h->h_err = 0;
+    if (h->use_dag)
+        FB_HIP(hipMemcpyAsync(h->h_err, h->d_ctl + fb::kDagCtl - 1, sizeof(int), hipMemcpyDeviceToHost, st));
     FB_HIP(hipStreamSynchronize(st));
     FB_HIP(hipGetLastError());
+    *dag_error = *h->h_err;
+    return FB_OK;
+}
+
+int fb_swarm_loglik(fb_handle *h, int s, const double *theta, double *ll, int *info)
+{
+    if (!h || !theta || !ll || !info) return fail(FB_E_ARG, "fb_swarm_loglik: null argument");
+    if (s < 1 || s > h->max_swarm) return fail(FB_E_CAPACITY, "fb_swarm_loglik: swarm size out of range");
+    const int P = h->prog.n_par;
+    if (h->use_dag) {
+        const int rc = dag_table(h, s);
+        if (rc) return rc;
+    }
+    std::memcpy(h->h_theta, theta, sizeof(double) * P * s);
+    FB_HIP(hipMemcpyAsync(h->d_theta, h->h_theta, sizeof(double) * P * s, hipMemcpyHostToDevice, h->stream));
+    int err = 0;
+    int rc = run_swarm(h, s, &err);
+    if (rc) return rc;
+    if (err == 1) return fail(FB_E_HIP, "fb_swarm_loglik: a dependency wait of the persistent factorisation timed out");
+    if (err == 2) {
+        // some XCD queue found no work-group of its own: this device does not
+        // dispatch the way the persistent schedule assumes; use lock-step
+        h->use_dag = false;
+        drop_graphs(h);
+        rc = run_swarm(h, s, &err);
+        if (rc) return rc;
+    }
     std::memcpy(ll, h->h_ll, sizeof(double) * s);
     std::memcpy(info, h->h_info, sizeof(int) * s);
     return h->profile ? drain_profile(h) : FB_OK;
@@ -462,8 +609,7 @@ int fb_particle_loglik(fb_handle *h, int s, const double *theta, double *ll, int
     for (int q = 0; q < s; ++q) {
         std::memcpy(h->h_theta, theta + (size_t)P * q, sizeof(double) * P);
         FB_HIP(hipMemcpyAsync(h->d_theta, h->h_theta, sizeof(double) * P, hipMemcpyHostToDevice, st));
-        fb::SwarmBuffers b = buffers(h, 1, h->d_theta, h->d_ll, h->d_info, 0);
-        enqueue_pipeline(h, b, st);
+        enqueue_group(h, buffers(h, 0, 1, h->d_theta, h->d_ll, h->d_info), st);
         FB_HIP(hipMemcpyAsync(h->h_ll, h->d_ll, sizeof(double), hipMemcpyDeviceToHost, st));
         FB_HIP(hipMemcpyAsync(h->h_info, h->d_info, sizeof(int), hipMemcpyDeviceToHost, st));
         FB_HIP(hipStreamSynchronize(st));
@@ -481,8 +627,8 @@ int fb_get_alpha(fb_handle *h, const double *theta, double *alpha, double *out4,
     hipStream_t st = h->stream;
     std::memcpy(h->h_theta, theta, sizeof(double) * P);
     FB_HIP(hipMemcpyAsync(h->d_theta, h->h_theta, sizeof(double) * P, hipMemcpyHostToDevice, st));
-    fb::SwarmBuffers b = buffers(h, 1, h->d_theta, h->d_ll, h->d_info, 0);
-    enqueue_pipeline(h, b, st);
+    fb::SwarmBuffers b = buffers(h, 0, 1, h->d_theta, h->d_ll, h->d_info);
+    enqueue_group(h, b, st);
     FB_HIP(hipMemcpyAsync(h->h_info, h->d_info, sizeof(int), hipMemcpyDeviceToHost, st));
     FB_HIP(hipStreamSynchronize(st));
     *info = h->h_info[0];
@@ -513,7 +659,7 @@ int fb_profile_enable(fb_handle *h, int enable)
 
 int fb_profile_read(fb_handle *h, int kind, double *total_ms, long long *launches)
 {
-    if (!h || kind < 0 || kind > 5 || !total_ms || !launches) return fail(FB_E_ARG, "fb_profile_read: bad argument");
+    if (!h || kind < 0 || kind > 6 || !total_ms || !launches) return fail(FB_E_ARG, "fb_profile_read: bad argument");
     int rc = drain_profile(h);
     *total_ms = h->prof_ms[kind];
     *launches = h->prof_n[kind];

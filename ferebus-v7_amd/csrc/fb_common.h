@@ -18,6 +18,8 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include <vector>
 #include <stdint.h>
 
 namespace fb {
@@ -58,6 +60,7 @@ constexpr int kNBI = 64;      // diagonal block order (potrf / trsm)
 constexpr int kMB = 64;       // rows of one update tile
 constexpr int kKC = 16;       // depth of one LDS stage of the update GEMM
 constexpr int kThreads = 256; // work-group size of every kernel (4 wave64)
+constexpr int kDinv = 4 * 16 * 16;   // doubles per 64x64 diagonal block: four 16x16 inverses
 
 inline __host__ __device__ int ceil_div(int a, int b) { return (a + b - 1) / b; }
 inline __host__ __device__ int round_up(int a, int b) { return ceil_div(a, b) * b; }
@@ -75,6 +78,7 @@ struct SwarmBuffers {
     double *K;             // S x np x np
     int *info;             // S
     double *logdet_part;   // S x (np / kNBI)
+    double *dinv;          // S x (np / kNBI) x kDinv: inverses of the 16x16 diagonal blocks of L
     double *ll;            // S
     double *extra;         // S x 4: ll, logdet, mu_hat, sigma2_hat
     double *zbuf;          // S x n: z = L^{-1} rhs_0 (or the concentrated residual)
@@ -86,6 +90,22 @@ void launch_gemm_update(const SwarmBuffers &b, int col0, int width, int row_star
 void launch_potrf64(const SwarmBuffers &b, int c0, hipStream_t st);
 void launch_trsm64(const SwarmBuffers &b, int c0, int row_start, hipStream_t st);
 void launch_likelihood(const SwarmBuffers &b, hipStream_t st);
+
+// Persistent task-graph factorisation (fb_dag.hip).
+constexpr int kDagMaxQueues = 16;
+constexpr int kDagCtl = 2 * kDagMaxQueues + 2;   // heads, owners, finished work-groups, error
+struct DagTask {
+    short type, s, a, i;
+};
+struct DagSchedule {
+    int n_queues = 0;
+    std::vector<DagTask> tasks;   // queue q = tasks[qoff[q], qoff[q + 1])
+    std::vector<int> qoff;
+};
+int dag_grid_size(int *n_xcd);
+DagSchedule dag_schedule(int S, int np, int n_queues);
+void launch_dag(const SwarmBuffers &b, const DagTask *tasks, const std::vector<int> &qoff, int *cnt, int *ctl,
+                int grid, hipStream_t st);
 void launch_backsolve(const SwarmBuffers &b, int particle, double *alpha, hipStream_t st);
 
 }  // namespace fb

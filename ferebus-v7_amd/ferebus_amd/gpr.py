@@ -17,7 +17,7 @@ from .kernel_expr import KernelConfig, KernelProgram, compile_kernel
 from .mean import MeanFit, fit_mean
 
 LIKELIHOODS = {"marginal": 0, "concentrated": 1}
-PROFILE_KINDS = {"build": 0, "gemm": 1, "potrf": 2, "trsm": 3, "likelihood": 4, "swarm": 5}
+PROFILE_KINDS = {"build": 0, "gemm": 1, "potrf": 2, "trsm": 3, "likelihood": 4, "swarm": 5, "factor": 6}
 
 
 class SwarmLikelihood:

@@ -40,8 +40,10 @@ KERNEL_TYPES = {
     "linear": LEAF_LINEAR,
 }
 
-_TOKEN = re.compile(r"\s*(?:(?P<num>\d+\.?\d*(?:[eE][-+]?\d+)?|\.\d+(?:[eE][-+]?\d+)?)"
-                    r"|(?P<id>[A-Za-z_][A-Za-z0-9_\-]*)|(?P<op>[-+*/()]))")
+# lexer.f90:95-160: numbers start with a digit (optional fraction and
+# exponent); identifiers are a letter followed by letters and digits
+_TOKEN = re.compile(r"\s*(?:(?P<num>\d+(?:\.\d*)?(?:[eE][-+]?\d*)?)"
+                    r"|(?P<id>[A-Za-z][A-Za-z0-9]*)|(?P<op>[-+*/()]))")
 
 
 class KernelExpressionError(ValueError):

@@ -190,17 +190,30 @@ contains
         return
       end if
       p%pos = p%pos + 1
-    else if (index("0123456789.", c) > 0 .and. c /= achar(0)) then
-      do while (p%pos <= len(p%s))
-        if (index("0123456789.eE", p%s(p%pos:p%pos)) == 0) exit
-        p%pos = p%pos + 1
-      end do
+    else if (index("0123456789", c) > 0 .and. c /= achar(0)) then
+      ! lexer.f90:95-130: digits, optional fraction, optional exponent
+      call skip_digits(p)
+      if (p%pos <= len(p%s)) then
+        if (p%s(p%pos:p%pos) == ".") then
+          p%pos = p%pos + 1
+          call skip_digits(p)
+        end if
+      end if
+      if (p%pos <= len(p%s)) then
+        if (p%s(p%pos:p%pos) == "e" .or. p%s(p%pos:p%pos) == "E") then
+          p%pos = p%pos + 1
+          if (p%pos <= len(p%s)) then
+            if (p%s(p%pos:p%pos) == "+" .or. p%s(p%pos:p%pos) == "-") p%pos = p%pos + 1
+          end if
+          call skip_digits(p)
+        end if
+      end if
       call emit(p, FB_LEAF_CONSTANT, 0, "c")
       name = "c"
     else if (is_alpha(c)) then
       start = p%pos
       do while (p%pos <= len(p%s))
-        if (.not. (is_alpha(p%s(p%pos:p%pos)) .or. index("0123456789_-", p%s(p%pos:p%pos)) > 0)) exit
+        if (.not. (is_alpha(p%s(p%pos:p%pos)) .or. index("0123456789", p%s(p%pos:p%pos)) > 0)) exit
         p%pos = p%pos + 1
       end do
       name = p%s(start:p%pos-1)
@@ -253,8 +266,17 @@ contains
     p%prog%active_dims = [p%prog%active_dims, int(dims, c_int)]
   end subroutine add_leaf
 
+  !> str.f90:174-180
   logical function is_alpha(c)
     character, intent(in) :: c
-    is_alpha = (c >= "a" .and. c <= "z") .or. (c >= "A" .and. c <= "Z") .or. c == "_"
+    is_alpha = (c >= "a" .and. c <= "z") .or. (c >= "A" .and. c <= "Z")
   end function is_alpha
+
+  subroutine skip_digits(p)
+    type(Parser), intent(inout) :: p
+    do while (p%pos <= len(p%s))
+      if (index("0123456789", p%s(p%pos:p%pos)) == 0) exit
+      p%pos = p%pos + 1
+    end do
+  end subroutine skip_digits
 end module fb_kernel_expr

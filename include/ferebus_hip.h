@@ -141,7 +141,8 @@ int fb_swarm_loglik_grad(fb_handle *h, int s, const double *theta, double *ll,
 /* Kernel timing: when enabled, each launch of the handle is bracketed by HIP
  * events and the per-kernel totals accumulate until read.
  * kind: 0 covariance build, 1 trailing-update GEMM, 2 diagonal potrf,
- *       3 panel trsm, 4 likelihood reduction, 5 whole swarm evaluation. */
+ *       3 panel trsm, 4 likelihood reduction, 5 whole swarm evaluation,
+ *       6 persistent (single-launch) factorisation of the batched path. */
 int fb_profile_enable(fb_handle *h, int enable);
 int fb_profile_read(fb_handle *h, int kind, double *total_ms, long long *launches);
 
