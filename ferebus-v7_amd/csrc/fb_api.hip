@@ -162,9 +162,9 @@ struct fb_handle {
     bool use_graph = true;
     bool use_dag = true;             // persistent factorisation for the batched path
     int dag_grid = 0, n_xcd = 1;
-    int *d_cnt = nullptr, *d_ctl = nullptr, *h_err = nullptr;
+    int *d_ctl = nullptr, *d_dep = nullptr, *d_dep0 = nullptr, *d_dead = nullptr, *h_err = nullptr;
+    unsigned long long *d_ready = nullptr;
     // swarm size -> device task table and its queue offsets
-    std::map<int, std::pair<fb::DagTask *, std::vector<int>>> dag_tables;
     hipStream_t group_stream[kMaxGroups] = {};
     hipEvent_t fork_ev = nullptr, join_ev[kMaxGroups] = {};
     int device = 0;
@@ -249,30 +249,14 @@ void enqueue_group(fb_handle *h, const fb::SwarmBuffers &b, hipStream_t st)
 }
 
 // Persistent pipeline: build, the whole factorisation as one task-graph
-// launch, likelihood.  The task table for S must exist (dag_table).
+// launch, likelihood.
 void enqueue_dag(fb_handle *h, const fb::SwarmBuffers &b, hipStream_t st)
 {
-    const auto &tab = h->dag_tables.at(b.S);
+    fb::DagWork w;
+    w.ctl = h->d_ctl; w.dep = h->d_dep; w.dep0 = h->d_dep0; w.slots = h->d_ready; w.dead = h->d_dead;
     timed(h, 0, st, [&] { fb::launch_build(b, st); });
-    timed(h, 6, st, [&] { fb::launch_dag(b, tab.first, tab.second, h->d_cnt, h->d_ctl, h->dag_grid, st); });
+    timed(h, 6, st, [&] { fb::launch_dag(b, w, h->n_xcd, h->dag_grid, st); });
     timed(h, 4, st, [&] { fb::launch_likelihood(b, st); });
-}
-
-// Uploads the task table of a swarm of S (outside any graph capture).
-int dag_table(fb_handle *h, int S)
-{
-    if (h->dag_tables.count(S)) return FB_OK;
-    fb::DagSchedule sch = fb::dag_schedule(S, h->np, h->n_xcd);
-    const size_t bytes = sizeof(fb::DagTask) * sch.tasks.size();
-    fb::DagTask *d = nullptr;
-    FB_HIP(hipMalloc((void **)&d, bytes));
-    hipError_t e = hipMemcpy(d, sch.tasks.data(), bytes, hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-        (void)hipFree(d);
-        return fail(FB_E_HIP, std::string("task table upload: ") + hipGetErrorString(e));
-    }
-    h->dag_tables[S] = std::make_pair(d, std::move(sch.qoff));
-    return FB_OK;
 }
 
 // The whole swarm [0, S): the persistent path, or groups of particles on
@@ -348,10 +332,8 @@ void drop_graphs(fb_handle *h)
 void free_all(fb_handle *h)
 {
     drop_graphs(h);
-    for (auto &t : h->dag_tables) (void)hipFree(t.second.first);
-    h->dag_tables.clear();
-    if (h->d_cnt) (void)hipFree(h->d_cnt);
-    if (h->d_ctl) (void)hipFree(h->d_ctl);
+    for (void *p : {(void *)h->d_ctl, (void *)h->d_dep, (void *)h->d_dep0, (void *)h->d_dead, (void *)h->d_ready})
+        if (p) (void)hipFree(p);
     if (h->h_err) (void)hipHostFree(h->h_err);
     for (const EventPair &p : h->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
     for (hipEvent_t e : h->event_pool) (void)hipEventDestroy(e);
@@ -450,8 +432,20 @@ int fb_create(fb_handle **out, int n, int d, const double *x, const double *y, c
     FB_ALLOC(h->d_z, sizeof(double) * n * S);
     FB_ALLOC(h->d_alpha, sizeof(double) * n);
     FB_ALLOC(h->d_info, sizeof(int) * S);
-    FB_ALLOC(h->d_cnt, sizeof(int) * S * (3 * (h->np / fb::kNBI) + 1));
+    // persistent scheduler: counters per task of every matrix, ready lists
+    // sized for the largest queue of any swarm size up to S
+    h->dag_grid = fb::dag_grid_size(&h->n_xcd);
+    size_t ready_slots = 0;
+    for (int s = 1; s <= S; ++s) {
+        const int Q = fb::dag_queues(s, h->n_xcd);
+        ready_slots = std::max(ready_slots, (size_t)Q * 2 * fb::dag_slot_capacity(s, h->np, Q));
+    }
+    const std::vector<int> dep0 = fb::dag_initial_counts(h->np);
     FB_ALLOC(h->d_ctl, sizeof(int) * fb::kDagCtl);
+    FB_ALLOC(h->d_dep, sizeof(int) * dep0.size() * S);
+    FB_ALLOC(h->d_dep0, sizeof(int) * dep0.size());
+    FB_ALLOC(h->d_dead, sizeof(int) * S);
+    FB_ALLOC(h->d_ready, sizeof(unsigned long long) * ready_slots);
 #undef FB_ALLOC
     if (hipHostMalloc((void **)&h->h_theta, sizeof(double) * std::max(h->prog.n_par, 1) * S) != hipSuccess ||
         hipHostMalloc((void **)&h->h_ll, sizeof(double) * S) != hipSuccess ||
@@ -471,7 +465,6 @@ int fb_create(fb_handle **out, int n, int d, const double *x, const double *y, c
     if (const char *env = std::getenv("FB_STREAMS")) h->n_groups = std::atoi(env);
     if (const char *env = std::getenv("FB_GRAPH")) h->use_graph = std::atoi(env) != 0;
     if (const char *env = std::getenv("FB_PERSISTENT")) h->use_dag = std::atoi(env) != 0;
-    h->dag_grid = fb::dag_grid_size(&h->n_xcd);
     if (h->dag_grid <= 0) h->use_dag = false;
     h->n_groups = std::max(1, std::min(h->n_groups, fb_handle::kMaxGroups));
     bool ev_ok = hipEventCreateWithFlags(&h->fork_ev, hipEventDisableTiming) == hipSuccess;
@@ -488,6 +481,7 @@ int fb_create(fb_handle **out, int n, int d, const double *x, const double *y, c
     if (e == hipSuccess) e = hipMemcpy(h->d_rhs, rhs.data(), sizeof(double) * rhs.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(h->d_info, 0, sizeof(int) * S);
     if (e == hipSuccess) e = hipMemset(h->d_ctl, 0, sizeof(int) * fb::kDagCtl);
+    if (e == hipSuccess) e = hipMemcpy(h->d_dep0, dep0.data(), sizeof(int) * dep0.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         free_all(h);
         delete h;
@@ -525,10 +519,6 @@ int fb_swarm_loglik_async(fb_handle *h, int s, const double *d_theta, double *d_
 {
     if (!h || !d_theta || !d_ll || !d_info) return fail(FB_E_ARG, "fb_swarm_loglik_async: null argument");
     if (s < 1 || s > h->max_swarm) return fail(FB_E_CAPACITY, "fb_swarm_loglik_async: swarm size out of range");
-    if (h->use_dag) {
-        const int rc = dag_table(h, s);
-        if (rc) return rc;
-    }
     enqueue_pipeline(h, s, d_theta, d_ll, d_info, h->stream);
     FB_HIP(hipGetLastError());
     return FB_OK;
@@ -536,7 +526,7 @@ int fb_swarm_loglik_async(fb_handle *h, int s, const double *d_theta, double *d_
 
 // One evaluation of the swarm already staged in d_theta; leaves ll / INFO in
 // the pinned host buffers.  *dag_error: the persistent factorisation's error
-// word (0 ok, 1 a wait timed out, 2 a queue was not drained).
+// word (0 ok, 1 no ready task for 2 s, 2 a queue was not drained).
 static int run_swarm(fb_handle *h, int s, int *dag_error)
 {
     hipStream_t st = h->stream;
@@ -565,7 +555,7 @@ static int run_swarm(fb_handle *h, int s, int *dag_error)
     FB_HIP(hipMemcpyAsync(h->h_info, h->d_info, sizeof(int) * s, hipMemcpyDeviceToHost, st));
     *h->h_err = 0;
     if (h->use_dag)
-        FB_HIP(hipMemcpyAsync(h->h_err, h->d_ctl + fb::kDagCtl - 1, sizeof(int), hipMemcpyDeviceToHost, st));
+        FB_HIP(hipMemcpyAsync(h->h_err, h->d_ctl + fb::kDagErrorWord, sizeof(int), hipMemcpyDeviceToHost, st));
     FB_HIP(hipStreamSynchronize(st));
     FB_HIP(hipGetLastError());
     *dag_error = *h->h_err;
@@ -577,16 +567,12 @@ int fb_swarm_loglik(fb_handle *h, int s, const double *theta, double *ll, int *i
     if (!h || !theta || !ll || !info) return fail(FB_E_ARG, "fb_swarm_loglik: null argument");
     if (s < 1 || s > h->max_swarm) return fail(FB_E_CAPACITY, "fb_swarm_loglik: swarm size out of range");
     const int P = h->prog.n_par;
-    if (h->use_dag) {
-        const int rc = dag_table(h, s);
-        if (rc) return rc;
-    }
     std::memcpy(h->h_theta, theta, sizeof(double) * P * s);
     FB_HIP(hipMemcpyAsync(h->d_theta, h->h_theta, sizeof(double) * P * s, hipMemcpyHostToDevice, h->stream));
     int err = 0;
     int rc = run_swarm(h, s, &err);
     if (rc) return rc;
-    if (err == 1) return fail(FB_E_HIP, "fb_swarm_loglik: a dependency wait of the persistent factorisation timed out");
+    if (err == 1) return fail(FB_E_HIP, "fb_swarm_loglik: the persistent factorisation found no ready task for 2 s");
     if (err == 2) {
         // some XCD queue found no work-group of its own: this device does not
         // dispatch the way the persistent schedule assumes; use lock-step

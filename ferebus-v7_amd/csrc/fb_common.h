@@ -58,7 +58,7 @@ constexpr double kTwoPi = 2.0 * kPi;
 constexpr int kNBO = 128;     // panel width of the trailing update
 constexpr int kNBI = 64;      // diagonal block order (potrf / trsm)
 constexpr int kMB = 64;       // rows of one update tile
-constexpr int kKC = 16;       // depth of one LDS stage of the update GEMM
+constexpr int kKC = 32;       // depth of one LDS stage of the update GEMM
 constexpr int kThreads = 256; // work-group size of every kernel (4 wave64)
 constexpr int kDinv = 4 * 16 * 16;   // doubles per 64x64 diagonal block: four 16x16 inverses
 
@@ -93,19 +93,21 @@ void launch_likelihood(const SwarmBuffers &b, hipStream_t st);
 
 // Persistent task-graph factorisation (fb_dag.hip).
 constexpr int kDagMaxQueues = 16;
-constexpr int kDagCtl = 2 * kDagMaxQueues + 2;   // heads, owners, finished work-groups, error
-struct DagTask {
-    short type, s, a, i;
-};
-struct DagSchedule {
-    int n_queues = 0;
-    std::vector<DagTask> tasks;   // queue q = tasks[qoff[q], qoff[q + 1])
-    std::vector<int> qoff;
+constexpr int kDagCtl = (6 * kDagMaxQueues + 2) * 32;   // control words, one per 128-byte line
+constexpr int kDagErrorWord = kDagCtl - 32;              // 0 ok, 1 no ready task for 2 s, 2 queue not drained
+struct DagWork {
+    int *ctl = nullptr;                  // kDagCtl control words
+    int *dep = nullptr;                  // max_swarm x dag_index_count(np) counters
+    const int *dep0 = nullptr;           // dag_index_count(np) initial counters
+    unsigned long long *slots = nullptr; // ready lists
+    int *dead = nullptr;                 // max_swarm
 };
 int dag_grid_size(int *n_xcd);
-DagSchedule dag_schedule(int S, int np, int n_queues);
-void launch_dag(const SwarmBuffers &b, const DagTask *tasks, const std::vector<int> &qoff, int *cnt, int *ctl,
-                int grid, hipStream_t st);
+int dag_queues(int S, int n_xcd);
+int dag_index_count(int np);
+int dag_slot_capacity(int S, int np, int Q);
+std::vector<int> dag_initial_counts(int np);
+void launch_dag(const SwarmBuffers &b, const DagWork &w, int n_queues, int grid, hipStream_t st);
 void launch_backsolve(const SwarmBuffers &b, int particle, double *alpha, hipStream_t st);
 
 }  // namespace fb

@@ -2,21 +2,41 @@
 //
 // Same arithmetic as the lock-step kernels (fb_tiles.h), scheduled as a task
 // graph instead of one launch per step: a fixed grid of resident work-groups
-// pulls tasks from per-XCD queues in topological order and waits, per task,
-// on the progress counters of the one matrix the task belongs to.  Every
-// matrix of the swarm advances at its own pace, so the latency-bound diagonal
-// factorisations and panel solves of one matrix overlap the matrix-core
-// trailing updates of the others, with no launch boundaries in between.
+// takes ready tasks from per-XCD queues, runs them, and makes their
+// dependants ready.  Every matrix of the swarm advances at its own pace, so
+// the latency-bound diagonal factorisations and panel solves of one matrix
+// overlap the matrix-core trailing updates of the others, with no launch
+// boundaries in between.
 //
 // Tasks of matrix s (64-row tiles i, 64-column blocks b, 128-column panels p
-// made of blocks b0 = 2p, b1 = 2p + 1):
-//   TG (p, i)   C(i, panel p) -= L(i, 0:128p) L(panel rows, 0:128p)^T    p >= 1
-//   TP (b)      factor the diagonal block b (INFO, partial log-det)
-//   TT (b, i)   L(i, b) = C(i, b) L(b, b)^{-T}                            i > b
-//   TG2(p, i)   C(i, b1) -= L(i, b0) L(b1, b0)^T                          i >= b1
-// Progress counters of matrix s: rowfin[i] = blocks of row tile i final,
-// gdone[i] / g2done[i] = last panel whose TG / TG2 of row tile i finished,
-// dead = a diagonal block was not positive definite (dependants skip).
+// made of blocks b0 = 2p, b1 = 2p + 1, nb = np / 64 tiles, P = nb / 2):
+//   TG (p, i)   C(i, panel p) -= L(i, 0:128p) L(panel rows, 0:128p)^T   1 <= p, 2p <= i
+//               (block b0 only for the diagonal row i = b0)
+//   TP (b)      factor the diagonal block b (INFO, partial log-det, the
+//               inverses of its 16x16 diagonal blocks)
+//   TT (b, i)   L(i, b) = C(i, b) L(b, b)^{-T}                           b < i
+//   TG2(p, i)   C(i, b1) -= L(i, b0) L(b1, b0)^T                         b1 <= i
+// Data-flow dependencies (each task has at most three):
+//   TG (p, i)  <- TT(b0-1, i), TT(b0-1, b0), TT(b0-1, b1)
+//   TP (b0)    <- TG(p, b0) (p >= 1);          TP(b1) <- TG2(p, b1)
+//   TT (b, i)  <- TP(b), and TG(p, i) (b = b0, p >= 1) or TG2(p, i) (b = b1)
+//   TG2(p, i)  <- TT(b0, i), TT(b0, b1)
+//
+// Scheduling.  Each task of each matrix has a counter holding its number of
+// unfinished predecessors.  A work-group that finishes a task decrements the
+// counters of the task's dependants (one task, or one row of tasks) and pushes
+// every dependant that reaches zero onto its queue's ready list -- the
+// high-priority list for the chain that bounds a matrix (diagonal blocks and
+// the rows of the next panel's diagonal blocks), the normal list otherwise.
+// Every task is pushed exactly once and the split between the lists is
+// static, so the lists are flat arrays (no wrap) whose final lengths are
+// known.  A work-group claims the filled head slot of the high list with a
+// compare-and-swap whenever there is one; otherwise it holds one ticket on
+// the normal list (fetch-and-add, always succeeds) and runs that slot's task
+// once the slot is filled.
+// Device-scope atomics go to memory on this chip, so every control word
+// sits on its own cache line and polls back off.  TP(0) of every matrix is
+// ready at the start.
 //
 // XCD pinning.  gfx950 has 8 accelerator dies (XCDs), each with its own L2;
 // the L2s are not coherent with each other inside a kernel.  Matrix s belongs
@@ -31,13 +51,16 @@
 // stale tiles are read.)  Kernel boundaries write L2 back and invalidate it
 // as usual, so the build before and the likelihood after see coherent data.
 //
-// Deadlock freedom: a task only waits on tasks of its own matrix that precede
-// it in its queue, and a task is taken only by a running work-group, so by
-// induction over the queue every taken task finishes.  Waits are bounded in
-// wall-clock time; a wait that exceeds the bound raises the error flag
-// instead of hanging.  The last work-group to leave checks that every queue
-// was drained (error 2 otherwise, e.g. a queue no XCD could claim), and the
-// host then evaluates the swarm on the lock-step path.
+// Termination: a work-group leaves once the high list is claimed to its final
+// length and its ticket is at or beyond the final length of the normal list.  Every slot
+// below the final length is filled eventually (by induction over the task
+// graph a ready task always exists while tasks remain and none runs), so no
+// wait is endless; waits are nevertheless bounded in wall-clock time (the
+// error flag is raised instead of hanging).  A matrix whose diagonal block is
+// not positive definite is marked dead: its remaining tasks skip their work
+// but still release their dependants.  The last work-group out checks that
+// every queue was drained (error 2 otherwise, e.g. a queue no XCD could
+// claim), and the host then evaluates the swarm on the lock-step path.
 #include <algorithm>
 #include <vector>
 
@@ -45,33 +68,63 @@
 
 namespace fb {
 
-enum DagTaskType : short { kTaskTG = 0, kTaskTP = 1, kTaskTT = 2, kTaskTG2 = 3 };
+enum DagTaskType : int { kTaskTG = 0, kTaskTP = 1, kTaskTT = 2, kTaskTG2 = 3 };
+
+// Control words, one per 128-byte line (kCtlLine ints): per queue q the
+// words ctl_word(q, k) for k = high head, high tail, normal head, normal
+// tail, owner XCD (one spare line); then finished work-groups and error.
+constexpr int kCtlLine = 32;
+enum : int { kHiHead = 0, kHiTail = 1, kLoHead = 2, kLoTail = 3, kOwner = 4, kCtlPerQueue = 6 };
+__host__ __device__ constexpr int ctl_word(int q, int k) { return (q * kCtlPerQueue + k) * kCtlLine; }
+constexpr int kCtlFinished = ctl_word(kDagMaxQueues, 0);
+constexpr int kCtlError = kCtlFinished + kCtlLine;
+static_assert(kCtlError + kCtlLine == kDagCtl && kCtlError == kDagErrorWord, "control word layout");
 
 struct DagArgs {
     double *K;
-    int np, n, nb, S;
-    const DagTask *tasks;
-    int n_queues;
-    int qoff[kDagMaxQueues + 1];   // queue q = tasks[qoff[q], qoff[q + 1])
-    int *ctl;           // heads[kDagMaxQueues], owners[kDagMaxQueues], finished work-groups, error
-    int *cnt;           // S x (3 nb + 1) progress counters
+    int np, n, nb, S, Q;
+    int tpm;            // tasks per matrix
+    int hpm;            // of them on the high list
+    int ntm;            // task index space per matrix (dag_index)
+    int slot_cap;       // slots per list (tasks of the largest queue)
+    int *ctl;
+    int *dep;           // S x ntm unfinished-predecessor counters
+    const int *dep0;    // ntm initial counters
+    unsigned long long *slots;   // Q x 2 lists x slot_cap
+    int *dead;          // S
     int *info;
     double *logdet_part;
     double *dinv;       // S x nb x kDinv diagonal-block inverses
 };
 
-constexpr unsigned long long kWaitTicks = 200000000ull;   // 2 s of the 100 MHz real-time clock
-
-// Counters are read and written coherently (device-scope atomics); the tile
-// data they guard stays in the XCD's L2 (see the XCD pinning note above).
-__device__ inline int ld_relaxed(const int *p)
+// Dense index of a task within its matrix.
+__host__ __device__ inline int dag_index(int type, int a, int i, int nb)
 {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int P = nb / 2;
+    switch (type) {
+    case kTaskTG: return a * nb + i;
+    case kTaskTP: return P * nb + a;
+    case kTaskTT: return P * nb + nb + a * nb + i;
+    default: return P * nb + nb + nb * nb + a * nb + i;
+    }
 }
 
-__device__ inline void st_relaxed(int *p, int v)
+__host__ __device__ inline int dag_index_space(int nb) { return 2 * nb * nb + nb; }
+
+// A ready-list entry: type + 1 (never zero), matrix, a, i.
+__device__ inline unsigned long long dag_encode(int type, int s, int a, int i)
 {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return (unsigned long long)(type + 1) | ((unsigned long long)s << 8) | ((unsigned long long)a << 24) |
+           ((unsigned long long)i << 40);
+}
+
+__device__ inline int ld_relaxed(const int *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+
+__device__ inline void st_relaxed(int *p, int v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+
+__device__ inline unsigned long long ld_slot(const unsigned long long *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // This wave's global stores have reached the XCD's L2.
@@ -87,46 +140,87 @@ __device__ inline int xcc_id()
     return (int)(__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 0xf);
 }
 
-// Waits until c1 >= t1 and c2 >= t2 and c3 >= t3 (null pointers are
-// satisfied).  Returns false when the matrix is dead or the wait timed out.
-__device__ inline bool dag_wait(const DagArgs &a, const int *dead, const int *c1, int t1, const int *c2,
-                                int t2, const int *c3, int t3, int *flag)
+constexpr unsigned long long kWaitTicks = 200000000ull;   // 2 s of the 100 MHz real-time clock
+
+__host__ __device__ inline bool dag_is_high(int type, int a, int i)
 {
-    int *error = a.ctl + 2 * kDagMaxQueues + 1;
-    if (threadIdx.x == 0) {
-        int ok = 1;
-        const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
-        for (;;) {
-            if (ld_relaxed(dead) || ld_relaxed(error)) {
-                ok = 0;
-                break;
-            }
-            if ((!c1 || ld_relaxed(c1) >= t1) && (!c2 || ld_relaxed(c2) >= t2) && (!c3 || ld_relaxed(c3) >= t3))
-                break;
-            if (__builtin_amdgcn_s_memrealtime() - t0 > kWaitTicks) {
-                // abort every other wait quickly: the launch reports failure
-                st_relaxed(error, 1);
-                ok = 0;
-                break;
-            }
-            __builtin_amdgcn_s_sleep(2);
-        }
-        *flag = ok;
-    }
-    // every wave drops stale L1 copies of the producers' tiles before the
-    // barrier; all of the task's loads come after it
-    xcd_acquire();
-    __syncthreads();
-    return *flag != 0;
+    const int panel = type == kTaskTT ? a / 2 : a;
+    return type == kTaskTP || i <= 2 * panel + 3;
 }
 
-// Publishes this work-group's results: every wave's stores reach L2, then one
-// thread advances the counter.
-__device__ inline void dag_signal(int *c, int v)
+// Sleeps between polls, doubling up to ~2 us.
+__device__ inline void backoff(int &sl)
 {
-    xcd_release();
-    __syncthreads();
-    if (threadIdx.x == 0) st_relaxed(c, v);
+    if (sl <= 1) __builtin_amdgcn_s_sleep(1);
+    else if (sl <= 2) __builtin_amdgcn_s_sleep(2);
+    else if (sl <= 4) __builtin_amdgcn_s_sleep(4);
+    else if (sl <= 8) __builtin_amdgcn_s_sleep(8);
+    else if (sl <= 16) __builtin_amdgcn_s_sleep(16);
+    else __builtin_amdgcn_s_sleep(32);
+    sl = sl < 32 ? 2 * sl : 32;
+}
+
+// Thread 0: the next task of queue q -- the high list's head slot when it
+// is filled (claimed by compare-and-swap), else the slot of this
+// work-group's ticket on the normal list once it is filled (*ticket < 0: no
+// ticket held yet).  Returns 0 when both lists are exhausted or on error.
+__device__ inline unsigned long long dag_pop(const DagArgs &a, int q, int n_hi, int n_lo, int *ticket)
+{
+    int *error = a.ctl + kCtlError;
+    int *hi_head = a.ctl + ctl_word(q, kHiHead);
+    const unsigned long long *hi = a.slots + (size_t)q * 2 * a.slot_cap;
+    const unsigned long long *lo = hi + a.slot_cap;
+    unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+    int sl = 1;
+    for (int k = 0;; ++k) {
+        const int hh = ld_relaxed(hi_head);
+        if (hh < n_hi) {
+            const unsigned long long v = ld_slot(hi + hh);
+            if (v != 0) {
+                int expected = hh;
+                if (__hip_atomic_compare_exchange_strong(hi_head, &expected, hh + 1, __ATOMIC_RELAXED,
+                                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                    return v;
+                t0 = __builtin_amdgcn_s_memrealtime();   // another work-group made progress
+                sl = 1;
+                continue;
+            }
+        }
+        if (*ticket < 0) *ticket = atomicAdd(a.ctl + ctl_word(q, kLoHead), 1);
+        if (*ticket < n_lo) {
+            const unsigned long long v = ld_slot(lo + *ticket);
+            if (v != 0) {
+                *ticket = -1;
+                return v;
+            }
+        } else if (hh >= n_hi) {
+            return 0;
+        }
+        if ((k & 15) == 15) {
+            if (ld_relaxed(error)) return 0;
+            if (__builtin_amdgcn_s_memrealtime() - t0 > kWaitTicks) {
+                st_relaxed(error, 1);
+                return 0;
+            }
+        }
+        backoff(sl);
+    }
+}
+
+// Decrements the counters of dependants (type, a, i) for i in [lo, hi) of
+// matrix s and pushes those that become ready.  Work-group cooperative.
+__device__ inline void dag_release(const DagArgs &a, int q, int s, int type, int da, int lo, int hi)
+{
+    int *dep = a.dep + (size_t)s * a.ntm;
+    unsigned long long *lists = a.slots + (size_t)q * 2 * a.slot_cap;
+    for (int i = lo + (int)threadIdx.x; i < hi; i += kThreads) {
+        if (atomicSub(dep + dag_index(type, da, i, a.nb), 1) == 1) {
+            const int l = dag_is_high(type, da, i) ? 0 : 1;
+            const int slot = atomicAdd(a.ctl + ctl_word(q, l == 0 ? kHiTail : kLoTail), 1);
+            __hip_atomic_store(lists + (size_t)l * a.slot_cap + slot, dag_encode(type, s, da, i), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
 }
 
 constexpr int kDagSmem = TileShape<kMB, kNBO>::SMEM > kPotrfSmem ? TileShape<kMB, kNBO>::SMEM : kPotrfSmem;
@@ -134,122 +228,166 @@ constexpr int kDagSmem = TileShape<kMB, kNBO>::SMEM > kPotrfSmem ? TileShape<kMB
 __global__ void __launch_bounds__(kThreads) dag_factor_kernel(DagArgs a)
 {
     __shared__ __attribute__((aligned(16))) double smem[kDagSmem];
-    __shared__ int sh_task, sh_flag, sh_queue;
+    __shared__ unsigned long long sh_task;
+    __shared__ int sh_queue;
     const int tid = threadIdx.x;
-    const int stride = 3 * a.nb + 1;
-    int *heads = a.ctl, *owners = a.ctl + kDagMaxQueues;
+    const int P = a.nb / 2;
     if (tid == 0) {
         const int x = xcc_id();
-        const int q = x % a.n_queues;
-        const int own = atomicCAS(owners + q, -1, x);
+        const int q = x % a.Q;
+        const int own = atomicCAS(a.ctl + ctl_word(q, kOwner), -1, x);
         sh_queue = (own == -1 || own == x) ? q : -1;
     }
     __syncthreads();
     const int q = sh_queue;
-    const int q_tasks = q >= 0 ? a.qoff[q + 1] - a.qoff[q] : 0;
-    for (;;) {
-        if (tid == 0) sh_task = q >= 0 ? atomicAdd(heads + q, 1) : q_tasks;
+    const int nmat = q >= 0 ? (a.S - q + a.Q - 1) / a.Q : 0;
+    const int n_hi = nmat * a.hpm, n_lo = nmat * (a.tpm - a.hpm);
+    int ticket = -1;   // thread 0's ticket on the normal list
+    while (q >= 0) {
+        if (tid == 0) sh_task = dag_pop(a, q, n_hi, n_lo, &ticket);
         __syncthreads();
-        const int t = sh_task;
+        const unsigned long long v = sh_task;
+        if (v == 0) break;
+        const int type = (int)(v & 0xff) - 1;
+        const int s = (int)((v >> 8) & 0xffff);
+        const int ta = (int)((v >> 24) & 0xffff);
+        const int ti = (int)((v >> 40) & 0xffff);
+        // every wave drops stale L1 copies of the producers' tiles; all of the
+        // task's loads come after the barrier
+        xcd_acquire();
         __syncthreads();
-        if (t >= q_tasks) break;
-        const DagTask task = a.tasks[a.qoff[q] + t];
-        const int s = task.s;
         double *A = a.K + (size_t)s * a.np * a.np;
-        int *rowfin = a.cnt + (size_t)s * stride;
-        int *gdone = rowfin + a.nb;
-        int *g2done = gdone + a.nb;
-        int *dead = g2done + a.nb;
-        if (task.type == kTaskTG) {
-            const int p = task.a, i = task.i, b0 = 2 * p, b1 = 2 * p + 1;
-            if (!dag_wait(a, dead, rowfin + i, b0, rowfin + b0, b0, rowfin + b1, b0, &sh_flag)) continue;
-            gemm_tile_update<kMB, kNBO>(A, a.np, i * kNBI, p * kNBO, 0, p * kNBO, smem);
-            dag_signal(gdone + i, p + 1);
-        } else if (task.type == kTaskTP) {
-            const int b = task.a, p = b / 2;
-            const int *dep = (b & 1) ? g2done + b : (p > 0 ? gdone + b : nullptr);
-            if (!dag_wait(a, dead, dep, p + 1, nullptr, 0, nullptr, 0, &sh_flag)) continue;
-            double lg = 0.0;
-            const int fail = potrf64_block(A, a.np, a.n, b * kNBI, &lg, smem,
-                                           a.dinv + ((size_t)s * a.nb + b) * kDinv);
-            if (fail) {
-                if (tid == 0) {
-                    a.info[s] = fail;
-                    st_relaxed(dead, 1);
-                }
-                __syncthreads();
-                continue;
+        const bool live = ld_relaxed(a.dead + s) == 0;
+        int rtype = kTaskTT, ra = 0, rlo = 0, rhi = 0;   // dependants to release
+        if (type == kTaskTG) {
+            const int p = ta, i = ti;
+            if (live) {
+                if (i == 2 * p)   // only the lower 64 x 64 block of the diagonal row is used
+                    gemm_tile_update<kMB, kNBI>(A, a.np, i * kNBI, p * kNBO, 0, p * kNBO, smem);
+                else
+                    gemm_tile_update<kMB, kNBO>(A, a.np, i * kNBI, p * kNBO, 0, p * kNBO, smem);
             }
-            if (tid == 0) a.logdet_part[(size_t)s * a.nb + b] = lg;
-            dag_signal(rowfin + b, b + 1);
-        } else if (task.type == kTaskTT) {
-            const int b = task.a, i = task.i, p = b / 2;
-            const int *dep = (b & 1) ? g2done + i : (p > 0 ? gdone + i : nullptr);
-            if (!dag_wait(a, dead, rowfin + b, b + 1, dep, p + 1, nullptr, 0, &sh_flag)) continue;
-            trsm64_mfma(A, a.np, b * kNBI, i * kNBI, a.dinv + ((size_t)s * a.nb + b) * kDinv);
-            dag_signal(rowfin + i, b + 1);
+            if (i == 2 * p) { rtype = kTaskTP; ra = i; rlo = 0; rhi = 1; }
+            else { rtype = kTaskTT; ra = 2 * p; rlo = i; rhi = i + 1; }
+        } else if (type == kTaskTP) {
+            const int b = ta;
+            if (live) {
+                double lg = 0.0;
+                const int fail = potrf64_block(A, a.np, a.n, b * kNBI, &lg, smem,
+                                               a.dinv + ((size_t)s * a.nb + b) * kDinv);
+                if (tid == 0) {
+                    if (fail) {
+                        a.info[s] = fail;
+                        st_relaxed(a.dead + s, 1);
+                    } else {
+                        a.logdet_part[(size_t)s * a.nb + b] = lg;
+                    }
+                }
+            }
+            rtype = kTaskTT; ra = b; rlo = b + 1; rhi = a.nb;
+        } else if (type == kTaskTT) {
+            const int b = ta, i = ti, p = b / 2;
+            if (live) trsm64_mfma(A, a.np, b * kNBI, i * kNBI, a.dinv + ((size_t)s * a.nb + b) * kDinv);
+            if ((b & 1) == 0) {
+                rtype = kTaskTG2; ra = p;
+                if (i == b + 1) { rlo = b + 1; rhi = a.nb; } else { rlo = i; rhi = i + 1; }
+            } else if (p + 1 < P) {
+                rtype = kTaskTG; ra = p + 1;
+                if (i == b + 1 || i == b + 2) { rlo = b + 1; rhi = a.nb; } else { rlo = i; rhi = i + 1; }
+            }
         } else {
-            const int p = task.a, i = task.i, b0 = 2 * p, b1 = 2 * p + 1;
-            if (!dag_wait(a, dead, rowfin + i, b0 + 1, rowfin + b1, b0 + 1, nullptr, 0, &sh_flag)) continue;
-            gemm_tile_update<kMB, kNBI>(A, a.np, i * kNBI, b1 * kNBI, b0 * kNBI, b1 * kNBI, smem);
-            dag_signal(g2done + i, p + 1);
+            const int p = ta, i = ti, b0 = 2 * p, b1 = 2 * p + 1;
+            if (live) gemm_tile_update<kMB, kNBI>(A, a.np, i * kNBI, b1 * kNBI, b0 * kNBI, b1 * kNBI, smem);
+            if (i == b1) { rtype = kTaskTP; ra = b1; rlo = 0; rhi = 1; }
+            else { rtype = kTaskTT; ra = b1; rlo = i; rhi = i + 1; }
         }
+        // publish: every wave's stores reach L2, then the dependants are released
+        xcd_release();
+        __syncthreads();
+        dag_release(a, q, s, rtype, ra, rlo, rhi);
     }
     // the last work-group out checks that every queue was drained
     if (tid == 0) {
-        int *finished = a.ctl + 2 * kDagMaxQueues;
-        if (atomicAdd(finished, 1) == (int)gridDim.x - 1) {
-            for (int k = 0; k < a.n_queues; ++k)
-                if (ld_relaxed(heads + k) < a.qoff[k + 1] - a.qoff[k]) st_relaxed(a.ctl + 2 * kDagMaxQueues + 1, 2);
+        if (atomicAdd(a.ctl + kCtlFinished, 1) == (int)gridDim.x - 1) {
+            for (int k = 0; k < a.Q; ++k) {
+                const int m = (a.S - k + a.Q - 1) / a.Q;
+                if (ld_relaxed(a.ctl + ctl_word(k, kHiHead)) < m * a.hpm ||
+                    ld_relaxed(a.ctl + ctl_word(k, kLoHead)) < m * (a.tpm - a.hpm))
+                    st_relaxed(a.ctl + kCtlError, 2);
+            }
         }
     }
 }
 
-__global__ void dag_reset_kernel(int *cnt, int n, int *ctl)
+// Fresh counters, empty lists holding only TP(0) of every matrix, control words.
+__global__ void dag_reset_kernel(DagArgs a)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) cnt[i] = 0;
-    if (i < kDagCtl) ctl[i] = (i >= kDagMaxQueues && i < 2 * kDagMaxQueues) ? -1 : 0;
-}
-
-// Task table of queue q (matrices s = q, q + Q, ...) of np x np in
-// topological order.  The chain that limits one matrix (diagonal tile update
-// -> factor -> next diagonal row) is listed first in every panel, the bulk of
-// the panel after it.
-static void dag_queue_tasks(int S, int np, int q, int Q, std::vector<DagTask> &t)
-{
-    const int nb = np / kNBI;
-    auto add = [&](short type, int a, int i) {
-        for (int s = q; s < S; s += Q) t.push_back(DagTask{type, (short)s, (short)a, (short)i});
-    };
-    for (int p = 0; p < nb / 2; ++p) {
-        const int b0 = 2 * p, b1 = 2 * p + 1;
-        if (p > 0) add(kTaskTG, p, b0);
-        add(kTaskTP, b0, 0);
-        if (p > 0) add(kTaskTG, p, b1);
-        add(kTaskTT, b0, b1);
-        add(kTaskTG2, p, b1);
-        add(kTaskTP, b1, 0);
-        for (int i = b1 + 1; i < nb; ++i) {
-            if (p > 0) add(kTaskTG, p, i);
+    const size_t n_dep = (size_t)a.S * a.ntm;
+    const size_t n_slot = (size_t)a.Q * 2 * a.slot_cap;
+    const size_t total = n_dep + n_slot + a.S + kDagCtl;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        if (e < n_dep) {
+            a.dep[e] = a.dep0[e % a.ntm];
+        } else if (e < n_dep + n_slot) {
+            const size_t k = e - n_dep;
+            const int q = (int)(k / (2 * (size_t)a.slot_cap));
+            const int l = (int)((k / a.slot_cap) % 2);
+            const int idx = (int)(k % a.slot_cap);
+            const int s = q + idx * a.Q;
+            a.slots[k] = (l == 0 && s < a.S) ? dag_encode(kTaskTP, s, 0, 0) : 0ull;
+        } else if (e < n_dep + n_slot + a.S) {
+            a.dead[e - n_dep - n_slot] = 0;
+        } else {
+            const int w = (int)(e - n_dep - n_slot - a.S);
+            const int q = w / (kCtlPerQueue * kCtlLine), k = (w / kCtlLine) % kCtlPerQueue;
+            int v = 0;
+            if (q < a.Q && k == kHiTail) v = (a.S - q + a.Q - 1) / a.Q;   // the TP(0)s
+            if (q < kDagMaxQueues && k == kOwner) v = -1;
+            a.ctl[w] = v;
         }
-        for (int i = b1 + 1; i < nb; ++i) add(kTaskTT, b0, i);
-        for (int i = b1 + 1; i < nb; ++i) add(kTaskTG2, p, i);
-        for (int i = b1 + 1; i < nb; ++i) add(kTaskTT, b1, i);
     }
 }
 
-DagSchedule dag_schedule(int S, int np, int n_queues)
+int dag_high_tasks_per_matrix(int np)
 {
-    DagSchedule d;
-    d.n_queues = std::max(1, std::min(std::min(n_queues, kDagMaxQueues), S));
-    d.qoff.push_back(0);
-    for (int q = 0; q < d.n_queues; ++q) {
-        dag_queue_tasks(S, np, q, d.n_queues, d.tasks);
-        d.qoff.push_back((int)d.tasks.size());
-    }
+    const int nb = np / kNBI, P = nb / 2;
+    int t = 0;
+    for (int p = 1; p < P; ++p)
+        for (int i = 2 * p; i < nb; ++i) t += dag_is_high(kTaskTG, p, i);
+    for (int b = 0; b < nb; ++b) t += dag_is_high(kTaskTP, b, 0);
+    for (int b = 0; b < nb; ++b)
+        for (int i = b + 1; i < nb; ++i) t += dag_is_high(kTaskTT, b, i);
+    for (int p = 0; p < P; ++p)
+        for (int i = 2 * p + 1; i < nb; ++i) t += dag_is_high(kTaskTG2, p, i);
+    return t;
+}
+
+int dag_tasks_per_matrix(int np)
+{
+    const int nb = np / kNBI, P = nb / 2;
+    int t = nb + nb * (nb - 1) / 2;   // TP, TT
+    for (int p = 1; p < P; ++p) t += nb - 2 * p;        // TG
+    for (int p = 0; p < P; ++p) t += nb - 2 * p - 1;    // TG2
+    return t;
+}
+
+std::vector<int> dag_initial_counts(int np)
+{
+    const int nb = np / kNBI, P = nb / 2;
+    std::vector<int> d(dag_index_space(nb), 0);
+    for (int p = 1; p < P; ++p)
+        for (int i = 2 * p; i < nb; ++i) d[dag_index(kTaskTG, p, i, nb)] = (i <= 2 * p + 1) ? 2 : 3;
+    for (int b = 0; b < nb; ++b) d[dag_index(kTaskTP, b, 0, nb)] = b == 0 ? 0 : 1;
+    for (int b = 0; b < nb; ++b)
+        for (int i = b + 1; i < nb; ++i) d[dag_index(kTaskTT, b, i, nb)] = b == 0 ? 1 : 2;
+    for (int p = 0; p < P; ++p)
+        for (int i = 2 * p + 1; i < nb; ++i) d[dag_index(kTaskTG2, p, i, nb)] = (i == 2 * p + 1) ? 1 : 2;
     return d;
 }
+
+int dag_index_count(int np) { return dag_index_space(np / kNBI); }
+
+int dag_slot_capacity(int S, int np, int Q) { return ((S + Q - 1) / Q) * dag_tasks_per_matrix(np); }
 
 int dag_grid_size(int *n_xcd)
 {
@@ -263,20 +401,22 @@ int dag_grid_size(int *n_xcd)
     return cus * std::max(per_cu, 1);
 }
 
-void launch_dag(const SwarmBuffers &b, const DagTask *tasks, const std::vector<int> &qoff, int *cnt, int *ctl,
-                int grid, hipStream_t st)
+int dag_queues(int S, int n_xcd) { return std::max(1, std::min(std::min(n_xcd, kDagMaxQueues), S)); }
+
+void launch_dag(const SwarmBuffers &b, const DagWork &w, int n_queues, int grid, hipStream_t st)
 {
-    const int nb = b.np / kNBI;
-    const int ncnt = b.S * (3 * nb + 1);
-    hipLaunchKernelGGL(dag_reset_kernel, dim3(ceil_div(std::max(ncnt, kDagCtl), 256)), dim3(256), 0, st, cnt, ncnt,
-                       ctl);
     DagArgs a;
-    a.K = b.K; a.np = b.np; a.n = b.n; a.nb = nb; a.S = b.S;
-    a.tasks = tasks;
-    a.n_queues = (int)qoff.size() - 1;
-    for (int q = 0; q <= kDagMaxQueues; ++q) a.qoff[q] = q < (int)qoff.size() ? qoff[q] : qoff.back();
-    a.ctl = ctl; a.cnt = cnt;
+    a.K = b.K; a.np = b.np; a.n = b.n; a.nb = b.np / kNBI; a.S = b.S;
+    a.Q = dag_queues(b.S, n_queues);
+    a.tpm = dag_tasks_per_matrix(b.np);
+    a.hpm = dag_high_tasks_per_matrix(b.np);
+    a.ntm = dag_index_space(a.nb);
+    a.slot_cap = dag_slot_capacity(b.S, b.np, a.Q);
+    a.ctl = w.ctl; a.dep = w.dep; a.dep0 = w.dep0; a.slots = w.slots; a.dead = w.dead;
     a.info = b.info; a.logdet_part = b.logdet_part; a.dinv = b.dinv;
+    const size_t total = (size_t)a.S * a.ntm + (size_t)a.Q * 2 * a.slot_cap + a.S + kDagCtl;
+    const int rgrid = (int)std::min<size_t>(1024, (total + 255) / 256);
+    hipLaunchKernelGGL(dag_reset_kernel, dim3(rgrid), dim3(256), 0, st, a);
     hipLaunchKernelGGL(dag_factor_kernel, dim3(std::max(1, grid)), dim3(kThreads), 0, st, a);
 }
 

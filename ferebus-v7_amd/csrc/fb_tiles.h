@@ -20,8 +20,9 @@ __device__ inline d4 mfma_f64(double a, double b, d4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-// Work-group tile of MB rows x NB columns computed by 4 waves.
-template <int MB, int NB>
+// Work-group tile of MB rows x NB columns computed by 4 waves, K streamed in
+// chunks of KC through two LDS stages.
+template <int MB, int NB, int KC = kKC>
 struct TileShape {
     static constexpr int WAVES_N = (NB / 32 < 4) ? NB / 32 : 4;
     static constexpr int WAVES_M = 4 / WAVES_N;
@@ -31,163 +32,148 @@ struct TileShape {
     static constexpr int BN = WN / 16;
     static constexpr int LDA = MB + 16;   // LDS strides in doubles; = 16 mod 32
     static constexpr int LDB = NB + 16;   // keeps the two k rows of a fragment read
-    static constexpr int LDC = MB + 4;    // on different LDS banks
-    static constexpr int PIPE = 2 * kKC * (LDA + LDB);
-    static constexpr int EPI = NB * LDC;
-    static constexpr int SMEM = PIPE > EPI ? PIPE : EPI;
+    static constexpr int ASZ = KC * LDA;  // on different LDS banks
+    static constexpr int BSZ = KC * LDB;
+    static constexpr int SMEM = 2 * (ASZ + BSZ);
     static_assert(WAVES_M * WAVES_N == 4, "4 waves per tile");
     static_assert(BM >= 1 && BN >= 1, "wave tile at least 16x16");
-    static_assert((MB * kKC / 2) % kThreads == 0 && (NB * kKC / 2) % kThreads == 0, "loads");
+    static_assert(KC % 4 == 0 && (MB * KC / 2) % kThreads == 0 && (NB * KC / 2) % kThreads == 0, "loads");
 };
 
-// acc = sum_k A[r][k] B[c][k] over k < depth, A[r][k] = Ag[k*lda + r],
-// B[c][k] = Bg[k*ldb + c].  depth is a multiple of kKC.  LDS holds two
-// stages [A0 A1 B0 B1]; the next stage is fetched into registers while the
-// current one feeds the matrix cores.
-template <int MB, int NB>
-__device__ inline void tile_gemm(const double *__restrict__ Ag, int lda,
-                                 const double *__restrict__ Bg, int ldb, int depth,
-                                 double *__restrict__ smem,
-                                 d4 (&acc)[TileShape<MB, NB>::BM][TileShape<MB, NB>::BN])
+// C(row0:row0+MB, col0:col0+NB) -= A(row0.., 0:depth) B(col0.., 0:depth)^T
+// with A[r][k] = Ag[k*lda + r], B[c][k] = Bg[k*ldb + c] and C column major
+// (leading dimension ldc); depth is a multiple of KC (may be 0).
+//
+// The products are issued with the B panel as the first MFMA operand, so
+// the accumulator of a 16x16 block holds C^T: lane l, component v is
+// C[r0 + l%16][c0 + 4v + l/16] -- sixteen consecutive rows of four columns
+// per load, which coalesces.  C is therefore read into registers before the
+// K loop (its latency hidden behind the loop) and written back straight from
+// the accumulators: no LDS epilogue, no barrier after the loop.  Each chunk
+// of KC is fetched into registers while the previous one feeds the matrix
+// cores from LDS.
+template <int MB, int NB, int KC>
+__device__ inline void tile_gemm_sub(double *__restrict__ Cg, int ldc,
+                                     const double *__restrict__ Ag, int lda,
+                                     const double *__restrict__ Bg, int ldb, int depth,
+                                     double *__restrict__ smem)
 {
-    using T = TileShape<MB, NB>;
-    constexpr int NA = MB * kKC / 2 / kThreads;   // double2 loads per thread (A)
-    constexpr int NBL = NB * kKC / 2 / kThreads;  // double2 loads per thread (B)
-    constexpr int ASZ = kKC * T::LDA;
-    constexpr int BSZ = kKC * T::LDB;
+    using T = TileShape<MB, NB, KC>;
+    constexpr int NA = MB * KC / 2 / kThreads;   // double2 loads per thread (A)
+    constexpr int NBL = NB * KC / 2 / kThreads;  // double2 loads per thread (B)
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
     const int wrow = (wave / T::WAVES_N) * T::WM;
     const int wcol = (wave % T::WAVES_N) * T::WN;
+    const int lr = lane & 15, lq = lane >> 4;
 
-#pragma unroll
-    for (int i = 0; i < T::BM; ++i)
-#pragma unroll
-        for (int j = 0; j < T::BN; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
-    if (depth <= 0) return;
-
-    // per-thread load coordinates (fixed for every stage)
-    int a_off[NA], a_lds[NA], b_off[NBL], b_lds[NBL];
-#pragma unroll
-    for (int u = 0; u < NA; ++u) {
-        const int q = tid + u * kThreads;
-        const int col = q / (MB / 2), r2 = (q % (MB / 2)) * 2;
-        a_off[u] = col * lda + r2;
-        a_lds[u] = col * T::LDA + r2;
-    }
-#pragma unroll
-    for (int u = 0; u < NBL; ++u) {
-        const int q = tid + u * kThreads;
-        const int col = q / (NB / 2), r2 = (q % (NB / 2)) * 2;
-        b_off[u] = col * ldb + r2;
-        b_lds[u] = col * T::LDB + r2;
-    }
-    // staging registers as scalar pairs: arrays of double2 are not promoted
-    // to registers by the compiler and would live in scratch
-    double rax[NA], ray[NA], rbx[NBL], rby[NBL];
-#pragma unroll
-    for (int u = 0; u < NA; ++u) {
-        const double2 t = *reinterpret_cast<const double2 *>(Ag + a_off[u]);
-        rax[u] = t.x; ray[u] = t.y;
-    }
-#pragma unroll
-    for (int u = 0; u < NBL; ++u) {
-        const double2 t = *reinterpret_cast<const double2 *>(Bg + b_off[u]);
-        rbx[u] = t.x; rby[u] = t.y;
-    }
-#pragma unroll
-    for (int u = 0; u < NA; ++u) *reinterpret_cast<double2 *>(smem + a_lds[u]) = double2{rax[u], ray[u]};
-#pragma unroll
-    for (int u = 0; u < NBL; ++u) *reinterpret_cast<double2 *>(smem + 2 * ASZ + b_lds[u]) = double2{rbx[u], rby[u]};
-    __syncthreads();
-
-    const int kq = lane >> 4, lr = lane & 15;
-    const int nchunks = depth / kKC;
-    for (int c = 0; c < nchunks; ++c) {
-        const int cur = c & 1;
-        const bool more = c + 1 < nchunks;
-        if (more) {
-            const size_t ka = (size_t)(c + 1) * kKC * lda, kb = (size_t)(c + 1) * kKC * ldb;
-#pragma unroll
-            for (int u = 0; u < NA; ++u) {
-                const double2 t = *reinterpret_cast<const double2 *>(Ag + ka + a_off[u]);
-                rax[u] = t.x; ray[u] = t.y;
-            }
-#pragma unroll
-            for (int u = 0; u < NBL; ++u) {
-                const double2 t = *reinterpret_cast<const double2 *>(Bg + kb + b_off[u]);
-                rbx[u] = t.x; rby[u] = t.y;
-            }
-        }
-        const double *as = smem + cur * ASZ;
-        const double *bs = smem + 2 * ASZ + cur * BSZ;
-#pragma unroll
-        for (int kk = 0; kk < kKC / 4; ++kk) {
-            double af[T::BM], bf[T::BN];
-#pragma unroll
-            for (int i = 0; i < T::BM; ++i) af[i] = as[(kk * 4 + kq) * T::LDA + wrow + i * 16 + lr];
-#pragma unroll
-            for (int j = 0; j < T::BN; ++j) bf[j] = bs[(kk * 4 + kq) * T::LDB + wcol + j * 16 + lr];
-#pragma unroll
-            for (int i = 0; i < T::BM; ++i)
-#pragma unroll
-                for (int j = 0; j < T::BN; ++j) acc[i][j] = mfma_f64(af[i], bf[j], acc[i][j]);
-        }
-        if (more) {
-            double *an = smem + (cur ^ 1) * ASZ;
-            double *bn = smem + 2 * ASZ + (cur ^ 1) * BSZ;
-#pragma unroll
-            for (int u = 0; u < NA; ++u) *reinterpret_cast<double2 *>(an + a_lds[u]) = double2{rax[u], ray[u]};
-#pragma unroll
-            for (int u = 0; u < NBL; ++u) *reinterpret_cast<double2 *>(bn + b_lds[u]) = double2{rbx[u], rby[u]};
-        }
-        __syncthreads();
-    }
-}
-
-// Writes acc into the LDS tile Cs[col * LDC + row] (tile-local coordinates).
-template <int MB, int NB>
-__device__ inline void acc_to_lds(double *Cs, d4 (&acc)[TileShape<MB, NB>::BM][TileShape<MB, NB>::BN])
-{
-    using T = TileShape<MB, NB>;
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int wrow = (wave / T::WAVES_N) * T::WM;
-    const int wcol = (wave % T::WAVES_N) * T::WN;
+    // C^T blocks: acc[i][j][v] = C[wrow + 16 i + lr][wcol + 16 j + 4 v + lq]
+    d4 acc[T::BM][T::BN];
 #pragma unroll
     for (int i = 0; i < T::BM; ++i)
 #pragma unroll
         for (int j = 0; j < T::BN; ++j)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int row = wrow + i * 16 + (lane >> 4) + 4 * v;
-                const int col = wcol + j * 16 + (lane & 15);
-                Cs[col * T::LDC + row] = acc[i][j][v];
+            for (int v = 0; v < 4; ++v)
+                acc[i][j][v] = Cg[(size_t)(wcol + 16 * j + 4 * v + lq) * ldc + wrow + 16 * i + lr];
+
+    if (depth > 0) {
+        // per-thread load coordinates (fixed for every stage)
+        int a_off[NA], a_lds[NA], b_off[NBL], b_lds[NBL];
+#pragma unroll
+        for (int u = 0; u < NA; ++u) {
+            const int q = tid + u * kThreads;
+            const int col = q / (MB / 2), r2 = (q % (MB / 2)) * 2;
+            a_off[u] = col * lda + r2;
+            a_lds[u] = col * T::LDA + r2;
+        }
+#pragma unroll
+        for (int u = 0; u < NBL; ++u) {
+            const int q = tid + u * kThreads;
+            const int col = q / (NB / 2), r2 = (q % (NB / 2)) * 2;
+            b_off[u] = col * ldb + r2;
+            b_lds[u] = col * T::LDB + r2;
+        }
+        // staging registers as scalar pairs: arrays of double2 are not
+        // promoted to registers by the compiler and would live in scratch
+        double rax[NA], ray[NA], rbx[NBL], rby[NBL];
+#pragma unroll
+        for (int u = 0; u < NA; ++u) {
+            const double2 t = *reinterpret_cast<const double2 *>(Ag + a_off[u]);
+            rax[u] = t.x; ray[u] = t.y;
+        }
+#pragma unroll
+        for (int u = 0; u < NBL; ++u) {
+            const double2 t = *reinterpret_cast<const double2 *>(Bg + b_off[u]);
+            rbx[u] = t.x; rby[u] = t.y;
+        }
+#pragma unroll
+        for (int u = 0; u < NA; ++u) *reinterpret_cast<double2 *>(smem + a_lds[u]) = double2{rax[u], ray[u]};
+#pragma unroll
+        for (int u = 0; u < NBL; ++u)
+            *reinterpret_cast<double2 *>(smem + 2 * T::ASZ + b_lds[u]) = double2{rbx[u], rby[u]};
+        __syncthreads();
+
+        const int nchunks = depth / KC;
+        for (int c = 0; c < nchunks; ++c) {
+            const int cur = c & 1;
+            const bool more = c + 1 < nchunks;
+            if (more) {
+                const size_t ka = (size_t)(c + 1) * KC * lda, kb = (size_t)(c + 1) * KC * ldb;
+#pragma unroll
+                for (int u = 0; u < NA; ++u) {
+                    const double2 t = *reinterpret_cast<const double2 *>(Ag + ka + a_off[u]);
+                    rax[u] = t.x; ray[u] = t.y;
+                }
+#pragma unroll
+                for (int u = 0; u < NBL; ++u) {
+                    const double2 t = *reinterpret_cast<const double2 *>(Bg + kb + b_off[u]);
+                    rbx[u] = t.x; rby[u] = t.y;
+                }
             }
+            const double *as = smem + cur * T::ASZ;
+            const double *bs = smem + 2 * T::ASZ + cur * T::BSZ;
+#pragma unroll
+            for (int kk = 0; kk < KC / 4; ++kk) {
+                double af[T::BM], bf[T::BN];
+#pragma unroll
+                for (int i = 0; i < T::BM; ++i) af[i] = -as[(kk * 4 + lq) * T::LDA + wrow + i * 16 + lr];
+#pragma unroll
+                for (int j = 0; j < T::BN; ++j) bf[j] = bs[(kk * 4 + lq) * T::LDB + wcol + j * 16 + lr];
+#pragma unroll
+                for (int i = 0; i < T::BM; ++i)
+#pragma unroll
+                    for (int j = 0; j < T::BN; ++j) acc[i][j] = mfma_f64(bf[j], af[i], acc[i][j]);
+            }
+            if (more) {
+                double *an = smem + (cur ^ 1) * T::ASZ;
+                double *bn = smem + 2 * T::ASZ + (cur ^ 1) * T::BSZ;
+#pragma unroll
+                for (int u = 0; u < NA; ++u) *reinterpret_cast<double2 *>(an + a_lds[u]) = double2{rax[u], ray[u]};
+#pragma unroll
+                for (int u = 0; u < NBL; ++u) *reinterpret_cast<double2 *>(bn + b_lds[u]) = double2{rbx[u], rby[u]};
+            }
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < T::BM; ++i)
+#pragma unroll
+        for (int j = 0; j < T::BN; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                Cg[(size_t)(wcol + 16 * j + 4 * v + lq) * ldc + wrow + 16 * i + lr] = acc[i][j][v];
 }
 
 // C(row0:row0+MB, col0:col0+NB) -= L(row0.., k0:k1) L(col0.., k0:k1)^T
 // for the matrix at A (trailing update of one tile).
-template <int MB, int NB>
+template <int MB, int NB, int KC = kKC>
 __device__ inline void gemm_tile_update(double *__restrict__ A, int np, int row0, int col0, int k0, int k1,
                                         double *__restrict__ smem)
 {
-    using T = TileShape<MB, NB>;
-    d4 acc[T::BM][T::BN];
-    tile_gemm<MB, NB>(A + (size_t)k0 * np + row0, np, A + (size_t)k0 * np + col0, np, k1 - k0, smem, acc);
-    acc_to_lds<MB, NB>(smem, acc);
-    __syncthreads();
-    double *Ct = A + (size_t)col0 * np + row0;
-    for (int q = threadIdx.x; q < MB * NB / 2; q += kThreads) {
-        const int col = q / (MB / 2), r2 = (q % (MB / 2)) * 2;
-        double2 *g = reinterpret_cast<double2 *>(Ct + (size_t)col * np + r2);
-        const double2 c = *reinterpret_cast<const double2 *>(smem + col * T::LDC + r2);
-        double2 v = *g;
-        v.x -= c.x;
-        v.y -= c.y;
-        *g = v;
-    }
+    tile_gemm_sub<MB, NB, KC>(A + (size_t)col0 * np + row0, np, A + (size_t)k0 * np + row0, np,
+                              A + (size_t)k0 * np + col0, np, k1 - k0, smem);
 }
 
 constexpr int kDiagLD = 17;                                         // LDS stride of a 16x16 block
