@@ -176,113 +176,157 @@ __device__ inline void gemm_tile_update(double *__restrict__ A, int np, int row0
                               A + (size_t)k0 * np + col0, np, k1 - k0, smem);
 }
 
-constexpr int kDiagLD = 17;                                         // LDS stride of a 16x16 block
-constexpr int kPotrfSmem = 2 * kNBI + 8 + 4 * 16 * kDiagLD;       // doubles
+constexpr int kPLD = 68;                        // LDS stride of the staged 64x64 block (doubles)
+constexpr int kPotrfSmem = kNBI * kPLD + kNBI + 8;   // doubles: block, 1 / L_ii, flags
 
-// Factors the 64x64 diagonal block at (c0, c0) of the matrix at A.  Thread
-// (lane i, wave w) keeps row i of columns w, w+4, ..., w+60 in registers; the
-// wave owning column c takes the pivot (LAPACK dpotf2 test: not positive or
-// NaN fails), scales the column and publishes it through LDS, then every
-// thread applies the rank-1 update to its registers: one barrier per column.
+// sqrt(x) and 1 / sqrt(x) for x > 0: the hardware reciprocal square root
+// estimate refined by two steps of Goldschmidt's coupled iteration
+// (g -> sqrt x, h -> 1 / (2 sqrt x); r = 1/2 - g h, g += g r, h += h r),
+// each step roughly doubling the correct bits; a few dependent FMAs instead
+// of the long correctly rounded sqrt and division sequences.
+__device__ inline void sqrt_rsqrt(double x, double &sq, double &rsq)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+        const double r = fma(-g, h, 0.5);
+        g = fma(g, r, g);
+        h = fma(h, r, h);
+    }
+    sq = g;
+    rsq = 2.0 * h;
+}
+
+// x from lane l (uniform l) of the wave.
+__device__ inline double readlane_f64(double x, int l)
+{
+    const long long v = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)v, l);
+    const int hi = __builtin_amdgcn_readlane((int)(v >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Factors the 64x64 diagonal block at (c0, c0) of the matrix at A,
+// right-looking in four 16-column panels, the block staged in LDS:
+//  * one wave factors panel j (rows 16j..63, lane = row): per column the
+//    pivot and the column entries come from other lanes by v_readlane, so
+//    the 16 dependent steps need no barrier (LAPACK dpotf2 test: a pivot
+//    that is not positive, or NaN, fails; L = a * (1 / sqrt(pivot)), the
+//    square root and its reciprocal from sqrt_rsqrt; the logarithms of the
+//    diagonal are taken afterwards, one per lane);
+//  * the four waves then apply the panel to the trailing 16x16 blocks on
+//    the matrix cores (accumulators hold C^T, as in tile_gemm_sub).
 // Columns at or beyond n (bordered right-hand-side rows and padding) are not
 // factored: they become the identity, leaving the right-hand-side rows
-// holding L^{-1} rhs.  Returns the 1-based global failing column (0 = success)
-// and, on success, writes sum(log L_ii) of the block's real columns to
-// *logdet (thread 0) and the inverses of the four 16x16 diagonal blocks of
-// L to dinv (block q column major at dinv[256 q]), for trsm64_mfma.
+// holding L^{-1} rhs.  Returns the 1-based global failing column (0 =
+// success); on success writes the block's lower triangle back, sum(log L_ii)
+// of its real columns to *logdet (thread 0 of wave 0) and the inverses of its four
+// 16x16 diagonal blocks to dinv (block q column major at dinv[256 q]), for
+// trsm64_mfma.
 __device__ inline int potrf64_block(double *__restrict__ A, int np, int n, int c0, double *logdet,
                                     double *__restrict__ smem, double *__restrict__ dinv)
 {
     constexpr int B = kNBI;
-    constexpr int M = B / 4;
-    double *col0 = smem, *col1 = smem + B;
-    double *red = smem + 2 * B;
-    int *fail_col = reinterpret_cast<int *>(smem + 2 * B + kThreads / 64);
+    double *Ls = smem;                               // A(r, c) at Ls[c * kPLD + r]
+    double *rdiag = smem + B * kPLD;                 // 1 / L_ii
+    int *flag = reinterpret_cast<int *>(smem + B * kPLD + B);
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
-    double *Ag = A + (size_t)c0 * np + c0 + lane;
-    double a[M];
+    const int lr = lane & 15, lq = lane >> 4;
+    double *Ag = A + (size_t)c0 * np + c0;
 #pragma unroll
-    for (int m = 0; m < M; ++m) {
-        const int k = wave + 4 * m;
-        a[m] = lane >= k ? Ag[(size_t)k * np] : 0.0;
+    for (int u = 0; u < B * B / 2 / kThreads; ++u) {
+        const int e = tid + u * kThreads;
+        const int col = e / (B / 2), r2 = (e % (B / 2)) * 2;
+        *reinterpret_cast<double2 *>(Ls + col * kPLD + r2) = *reinterpret_cast<const double2 *>(Ag + (size_t)col * np + r2);
     }
-    if (tid == 0) *fail_col = 0;
+    if (tid == 0) *flag = 0;
     const int ncol = min(B, n - c0);
     __syncthreads();
-    // every thread runs all B steps (the barrier is unconditional); steps at
-    // or beyond ncol, or after a failed pivot, do no work
+    for (int j = 0; j < 4; ++j) {
+        if (wave == 0) {
+            double p[16];
 #pragma unroll
-    for (int c = 0; c < B; ++c) {
-        double *col = (c & 1) ? col1 : col0;
-        const bool live = c < ncol && *fail_col == 0;
-        if (live && wave == (c & 3)) {
-            const double piv = __shfl(a[c >> 2], c, 64);
-            if (!(piv > 0.0)) {
-                if (lane == 0) *fail_col = c0 + c + 1;
-            } else {
-                const double d = sqrt(piv);
-                const double l = lane > c ? a[c >> 2] * (1.0 / d) : (lane == c ? d : 0.0);
-                if (lane >= c) a[c >> 2] = l;
-                col[lane] = l;
+            for (int k = 0; k < 16; ++k) p[k] = Ls[(16 * j + k) * kPLD + lane];
+            int fail = 0;
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const int c = 16 * j + k;
+                if (c < ncol && fail == 0) {
+                    const double piv = readlane_f64(p[k], c);
+                    if (!(piv > 0.0)) {
+                        fail = c0 + c + 1;
+                    } else {
+                        double d, rinv;
+                        sqrt_rsqrt(piv, d, rinv);
+                        if (lane == 0) rdiag[c] = rinv;
+                        p[k] = lane > c ? p[k] * rinv : (lane == c ? d : p[k]);
+#pragma unroll
+                        for (int k2 = k + 1; k2 < 16; ++k2) {
+                            const double lc = readlane_f64(p[k], 16 * j + k2);
+                            if (lane >= 16 * j + k2) p[k2] -= p[k] * lc;
+                        }
+                    }
+                }
             }
+            if (lane >= 16 * j) {
+#pragma unroll
+                for (int k = 0; k < 16; ++k) Ls[(16 * j + k) * kPLD + lane] = p[k];
+            }
+            if (lane == 0) *flag = fail;
         }
         __syncthreads();
-        if (c < ncol && *fail_col == 0) {
-            // all reads first (they are independent), then the predicated update
-            double lk[M];
+        if (*flag != 0) return *flag;
+        // trailing blocks (bi, bj), j < bj <= bi <= 3, round robin over the waves
+        int t = 0;
+        for (int bj = j + 1; bj < 4; ++bj)
+            for (int bi = bj; bi < 4; ++bi, ++t) {
+                if ((t & 3) != wave) continue;
+                d4 acc;
 #pragma unroll
-            for (int m = 0; m < M; ++m) lk[m] = col[wave + 4 * m];
-            const double li = col[lane];
+                for (int v = 0; v < 4; ++v) acc[v] = Ls[(16 * bj + 4 * v + lq) * kPLD + 16 * bi + lr];
 #pragma unroll
-            for (int m = 0; m < M; ++m) {
-                const double t = a[m] - li * lk[m];
-                a[m] = (wave + 4 * m > c) ? t : a[m];
+                for (int kk = 0; kk < 4; ++kk) {
+                    const double af = Ls[(16 * j + 4 * kk + lq) * kPLD + 16 * bi + lr];
+                    const double bf = Ls[(16 * j + 4 * kk + lq) * kPLD + 16 * bj + lr];
+                    acc = mfma_f64(bf, -af, acc);
+                }
+#pragma unroll
+                for (int v = 0; v < 4; ++v) Ls[(16 * bj + 4 * v + lq) * kPLD + 16 * bi + lr] = acc[v];
             }
-        }
+        __syncthreads();
     }
-    const int failed = *fail_col;
-    if (failed != 0) return failed;
-    double dv = 1.0;
-#pragma unroll
-    for (int m = 0; m < M; ++m) {
-        const int k = wave + 4 * m;
-        if (k >= ncol) a[m] = lane == k ? 1.0 : 0.0;
-        if (k == lane) dv = a[m];
+    // log-determinant of the real columns: one log per lane
+    if (wave == 0) {
+        double lg = lane < ncol ? log(Ls[lane * kPLD + lane]) : 0.0;
+        for (int off = 32; off > 0; off >>= 1) lg += __shfl_xor(lg, off, 64);
+        if (lane == 0) *logdet = lg;
     }
-    double lg = ((lane & 3) == wave && lane < ncol) ? log(dv) : 0.0;
-    for (int off = 32; off > 0; off >>= 1) lg += __shfl_xor(lg, off, 64);
-    if (lane == 0) red[wave] = lg;
-    // diagonal 16x16 blocks to LDS: row lane of block q = lane / 16 holds
-    // columns k = wave + 4 m with m in [4q, 4q + 4)
-    double *Ds = smem + 2 * B + 8;
-    const int q = lane >> 4, il = lane & 15;
-#pragma unroll
-    for (int mm = 0; mm < 4; ++mm) {
-        double v = 0.0;
-#pragma unroll
-        for (int qq = 0; qq < 4; ++qq) v = (q == qq) ? a[4 * qq + mm] : v;
-        Ds[q * 16 * kDiagLD + (wave + 4 * mm) * kDiagLD + il] = v;   // column wave + 4 mm of block q
+    // columns at or beyond n become the identity
+    for (int e = tid; e < B * B; e += kThreads) {
+        const int col = e / B, r = e % B;
+        if (col >= ncol) Ls[col * kPLD + r] = r == col ? 1.0 : 0.0;
     }
-#pragma unroll
-    for (int m = 0; m < M; ++m) {
-        const int k = wave + 4 * m;
-        if (lane >= k) Ag[(size_t)k * np] = a[m];
-    }
+    if (tid >= ncol && tid < B) rdiag[tid] = 1.0;
     __syncthreads();
-    if (tid == 0) *logdet = (red[0] + red[1]) + (red[2] + red[3]);
+    for (int e = tid; e < B * B; e += kThreads) {
+        const int col = e / B, r = e % B;
+        if (r >= col) Ag[(size_t)col * np + r] = Ls[col * kPLD + r];
+    }
     // wave w inverts diagonal block w, lane c < 16 one column (column-oriented
     // forward substitution of L_ww x = e_c)
     if (lane < 16) {
-        const double *D = Ds + wave * 16 * kDiagLD;   // D(i, k) at D[k * kDiagLD + i]
+        const double *D = Ls + 16 * wave * kPLD + 16 * wave;   // D(i, k) at D[k * kPLD + i]
+        const double *rd = rdiag + 16 * wave;
         double x[16];
 #pragma unroll
         for (int i = 0; i < 16; ++i) x[i] = i == lane ? 1.0 : 0.0;
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
-            x[k] = x[k] / D[k * kDiagLD + k];
+            x[k] = x[k] * rd[k];
 #pragma unroll
-            for (int i = k + 1; i < 16; ++i) x[i] -= D[k * kDiagLD + i] * x[k];
+            for (int i = k + 1; i < 16; ++i) x[i] -= D[k * kPLD + i] * x[k];
         }
         double *W = dinv + wave * 256 + lane * 16;
 #pragma unroll
