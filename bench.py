@@ -30,6 +30,21 @@ sys.path.insert(0, str(REPO))
 NBO, NBI = 128, 64  # panel width / diagonal block of the factorisation (fb_common.h)
 
 
+PROFILE = REPO / "profiles" / "dag_factor_kernel_methanol_n2000_s50.md"
+
+
+def profiled_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), or None."""
+    try:
+        for line in PROFILE.read_text().splitlines():
+            if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def potrf_flops(n: int) -> float:
     """LAPACK dpotrf operation count."""
     return n ** 3 / 3.0 + n ** 2 / 2.0 + n / 6.0
@@ -46,6 +61,9 @@ def main():
                     help="bound on the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="only run 5 device-to-device copies of this many bytes (a known-size "
+                         "copy kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the swarm evaluation is not sharded across GPUs "
@@ -54,6 +72,15 @@ def main():
 
     import torch
     from ferebus_amd import SwarmLikelihood, compile_kernel, device_name, fit_mean, synthetic
+
+    if args.copy_bytes:
+        src = torch.ones(args.copy_bytes // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": args.copy_bytes, "copies": 5}))
+        return
 
     w = synthetic.methanol(n=args.n, seed=0)
     d = w.x.shape[1]
@@ -140,8 +167,10 @@ def main():
             "peak": 78.6,
             "unit": "TFLOP/s",
             "frac": round(achieved / 78.6, 4),
-            "traffic": None,
+            "traffic": (profiled_traffic("dag_factor_kernel")
+                        if persistent and n == 2000 and S == 50 else None),
             "algorithmic_flops_per_launch": fact_flops,
+            "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE, " + str(PROFILE.relative_to(REPO)),
             "avg_duration_ms": round(fact_ms, 4),
         },
         "kernels_ms_per_step": {k: round(v[0] / prof_steps, 4) for k, v in prof.items()},
