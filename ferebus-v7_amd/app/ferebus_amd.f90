@@ -73,6 +73,8 @@ program ferebus_amd
   pso%relative_change = cfg%stopping == "relative_change"
   pso%tolerance = cfg%tolerance
   pso%stall_iterations = cfg%stall_iterations
+  pso%lbfgs_every = cfg%lbfgs_every
+  pso%lbfgs%steps = cfg%lbfgs_steps
   pso%seed = seed
   if (len_trim(trace_path) > 0) then
     open(newunit=u, file=trim(trace_path), status="replace", action="write")
@@ -102,6 +104,8 @@ program ferebus_amd
       print '(A,F12.4,A)', " Opt Time Taken = ", res%opt_seconds, " seconds"
       print '(A,I0,A,I0,A,I0)', " Iterations: ", res%iterations, "  evaluations: ", res%evaluations, &
             "  rejected: ", res%rejections
+      if (res%polish_calls > 0) print '(A,I0,A,I0,A,I0)', " L-BFGS polishes: ", res%polish_calls, &
+            "  gradient evaluations: ", res%polish_evals, "  improved gbest: ", res%polish_improvements
       meta%system_name = cfg%system_name
       meta%atom_name = trim(cfg%atoms(a)%name)
       meta%property = trim(labels(p))

@@ -163,6 +163,10 @@ struct fb_handle {
     bool use_dag = true;             // persistent factorisation for the batched path
     int dag_grid = 0, n_xcd = 1;
     int *d_ctl = nullptr, *d_dep = nullptr, *d_dep0 = nullptr, *d_dead = nullptr, *h_err = nullptr;
+    // adjoint gradient workspace, allocated on first use for up to grad_cap particles
+    int grad_cap = 0;
+    fb::GradWork grad;
+    double *d_grad = nullptr, *h_grad = nullptr;
     unsigned long long *d_ready = nullptr;
     // swarm size -> device task table and its queue offsets
     hipStream_t group_stream[kMaxGroups] = {};
@@ -329,9 +333,21 @@ void drop_graphs(fb_handle *h)
     h->graphs.clear();
 }
 
+void free_grad(fb_handle *h)
+{
+    for (void *p : {(void *)h->grad.U, (void *)h->grad.NK, (void *)h->grad.alpha, (void *)h->grad.partial,
+                    (void *)h->d_grad})
+        if (p) (void)hipFree(p);
+    if (h->h_grad) (void)hipHostFree(h->h_grad);
+    h->grad = fb::GradWork();
+    h->d_grad = h->h_grad = nullptr;
+    h->grad_cap = 0;
+}
+
 void free_all(fb_handle *h)
 {
     drop_graphs(h);
+    free_grad(h);
     for (void *p : {(void *)h->d_ctl, (void *)h->d_dep, (void *)h->d_dep0, (void *)h->d_dead, (void *)h->d_ready})
         if (p) (void)hipFree(p);
     if (h->h_err) (void)hipHostFree(h->h_err);
@@ -630,9 +646,47 @@ int fb_get_alpha(fb_handle *h, const double *theta, double *alpha, double *out4,
 
 int fb_swarm_loglik_grad(fb_handle *h, int s, const double *theta, double *ll, double *grad, int *info)
 {
-    (void)s; (void)theta; (void)ll; (void)grad; (void)info;
-    if (!h) return fail(FB_E_ARG, "fb_swarm_loglik_grad: null handle");
-    return fail(FB_E_ARG, "fb_swarm_loglik_grad: adjoint gradient not built yet");
+    if (!h || !theta || !ll || !grad || !info) return fail(FB_E_ARG, "fb_swarm_loglik_grad: null argument");
+    if (s < 1 || s > h->max_swarm) return fail(FB_E_CAPACITY, "fb_swarm_loglik_grad: swarm size out of range");
+    const int P = h->prog.n_par;
+    if (s > h->grad_cap) {
+        free_grad(h);
+        const size_t mat = (size_t)h->np * h->np * s;
+        const size_t npart = fb::gradient_partial_count(h->n, h->n_slots, s, P);
+        if (hipMalloc((void **)&h->grad.U, sizeof(double) * mat) != hipSuccess ||
+            hipMalloc((void **)&h->grad.NK, sizeof(double) * mat) != hipSuccess ||
+            hipMalloc((void **)&h->grad.alpha, sizeof(double) * (size_t)h->n * s) != hipSuccess ||
+            hipMalloc((void **)&h->grad.partial, sizeof(double) * npart) != hipSuccess ||
+            hipMalloc((void **)&h->d_grad, sizeof(double) * (size_t)P * s) != hipSuccess ||
+            hipHostMalloc((void **)&h->h_grad, sizeof(double) * (size_t)P * s) != hipSuccess) {
+            free_grad(h);
+            (void)hipGetLastError();
+            return fail(FB_E_NOMEM, "fb_swarm_loglik_grad: device memory for the gradient workspace");
+        }
+        h->grad_cap = s;
+    }
+    h->grad.P = P;
+    std::memcpy(h->h_theta, theta, sizeof(double) * P * s);
+    FB_HIP(hipMemcpyAsync(h->d_theta, h->h_theta, sizeof(double) * P * s, hipMemcpyHostToDevice, h->stream));
+    int err = 0;
+    int rc = run_swarm(h, s, &err);
+    if (rc) return rc;
+    if (err == 1) return fail(FB_E_HIP, "fb_swarm_loglik_grad: the persistent factorisation found no ready task for 2 s");
+    if (err == 2) {
+        h->use_dag = false;
+        drop_graphs(h);
+        rc = run_swarm(h, s, &err);
+        if (rc) return rc;
+    }
+    hipStream_t st = h->stream;
+    fb::launch_gradient(buffers(h, 0, s, h->d_theta, h->d_ll, h->d_info), h->grad, h->d_grad, st);
+    FB_HIP(hipGetLastError());
+    FB_HIP(hipMemcpyAsync(h->h_grad, h->d_grad, sizeof(double) * P * s, hipMemcpyDeviceToHost, st));
+    FB_HIP(hipStreamSynchronize(st));
+    std::memcpy(ll, h->h_ll, sizeof(double) * s);
+    std::memcpy(info, h->h_info, sizeof(int) * s);
+    std::memcpy(grad, h->h_grad, sizeof(double) * P * s);
+    return h->profile ? drain_profile(h) : FB_OK;
 }
 
 int fb_profile_enable(fb_handle *h, int enable)

@@ -91,6 +91,18 @@ void launch_potrf64(const SwarmBuffers &b, int c0, hipStream_t st);
 void launch_trsm64(const SwarmBuffers &b, int c0, int row_start, hipStream_t st);
 void launch_likelihood(const SwarmBuffers &b, hipStream_t st);
 
+// Adjoint gradient (fb_grad.hip).
+struct GradWork {
+    double *U = nullptr;        // S x np x np: L^{-T}
+    double *NK = nullptr;       // S x np x np: -K^{-1} (lower tiles)
+    double *alpha = nullptr;    // S x n
+    double *partial = nullptr;  // gradient_partial_count(n, n_slots, S, P)
+    int P = 0;
+};
+int gradient_tile(int n_slots);
+size_t gradient_partial_count(int n, int n_slots, int S, int P);
+void launch_gradient(const SwarmBuffers &b, const GradWork &w, double *grad, hipStream_t st);
+
 // Persistent task-graph factorisation (fb_dag.hip).
 constexpr int kDagMaxQueues = 16;
 constexpr int kDagCtl = (6 * kDagMaxQueues + 2) * 32;   // control words, one per 128-byte line

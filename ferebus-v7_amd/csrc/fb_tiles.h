@@ -335,27 +335,31 @@ __device__ inline int potrf64_block(double *__restrict__ A, int np, int n, int c
     return 0;
 }
 
-// L(row0:row0+64, c0:c0+64) = C L_bb^{-T} in place, C the tile and L_bb the
-// factored diagonal block at (c0, c0).  Each wave solves its 16 rows alone:
-// with Y = X^T (X the wave's 16 x 64 slice of the result), L_bb Y = C^T is
-// block forward substitution over four 16-row groups,
+// X = C L_bb^{-T} in place on the 64x64 tile at Ct (leading dimension ldc),
+// L_bb the factored diagonal block at Lb (leading dimension ldl) with the
+// inverses of its 16x16 diagonal blocks in dinv; IDENT: C is the identity
+// (the tile is only written), giving L_bb^{-T} itself.  Each wave solves its
+// 16 rows alone: with Y = X^T (X the wave's 16 x 64 slice of the result),
+// L_bb Y = C^T is block forward substitution over four 16-row groups,
 //     Y_j = W_j (C_j^T - sum_{k<j} L_jk Y_k),    W_j = L_jj^{-1} (dinv),
 // every product on the matrix cores.  An MFMA result holds D[4v + l/16][l%16]
 // in lane l, component v: the B-operand layout of four consecutive k-steps,
 // so each Y_k feeds the later products straight from registers.  No LDS, no
 // barrier; the tile is read completely before it is overwritten.
-__device__ inline void trsm64_mfma(double *__restrict__ A, int np, int c0, int row0,
-                                   const double *__restrict__ dinv)
+template <bool IDENT>
+__device__ inline void trsm64_solve(double *__restrict__ Ct, int ldc, const double *__restrict__ Lb, int ldl,
+                                    const double *__restrict__ dinv)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int lr = lane & 15, lq = lane >> 4;
-    double *Ct = A + row0 + 16 * wave + lr;
-    const double *Lb = A + (size_t)c0 * np + c0;
+    double *Cw = Ct + 16 * wave + lr;
     d4 t[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) t[j][v] = Ct[(size_t)(c0 + 16 * j + 4 * v + lq) * np];
+        for (int v = 0; v < 4; ++v)
+            t[j][v] = IDENT ? ((16 * j + 4 * v + lq == 16 * wave + lr) ? 1.0 : 0.0)
+                            : Cw[(size_t)(16 * j + 4 * v + lq) * ldc];
     // L_jk (j > k) in A layout: lane l holds L_jk[l%16][4kk + l/16]
     double lf[6][4];
 #pragma unroll
@@ -363,7 +367,7 @@ __device__ inline void trsm64_mfma(double *__restrict__ A, int np, int c0, int r
 #pragma unroll
         for (int k = 0; k < j; ++k, ++p)
 #pragma unroll
-            for (int kk = 0; kk < 4; ++kk) lf[p][kk] = Lb[(size_t)(16 * k + 4 * kk + lq) * np + 16 * j + lr];
+            for (int kk = 0; kk < 4; ++kk) lf[p][kk] = Lb[(size_t)(16 * k + 4 * kk + lq) * ldl + 16 * j + lr];
     double wf[4][4];
 #pragma unroll
     for (int j = 0; j < 4; ++j)
@@ -385,7 +389,15 @@ __device__ inline void trsm64_mfma(double *__restrict__ A, int np, int c0, int r
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) Ct[(size_t)(c0 + 16 * j + 4 * v + lq) * np] = y[j][v];
+        for (int v = 0; v < 4; ++v) Cw[(size_t)(16 * j + 4 * v + lq) * ldc] = y[j][v];
+}
+
+// L(row0:row0+64, c0:c0+64) = C L_bb^{-T} in place, C the tile and L_bb the
+// factored diagonal block at (c0, c0) of the matrix at A (the panel solve).
+__device__ inline void trsm64_mfma(double *__restrict__ A, int np, int c0, int row0,
+                                   const double *__restrict__ dinv)
+{
+    trsm64_solve<false>(A + (size_t)c0 * np + row0, np, A + (size_t)c0 * np + c0, np, dinv);
 }
 
 }  // namespace fb

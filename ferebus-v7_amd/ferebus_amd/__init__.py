@@ -9,6 +9,7 @@ from .kernel_expr import (KernelConfig, KernelExpressionError, KernelProgram, co
 from .mean import MeanFit, fit_mean
 from .gpr import SwarmLikelihood
 from ._abi import FbError, device_name, library_path, load
+from . import lbfgs
 
 __all__ = ["KernelConfig", "KernelExpressionError", "KernelProgram", "compile_kernel", "parse",
-           "MeanFit", "fit_mean", "SwarmLikelihood", "FbError", "device_name", "library_path", "load"]
+           "MeanFit", "fit_mean", "SwarmLikelihood", "FbError", "device_name", "library_path", "load", "lbfgs"]

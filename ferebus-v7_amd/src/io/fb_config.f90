@@ -63,6 +63,7 @@ module fb_config
     character(len=:), allocatable :: stopping
     real(wp) :: tolerance = 1e-6_wp
     integer :: stall_iterations = 20
+    integer :: lbfgs_every = 0, lbfgs_steps = 5
     type(KernelConfigEntry), allocatable :: kernels(:)
     character(len=256), allocatable :: note_keys(:), note_values(:)
   end type FerebusConfig
@@ -367,6 +368,9 @@ contains
     cfg%stopping = doc%get_str("optimiser.pso.stopping_criteria", "none")
     cfg%tolerance = doc%get_real("optimiser.pso.relative_change.tolerance", 1e-6_wp)
     cfg%stall_iterations = doc%get_int("optimiser.pso.relative_change.stall_iterations", 20)
+    ! build-side extension (no reference key): L-BFGS polishing of gbest
+    cfg%lbfgs_every = doc%get_int("optimiser.pso.lbfgs_every", 0)
+    cfg%lbfgs_steps = doc%get_int("optimiser.pso.lbfgs_steps", 5)
     knames = doc%keys_under("kernels")
     allocate(cfg%kernels(size(knames)))
     do k = 1, size(knames)

@@ -19,11 +19,15 @@
 !>    `stall_iterations` iterations (pso.f90:619-636), or the iteration cap
 !>    (optimiser_type.f90:122-132).
 !> Every evaluation of a whole swarm is one call of the gfx950 library.
+!> Optional (off by default, lbfgs_every = 0): every lbfgs_every iterations
+!> the global best is polished by a few L-BFGS steps on the adjoint gradient
+!> (fb_lbfgs) and replaces gbest when the polished cost is lower.
 module fb_pso
   use iso_c_binding, only: c_int, c_double
   use ieee_arithmetic, only: ieee_value, ieee_positive_inf
   use ferebus_hip, only: FB_OK, fb_error_message
   use fb_gpr, only: GaussianProcessRegressor
+  use fb_lbfgs, only: LbfgsSettings, lbfgs_polish
   implicit none
   private
   integer, parameter :: wp = c_double
@@ -41,6 +45,8 @@ module fb_pso
     integer :: max_retries = 10000
     logical :: verbose = .true.
     integer :: trace_unit = -1                    ! >0: one line per evaluated particle
+    integer :: lbfgs_every = 0                    ! >0: polish gbest every this many iterations
+    type(LbfgsSettings) :: lbfgs
   end type PsoSettings
 
   type, public :: PsoResult
@@ -51,6 +57,9 @@ module fb_pso
     integer :: rejections = 0
     integer :: best_iter = 0
     real(wp) :: opt_seconds = 0.0_wp
+    integer :: polish_calls = 0
+    integer :: polish_evals = 0
+    integer :: polish_improvements = 0
     integer :: status = 0
     character(len=:), allocatable :: message
   end type PsoResult
@@ -144,8 +153,9 @@ contains
     type(PsoResult), intent(out) :: res
     real(wp), allocatable :: x(:, :), v(:, :), pbest(:, :), r1(:, :), r2(:, :)
     real(wp), allocatable :: cost(:), pcost(:), lo(:), hi(:), gbest(:)
-    real(wp) :: gcost, stall_cost
-    integer :: p, s, i, it, stall_count, all_idx(cfg%swarm_size)
+    real(wp) :: gcost, stall_cost, fp
+    real(wp), allocatable :: xp(:)
+    integer :: p, s, i, it, stall_count, all_idx(cfg%swarm_size), ne, na, rc
     integer(8) :: t0, t1, rate
     p = gp%nparams
     s = cfg%swarm_size
@@ -195,6 +205,28 @@ contains
           res%best_iter = it
         end if
       end do
+      if (cfg%lbfgs_every > 0) then
+        if (mod(it, cfg%lbfgs_every) == 0) then
+          xp = gbest
+          fp = gcost
+          call lbfgs_polish(gp, cfg%lbfgs, xp, fp, ne, na, rc)
+          if (rc /= FB_OK) then
+            res%status = rc
+            res%message = fb_error_message()
+            return
+          end if
+          res%polish_calls = res%polish_calls + 1
+          res%polish_evals = res%polish_evals + ne
+          if (fp < gcost) then
+            gcost = fp
+            gbest = xp
+            res%best_iter = it
+            res%polish_improvements = res%polish_improvements + 1
+          end if
+          if (cfg%trace_unit > 0) write(cfg%trace_unit, '(A,1X,I0,1X,I0,1X,I0,*(1X,ES24.16E3))') &
+              "polish", it, ne, na, fp, xp
+        end if
+      end if
       res%iterations = it
       if (cfg%relative_change) then
         if (abs((gcost - stall_cost) / stall_cost) <= cfg%tolerance) then

@@ -130,11 +130,18 @@ int fb_particle_loglik(fb_handle *h, int s, const double *theta, double *ll, int
 int fb_get_alpha(fb_handle *h, const double *theta, double *alpha, double *out4, int *info);
 
 /* Gradient of the log-likelihood with respect to theta for s particles:
- * d logL / d theta_k = 1/2 tr((alpha alpha^T - K^{-1}) dK/dtheta_k), evaluated
- * without forming dK/dtheta.  The reference parses a gradient flag but never
- * computes one (gpr.f90:272-282); this entry has no reference counterpart and
- * serves gradient polishing of the swarm's best particle.
- *   grad  P*s FP64, column s = gradient of particle s (of ll, not of cost). */
+ * d logL / d theta_k = 1/2 tr((alpha alpha^T / s2 - K^{-1}) dK/dtheta_k)
+ * (s2 = 1 marginal, sigma2_hat concentrated), evaluated without forming
+ * dK/dtheta: K^{-1} = L^{-T} L^{-1} from the factor on the matrix cores, then
+ * one fused pass over the lower triangle that recomputes each kernel leaf.
+ * The reference parses a gradient flag but never computes one
+ * (gpr.f90:272-282); this entry has no reference counterpart and serves
+ * L-BFGS polishing of the swarm's best particle (src/optimiser/fb_lbfgs.f90).
+ * Allocates a workspace of 2 s np^2 FP64 on the first call with this many
+ * particles (np = N + right-hand sides, rounded up to 128).
+ *   ll    s FP64, bit-identical to fb_swarm_loglik on the same theta
+ *   grad  P*s FP64, column s = gradient of particle s (of ll, not of cost);
+ *         NaN where info[s] != 0. */
 int fb_swarm_loglik_grad(fb_handle *h, int s, const double *theta, double *ll,
                          double *grad, int *info);
 
