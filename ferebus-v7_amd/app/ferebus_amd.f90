@@ -79,6 +79,7 @@ program ferebus_amd
   if (len_trim(trace_path) > 0) then
     open(newunit=u, file=trim(trace_path), status="replace", action="write")
     pso%trace_unit = u
+    pso%trace = .true.
   end if
 
   do a = 1, size(cfg%atoms)
@@ -122,5 +123,5 @@ program ferebus_amd
       print '(A,F12.4,A)', " Time Taken = ", real(t1 - t0, wp) / real(rate, wp), " seconds"
     end do
   end do
-  if (pso%trace_unit > 0) close(pso%trace_unit)
+  if (pso%trace) close(pso%trace_unit)
 end program ferebus_amd

@@ -44,7 +44,8 @@ module fb_pso
     integer :: seed = 1
     integer :: max_retries = 10000
     logical :: verbose = .true.
-    integer :: trace_unit = -1                    ! >0: one line per evaluated particle
+    logical :: trace = .false.                    ! one line per evaluated particle on trace_unit
+    integer :: trace_unit = 0
     integer :: lbfgs_every = 0                    ! >0: polish gbest every this many iterations
     type(LbfgsSettings) :: lbfgs
   end type PsoSettings
@@ -118,7 +119,7 @@ contains
       end if
       res%evaluations = res%evaluations + size(todo)
       do k = 1, size(todo)
-        if (cfg%trace_unit > 0) write(cfg%trace_unit, '(I0,1X,I0,1X,I0,1X,I0,*(1X,ES24.16E3))') &
+        if (cfg%trace) write(cfg%trace_unit, '(I0,1X,I0,1X,I0,1X,I0,*(1X,ES24.16E3))') &
             it, round, todo(k), info(k), c(k), th(:, k)
         if (ok(k)) then
           cost(todo(k)) = c(k)
@@ -223,7 +224,7 @@ contains
             res%best_iter = it
             res%polish_improvements = res%polish_improvements + 1
           end if
-          if (cfg%trace_unit > 0) write(cfg%trace_unit, '(A,1X,I0,1X,I0,1X,I0,*(1X,ES24.16E3))') &
+          if (cfg%trace) write(cfg%trace_unit, '(A,1X,I0,1X,I0,1X,I0,*(1X,ES24.16E3))') &
               "polish", it, ne, na, fp, xp
         end if
       end if

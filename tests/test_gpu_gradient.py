@@ -169,3 +169,19 @@ def test_gradient_nma_full_size(gpu):
         err = abs(fd - g[1, q]) / max(np.abs(g[1]).max(), 1e-300)
         print(f"nma param {q}: adjoint {g[1, q]:.6e} fd {fd:.6e} rel {err:.2e}")
         assert err <= FD_RTOL
+
+
+def test_lbfgs_polish_improves(gpu):
+    """ferebus_amd.lbfgs (mirror of src/optimiser/fb_lbfgs.f90) on the adjoint
+    gradient: every accepted step lowers the cost, the result is
+    reproducible, and the final cost is the swarm evaluation's at that theta."""
+    from ferebus_amd import lbfgs
+    x, y, prog, m, gp, th = _setup(180, 6, seed=71, S=1)
+    ll0, _ = gp.swarm_loglik(th)
+    xp, fp, evals, accepted = lbfgs.polish(gp, th[0], lbfgs.LbfgsSettings(steps=6))
+    print(f"lbfgs: cost {-ll0[0]:.10e} -> {fp:.10e} in {accepted} steps, {evals} evaluations")
+    assert accepted >= 1 and fp < -ll0[0]
+    xp2, fp2, _, _ = lbfgs.polish(gp, th[0], lbfgs.LbfgsSettings(steps=6))
+    assert np.array_equal(xp, xp2) and fp == fp2
+    ll, info = gp.swarm_loglik(xp)
+    assert info[0] == 0 and -ll[0] == fp
