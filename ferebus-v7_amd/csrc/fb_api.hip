@@ -162,7 +162,9 @@ struct fb_handle {
     bool use_graph = true;
     bool use_dag = true;             // persistent factorisation for the batched path
     int dag_grid = 0, n_xcd = 1;
-    int *d_ctl = nullptr, *d_dep = nullptr, *d_dep0 = nullptr, *d_dead = nullptr, *h_err = nullptr;
+    int *d_ctl = nullptr, *d_dep = nullptr, *d_dead = nullptr, *h_err = nullptr;
+    // swarm size -> schedule plan and device matrix descriptors
+    std::map<int, std::pair<fb::DagPlan, fb::DagMatrix *>> dag_plans;
     // adjoint gradient workspace, allocated on first use for up to grad_cap particles
     int grad_cap = 0;
     fb::GradWork grad;
@@ -257,10 +259,39 @@ void enqueue_group(fb_handle *h, const fb::SwarmBuffers &b, hipStream_t st)
 void enqueue_dag(fb_handle *h, const fb::SwarmBuffers &b, hipStream_t st)
 {
     fb::DagWork w;
-    w.ctl = h->d_ctl; w.dep = h->d_dep; w.dep0 = h->d_dep0; w.slots = h->d_ready; w.dead = h->d_dead;
+    w.ctl = h->d_ctl; w.dep = h->d_dep; w.slots = h->d_ready; w.dead = h->d_dead;
+    const auto &pl = h->dag_plans.at(b.S);
     timed(h, 0, st, [&] { fb::launch_build(b, st); });
-    timed(h, 6, st, [&] { fb::launch_dag(b, w, h->n_xcd, h->dag_grid, st); });
+    timed(h, 6, st, [&] { fb::launch_dag(pl.first, pl.second, w, h->dag_grid, st); });
     timed(h, 4, st, [&] { fb::launch_likelihood(b, st); });
+}
+
+// Schedule plan and device descriptors of a swarm of S (outside any graph
+// capture; cached per S).
+int dag_plan_for(fb_handle *h, int S)
+{
+    if (h->dag_plans.count(S)) return FB_OK;
+    const fb::SwarmBuffers b = buffers(h, 0, S, h->d_theta, h->d_ll, h->d_info);
+    std::vector<fb::DagMatrix> mats(S);
+    for (int s = 0; s < S; ++s) {
+        fb::DagMatrix &m = mats[s];
+        m.K = b.K + (size_t)s * h->np * h->np;
+        m.dinv = b.dinv + (size_t)s * h->nsteps * fb::kDinv;
+        m.logdet_part = b.logdet_part + (size_t)s * h->nsteps;
+        m.info = b.info + s;
+        m.np = h->np; m.n = h->n; m.nb = h->nsteps;
+    }
+    fb::DagPlan pl = fb::dag_plan(mats, h->n_xcd);
+    if (!pl.ok) return fail(FB_E_CAPACITY, "swarm too large for one persistent launch (matrices per XCD queue)");
+    fb::DagMatrix *d = nullptr;
+    FB_HIP(hipMalloc((void **)&d, sizeof(fb::DagMatrix) * S));
+    hipError_t e = hipMemcpy(d, mats.data(), sizeof(fb::DagMatrix) * S, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(d);
+        return fail(FB_E_HIP, std::string("schedule upload: ") + hipGetErrorString(e));
+    }
+    h->dag_plans[S] = std::make_pair(pl, d);
+    return FB_OK;
 }
 
 // The whole swarm [0, S): the persistent path, or groups of particles on
@@ -348,8 +379,10 @@ void free_all(fb_handle *h)
 {
     drop_graphs(h);
     free_grad(h);
-    for (void *p : {(void *)h->d_ctl, (void *)h->d_dep, (void *)h->d_dep0, (void *)h->d_dead, (void *)h->d_ready})
+    for (void *p : {(void *)h->d_ctl, (void *)h->d_dep, (void *)h->d_dead, (void *)h->d_ready})
         if (p) (void)hipFree(p);
+    for (auto &pl : h->dag_plans) (void)hipFree(pl.second.second);
+    h->dag_plans.clear();
     if (h->h_err) (void)hipHostFree(h->h_err);
     for (const EventPair &p : h->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
     for (hipEvent_t e : h->event_pool) (void)hipEventDestroy(e);
@@ -451,17 +484,10 @@ int fb_create(fb_handle **out, int n, int d, const double *x, const double *y, c
     // persistent scheduler: counters per task of every matrix, ready lists
     // sized for the largest queue of any swarm size up to S
     h->dag_grid = fb::dag_grid_size(&h->n_xcd);
-    size_t ready_slots = 0;
-    for (int s = 1; s <= S; ++s) {
-        const int Q = fb::dag_queues(s, h->n_xcd);
-        ready_slots = std::max(ready_slots, (size_t)Q * 2 * fb::dag_slot_capacity(s, h->np, Q));
-    }
-    const std::vector<int> dep0 = fb::dag_initial_counts(h->np);
     FB_ALLOC(h->d_ctl, sizeof(int) * fb::kDagCtl);
-    FB_ALLOC(h->d_dep, sizeof(int) * dep0.size() * S);
-    FB_ALLOC(h->d_dep0, sizeof(int) * dep0.size());
+    FB_ALLOC(h->d_dep, sizeof(int) * fb::dag_index_count(h->np) * S);
     FB_ALLOC(h->d_dead, sizeof(int) * S);
-    FB_ALLOC(h->d_ready, sizeof(unsigned long long) * ready_slots);
+    FB_ALLOC(h->d_ready, sizeof(unsigned long long) * 2 * fb::dag_tasks_per_matrix(h->np) * S);
 #undef FB_ALLOC
     if (hipHostMalloc((void **)&h->h_theta, sizeof(double) * std::max(h->prog.n_par, 1) * S) != hipSuccess ||
         hipHostMalloc((void **)&h->h_ll, sizeof(double) * S) != hipSuccess ||
@@ -497,7 +523,6 @@ int fb_create(fb_handle **out, int n, int d, const double *x, const double *y, c
     if (e == hipSuccess) e = hipMemcpy(h->d_rhs, rhs.data(), sizeof(double) * rhs.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(h->d_info, 0, sizeof(int) * S);
     if (e == hipSuccess) e = hipMemset(h->d_ctl, 0, sizeof(int) * fb::kDagCtl);
-    if (e == hipSuccess) e = hipMemcpy(h->d_dep0, dep0.data(), sizeof(int) * dep0.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         free_all(h);
         delete h;
@@ -535,7 +560,15 @@ int fb_swarm_loglik_async(fb_handle *h, int s, const double *d_theta, double *d_
 {
     if (!h || !d_theta || !d_ll || !d_info) return fail(FB_E_ARG, "fb_swarm_loglik_async: null argument");
     if (s < 1 || s > h->max_swarm) return fail(FB_E_CAPACITY, "fb_swarm_loglik_async: swarm size out of range");
-    enqueue_pipeline(h, s, d_theta, d_ll, d_info, h->stream);
+    if (h->use_dag) {
+        // the persistent schedule's descriptors name the handle's INFO buffer
+        const int rc = dag_plan_for(h, s);
+        if (rc) return rc;
+        enqueue_pipeline(h, s, d_theta, d_ll, h->d_info, h->stream);
+        FB_HIP(hipMemcpyAsync(d_info, h->d_info, sizeof(int) * s, hipMemcpyDeviceToDevice, h->stream));
+    } else {
+        enqueue_pipeline(h, s, d_theta, d_ll, d_info, h->stream);
+    }
     FB_HIP(hipGetLastError());
     return FB_OK;
 }
@@ -583,6 +616,10 @@ int fb_swarm_loglik(fb_handle *h, int s, const double *theta, double *ll, int *i
     if (!h || !theta || !ll || !info) return fail(FB_E_ARG, "fb_swarm_loglik: null argument");
     if (s < 1 || s > h->max_swarm) return fail(FB_E_CAPACITY, "fb_swarm_loglik: swarm size out of range");
     const int P = h->prog.n_par;
+    if (h->use_dag) {
+        const int rc = dag_plan_for(h, s);
+        if (rc) return rc;
+    }
     std::memcpy(h->h_theta, theta, sizeof(double) * P * s);
     FB_HIP(hipMemcpyAsync(h->d_theta, h->h_theta, sizeof(double) * P * s, hipMemcpyHostToDevice, h->stream));
     int err = 0;
@@ -666,6 +703,10 @@ int fb_swarm_loglik_grad(fb_handle *h, int s, const double *theta, double *ll, d
         h->grad_cap = s;
     }
     h->grad.P = P;
+    if (h->use_dag) {
+        const int prc = dag_plan_for(h, s);
+        if (prc) return prc;
+    }
     std::memcpy(h->h_theta, theta, sizeof(double) * P * s);
     FB_HIP(hipMemcpyAsync(h->d_theta, h->h_theta, sizeof(double) * P * s, hipMemcpyHostToDevice, h->stream));
     int err = 0;
@@ -687,6 +728,160 @@ int fb_swarm_loglik_grad(fb_handle *h, int s, const double *theta, double *ll, d
     std::memcpy(info, h->h_info, sizeof(int) * s);
     std::memcpy(grad, h->h_grad, sizeof(double) * P * s);
     return h->profile ? drain_profile(h) : FB_OK;
+}
+
+// ---- several handles in one persistent factorisation ------------------------
+
+struct fb_multi {
+    std::vector<fb_handle *> gps;
+    int dag_grid = 0, n_xcd = 1;
+    fb::DagWork w;
+    int *h_err = nullptr;
+    std::map<std::vector<int>, std::pair<fb::DagPlan, fb::DagMatrix *>> plans;   // per particle counts
+};
+
+namespace {
+
+void free_multi(fb_multi *m)
+{
+    for (void *p : {(void *)m->w.ctl, (void *)m->w.dep, (void *)m->w.slots, (void *)m->w.dead})
+        if (p) (void)hipFree(p);
+    for (auto &pl : m->plans) (void)hipFree(pl.second.second);
+    if (m->h_err) (void)hipHostFree(m->h_err);
+}
+
+int multi_plan(fb_multi *m, const std::vector<int> &s)
+{
+    if (m->plans.count(s)) return FB_OK;
+    std::vector<fb::DagMatrix> mats;
+    for (size_t g = 0; g < m->gps.size(); ++g) {
+        fb_handle *h = m->gps[g];
+        const fb::SwarmBuffers b = buffers(h, 0, std::max(s[g], 1), h->d_theta, h->d_ll, h->d_info);
+        for (int k = 0; k < s[g]; ++k) {
+            fb::DagMatrix d;
+            d.K = b.K + (size_t)k * h->np * h->np;
+            d.dinv = b.dinv + (size_t)k * h->nsteps * fb::kDinv;
+            d.logdet_part = b.logdet_part + (size_t)k * h->nsteps;
+            d.info = b.info + k;
+            d.np = h->np; d.n = h->n; d.nb = h->nsteps;
+            mats.push_back(d);
+        }
+    }
+    if (mats.empty()) return FB_OK;
+    fb::DagPlan pl = fb::dag_plan(mats, m->n_xcd);
+    if (!pl.ok) return fail(FB_E_CAPACITY, "fb_multi: too many matrices for one persistent launch");
+    fb::DagMatrix *d = nullptr;
+    FB_HIP(hipMalloc((void **)&d, sizeof(fb::DagMatrix) * mats.size()));
+    hipError_t e = hipMemcpy(d, mats.data(), sizeof(fb::DagMatrix) * mats.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(d);
+        return fail(FB_E_HIP, std::string("fb_multi: schedule upload: ") + hipGetErrorString(e));
+    }
+    m->plans[s] = std::make_pair(pl, d);
+    return FB_OK;
+}
+
+}  // namespace
+
+int fb_multi_create(fb_multi **out, int n_gps, fb_handle *const *gps)
+{
+    if (!out || n_gps < 1 || !gps) return fail(FB_E_ARG, "fb_multi_create: bad argument");
+    *out = nullptr;
+    fb_multi *m = new fb_multi();
+    size_t dep = 0, slots = 0, mats = 0;
+    for (int g = 0; g < n_gps; ++g) {
+        if (!gps[g]) {
+            delete m;
+            return fail(FB_E_ARG, "fb_multi_create: null handle");
+        }
+        fb_handle *h = gps[g];
+        m->gps.push_back(h);
+        dep += (size_t)fb::dag_index_count(h->np) * h->max_swarm;
+        slots += 2 * (size_t)fb::dag_tasks_per_matrix(h->np) * h->max_swarm;
+        mats += h->max_swarm;
+    }
+    m->dag_grid = fb::dag_grid_size(&m->n_xcd);
+    if (m->dag_grid <= 0 || hipMalloc((void **)&m->w.ctl, sizeof(int) * fb::kDagCtl) != hipSuccess ||
+        hipMalloc((void **)&m->w.dep, sizeof(int) * dep) != hipSuccess ||
+        hipMalloc((void **)&m->w.slots, sizeof(unsigned long long) * slots) != hipSuccess ||
+        hipMalloc((void **)&m->w.dead, sizeof(int) * mats) != hipSuccess ||
+        hipHostMalloc((void **)&m->h_err, sizeof(int)) != hipSuccess) {
+        free_multi(m);
+        delete m;
+        (void)hipGetLastError();
+        return fail(FB_E_NOMEM, "fb_multi_create: scheduler workspace");
+    }
+    *out = m;
+    return FB_OK;
+}
+
+int fb_multi_destroy(fb_multi *m)
+{
+    if (!m) return FB_OK;
+    if (!m->gps.empty() && m->gps[0]->stream) (void)hipStreamSynchronize(m->gps[0]->stream);
+    free_multi(m);
+    delete m;
+    return FB_OK;
+}
+
+int fb_multi_swarm_loglik(fb_multi *m, const int *s, const double *const *theta, double *const *ll,
+                          int *const *info)
+{
+    if (!m || !s || !theta || !ll || !info) return fail(FB_E_ARG, "fb_multi_swarm_loglik: null argument");
+    const int G = (int)m->gps.size();
+    std::vector<int> sv(s, s + G);
+    for (int g = 0; g < G; ++g) {
+        if (sv[g] < 0 || sv[g] > m->gps[g]->max_swarm)
+            return fail(FB_E_CAPACITY, "fb_multi_swarm_loglik: swarm size out of range");
+        if (sv[g] > 0 && (!theta[g] || !ll[g] || !info[g]))
+            return fail(FB_E_ARG, "fb_multi_swarm_loglik: null argument");
+    }
+    int rc = multi_plan(m, sv);
+    if (rc) return rc;
+    hipStream_t st = m->gps[0]->stream;
+    for (int g = 0; g < G; ++g) {
+        if (sv[g] == 0) continue;
+        fb_handle *h = m->gps[g];
+        const int P = h->prog.n_par;
+        std::memcpy(h->h_theta, theta[g], sizeof(double) * P * sv[g]);
+        FB_HIP(hipMemcpyAsync(h->d_theta, h->h_theta, sizeof(double) * P * sv[g], hipMemcpyHostToDevice, st));
+    }
+    auto run = [&](bool persistent) -> int {
+        for (int g = 0; g < G; ++g) {
+            if (sv[g] == 0) continue;
+            fb_handle *h = m->gps[g];
+            const fb::SwarmBuffers b = buffers(h, 0, sv[g], h->d_theta, h->d_ll, h->d_info);
+            if (persistent) fb::launch_build(b, st);
+            else enqueue_group(h, b, st);
+        }
+        if (persistent && m->plans.at(sv).second)
+            fb::launch_dag(m->plans.at(sv).first, m->plans.at(sv).second, m->w, m->dag_grid, st);
+        for (int g = 0; g < G; ++g) {
+            if (sv[g] == 0) continue;
+            fb_handle *h = m->gps[g];
+            if (persistent) fb::launch_likelihood(buffers(h, 0, sv[g], h->d_theta, h->d_ll, h->d_info), st);
+            FB_HIP(hipMemcpyAsync(h->h_ll, h->d_ll, sizeof(double) * sv[g], hipMemcpyDeviceToHost, st));
+            FB_HIP(hipMemcpyAsync(h->h_info, h->d_info, sizeof(int) * sv[g], hipMemcpyDeviceToHost, st));
+        }
+        *m->h_err = 0;
+        if (persistent) FB_HIP(hipMemcpyAsync(m->h_err, m->w.ctl + fb::kDagErrorWord, sizeof(int), hipMemcpyDeviceToHost, st));
+        FB_HIP(hipStreamSynchronize(st));
+        FB_HIP(hipGetLastError());
+        return FB_OK;
+    };
+    rc = run(true);
+    if (rc) return rc;
+    if (*m->h_err == 1) return fail(FB_E_HIP, "fb_multi_swarm_loglik: the persistent factorisation found no ready task for 2 s");
+    if (*m->h_err == 2) {
+        rc = run(false);   // some XCD queue was not served: lock-step path
+        if (rc) return rc;
+    }
+    for (int g = 0; g < G; ++g) {
+        if (sv[g] == 0) continue;
+        std::memcpy(ll[g], m->gps[g]->h_ll, sizeof(double) * sv[g]);
+        std::memcpy(info[g], m->gps[g]->h_info, sizeof(int) * sv[g]);
+    }
+    return FB_OK;
 }
 
 int fb_profile_enable(fb_handle *h, int enable)

@@ -105,21 +105,38 @@ void launch_gradient(const SwarmBuffers &b, const GradWork &w, double *grad, hip
 
 // Persistent task-graph factorisation (fb_dag.hip).
 constexpr int kDagMaxQueues = 16;
+constexpr int kDagMaxQueueMats = 128;   // matrices per XCD queue in one launch
 constexpr int kDagCtl = (6 * kDagMaxQueues + 2) * 32;   // control words, one per 128-byte line
 constexpr int kDagErrorWord = kDagCtl - 32;              // 0 ok, 1 no ready task for 2 s, 2 queue not drained
+// One matrix of a persistent factorisation.
+struct DagMatrix {
+    double *K;               // np x np, column major
+    double *dinv;            // nb x kDinv
+    double *logdet_part;     // nb
+    int *info;               // INFO of this matrix
+    int np, n, nb;
+    int dep_off;             // first counter (dag_plan)
+    int queue, rank;         // XCD queue and position among its matrices (dag_plan)
+};
+struct DagPlan {
+    bool ok = true;
+    int Q = 1, n_mats = 0, n_dep = 0, n_slots = 0;
+    int n_hi[kDagMaxQueues], n_lo[kDagMaxQueues], n_first[kDagMaxQueues];
+    int slot_off[kDagMaxQueues], slot_cap[kDagMaxQueues];
+};
 struct DagWork {
     int *ctl = nullptr;                  // kDagCtl control words
-    int *dep = nullptr;                  // max_swarm x dag_index_count(np) counters
-    const int *dep0 = nullptr;           // dag_index_count(np) initial counters
-    unsigned long long *slots = nullptr; // ready lists
-    int *dead = nullptr;                 // max_swarm
+    int *dep = nullptr;                  // DagPlan::n_dep counters
+    unsigned long long *slots = nullptr; // DagPlan::n_slots ready-list slots
+    int *dead = nullptr;                 // one per matrix
 };
 int dag_grid_size(int *n_xcd);
-int dag_queues(int S, int n_xcd);
 int dag_index_count(int np);
-int dag_slot_capacity(int S, int np, int Q);
-std::vector<int> dag_initial_counts(int np);
-void launch_dag(const SwarmBuffers &b, const DagWork &w, int n_queues, int grid, hipStream_t st);
+int dag_tasks_per_matrix(int np);
+// Fills queue, rank, dep_off; plan.ok = false when a queue would hold more
+// than kDagMaxQueueMats matrices.
+DagPlan dag_plan(std::vector<DagMatrix> &mats, int n_xcd);
+void launch_dag(const DagPlan &plan, const DagMatrix *d_mats, const DagWork &w, int grid, hipStream_t st);
 void launch_backsolve(const SwarmBuffers &b, int particle, double *alpha, hipStream_t st);
 
 }  // namespace fb

@@ -8,6 +8,11 @@
 // overlap the matrix-core trailing updates of the others, with no launch
 // boundaries in between.
 //
+// The matrices are described one by one (DagMatrix: storage, order, INFO,
+// log-det and inverse slots), so one launch factors the swarms of several
+// training sets of different N together -- the batched uneven case -- as
+// well as the equal-size swarm of one training set.
+//
 // Tasks of matrix s (64-row tiles i, 64-column blocks b, 128-column panels p
 // made of blocks b0 = 2p, b1 = 2p + 1, nb = np / 64 tiles, P = nb / 2):
 //   TG (p, i)   C(i, panel p) -= L(i, 0:128p) L(panel rows, 0:128p)^T   1 <= p, 2p <= i
@@ -39,10 +44,11 @@
 // ready at the start.
 //
 // XCD pinning.  gfx950 has 8 accelerator dies (XCDs), each with its own L2;
-// the L2s are not coherent with each other inside a kernel.  Matrix s belongs
-// to queue s mod Q, and a queue is served by the work-groups of exactly one
-// XCD (the first XCD whose work-group claims it), so producer and consumer of
-// every tile share one L2.  Publishing a tile then needs only its stores to
+// the L2s are not coherent with each other inside a kernel.  Every matrix
+// belongs to one queue (chosen on the host: round robin for equal sizes,
+// longest-processing-time on N^3 for uneven ones), and a queue is served by
+// the work-groups of exactly one XCD (the first XCD whose work-group claims
+// it), so producer and consumer of every tile share one L2.  Publishing a tile then needs only its stores to
 // have reached that L2 (s_waitcnt vmcnt(0); the vector L1 is write-through),
 // and taking one drops the consumer CU's L1 and the L2's clean non-coherent
 // lines (buffer_inv sc1; dirty lines stay), instead of writing the whole L2
@@ -62,6 +68,7 @@
 // every queue was drained (error 2 otherwise, e.g. a queue no XCD could
 // claim), and the host then evaluates the swarm on the lock-step path.
 #include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "fb_tiles.h"
@@ -81,20 +88,17 @@ constexpr int kCtlError = kCtlFinished + kCtlLine;
 static_assert(kCtlError + kCtlLine == kDagCtl && kCtlError == kDagErrorWord, "control word layout");
 
 struct DagArgs {
-    double *K;
-    int np, n, nb, S, Q;
-    int tpm;            // tasks per matrix
-    int hpm;            // of them on the high list
-    int ntm;            // task index space per matrix (dag_index)
-    int slot_cap;       // slots per list (tasks of the largest queue)
+    const DagMatrix *mats;   // device copy of the matrix descriptors
+    int n_mats, Q;
+    int n_hi[kDagMaxQueues], n_lo[kDagMaxQueues];   // final list lengths per queue
+    int n_first[kDagMaxQueues];                     // matrices per queue (TP(0) ready at the start)
+    int slot_off[kDagMaxQueues];                    // the queue's high list; normal list after it
+    int slot_cap[kDagMaxQueues];                    // slots per list of the queue
+    int n_dep, n_slots;
     int *ctl;
-    int *dep;           // S x ntm unfinished-predecessor counters
-    const int *dep0;    // ntm initial counters
-    unsigned long long *slots;   // Q x 2 lists x slot_cap
-    int *dead;          // S
-    int *info;
-    double *logdet_part;
-    double *dinv;       // S x nb x kDinv diagonal-block inverses
+    int *dep;                  // unfinished-predecessor counters, mats[s].dep_off onwards
+    unsigned long long *slots;
+    int *dead;                 // n_mats
 };
 
 // Dense index of a task within its matrix.
@@ -111,7 +115,28 @@ __host__ __device__ inline int dag_index(int type, int a, int i, int nb)
 
 __host__ __device__ inline int dag_index_space(int nb) { return 2 * nb * nb + nb; }
 
-// A ready-list entry: type + 1 (never zero), matrix, a, i.
+// Number of predecessors of the task with dense index idx (0 for indices
+// that name no task).
+__host__ __device__ inline int dag_initial_count(int idx, int nb)
+{
+    const int P = nb / 2;
+    if (idx < P * nb) {                                   // TG(a, i)
+        const int a = idx / nb, i = idx % nb;
+        return (a >= 1 && i >= 2 * a) ? (i <= 2 * a + 1 ? 2 : 3) : 0;
+    }
+    idx -= P * nb;
+    if (idx < nb) return idx == 0 ? 0 : 1;               // TP(b)
+    idx -= nb;
+    if (idx < nb * nb) {                                  // TT(b, i)
+        const int b = idx / nb, i = idx % nb;
+        return i > b ? (b == 0 ? 1 : 2) : 0;
+    }
+    idx -= nb * nb;                                       // TG2(a, i)
+    const int a = idx / nb, i = idx % nb;
+    return i >= 2 * a + 1 ? (i == 2 * a + 1 ? 1 : 2) : 0;
+}
+
+// A ready-list entry: type + 1 (never zero), the matrix's rank in its queue, a, i.
 __device__ inline unsigned long long dag_encode(int type, int s, int a, int i)
 {
     return (unsigned long long)(type + 1) | ((unsigned long long)s << 8) | ((unsigned long long)a << 24) |
@@ -142,6 +167,19 @@ __device__ inline int xcc_id()
 
 constexpr unsigned long long kWaitTicks = 200000000ull;   // 2 s of the 100 MHz real-time clock
 
+// Values every lane holds alike, made visibly uniform to the compiler so
+// that strides and addresses derived from them stay in scalar registers.
+__device__ inline int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+template <typename T>
+__device__ inline T *uni(T *p)
+{
+    const unsigned long long v = (unsigned long long)p;
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)v);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v >> 32));
+    return (T *)(((unsigned long long)hi << 32) | lo);
+}
+
 __host__ __device__ inline bool dag_is_high(int type, int a, int i)
 {
     const int panel = type == kTaskTT ? a / 2 : a;
@@ -168,8 +206,8 @@ __device__ inline unsigned long long dag_pop(const DagArgs &a, int q, int n_hi, 
 {
     int *error = a.ctl + kCtlError;
     int *hi_head = a.ctl + ctl_word(q, kHiHead);
-    const unsigned long long *hi = a.slots + (size_t)q * 2 * a.slot_cap;
-    const unsigned long long *lo = hi + a.slot_cap;
+    const unsigned long long *hi = a.slots + a.slot_off[q];
+    const unsigned long long *lo = hi + a.slot_cap[q];
     unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
     int sl = 1;
     for (int k = 0;; ++k) {
@@ -209,29 +247,33 @@ __device__ inline unsigned long long dag_pop(const DagArgs &a, int q, int n_hi, 
 
 // Decrements the counters of dependants (type, a, i) for i in [lo, hi) of
 // matrix s and pushes those that become ready.  Work-group cooperative.
-__device__ inline void dag_release(const DagArgs &a, int q, int s, int type, int da, int lo, int hi)
+__device__ inline void dag_release(const DagArgs &a, int q, int rank, int nb, int dep_off, int type, int da,
+                                   int lo, int hi)
 {
-    int *dep = a.dep + (size_t)s * a.ntm;
-    unsigned long long *lists = a.slots + (size_t)q * 2 * a.slot_cap;
+    int *dep = a.dep + dep_off;
+    unsigned long long *lists = a.slots + a.slot_off[q];
     for (int i = lo + (int)threadIdx.x; i < hi; i += kThreads) {
-        if (atomicSub(dep + dag_index(type, da, i, a.nb), 1) == 1) {
+        if (atomicSub(dep + dag_index(type, da, i, nb), 1) == 1) {
             const int l = dag_is_high(type, da, i) ? 0 : 1;
             const int slot = atomicAdd(a.ctl + ctl_word(q, l == 0 ? kHiTail : kLoTail), 1);
-            __hip_atomic_store(lists + (size_t)l * a.slot_cap + slot, dag_encode(type, s, da, i), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(lists + (size_t)l * a.slot_cap[q] + slot, dag_encode(type, rank, da, i),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
 }
 
 constexpr int kDagSmem = TileShape<kMB, kNBO>::SMEM > kPotrfSmem ? TileShape<kMB, kNBO>::SMEM : kPotrfSmem;
 
-__global__ void __launch_bounds__(kThreads) dag_factor_kernel(DagArgs a)
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, 1))) dag_factor_kernel(DagArgs a)
 {
     __shared__ __attribute__((aligned(16))) double smem[kDagSmem];
     __shared__ unsigned long long sh_task;
     __shared__ int sh_queue;
+    // the descriptors of this queue's matrices, by rank (read once: a global
+    // read per task would come after the acquire's cache invalidation)
+    __shared__ DagMatrix sh_mats[kDagMaxQueueMats];
+    __shared__ int sh_index[kDagMaxQueueMats];
     const int tid = threadIdx.x;
-    const int P = a.nb / 2;
     if (tid == 0) {
         const int x = xcc_id();
         const int q = x % a.Q;
@@ -240,32 +282,44 @@ __global__ void __launch_bounds__(kThreads) dag_factor_kernel(DagArgs a)
     }
     __syncthreads();
     const int q = sh_queue;
-    const int nmat = q >= 0 ? (a.S - q + a.Q - 1) / a.Q : 0;
-    const int n_hi = nmat * a.hpm, n_lo = nmat * (a.tpm - a.hpm);
+    if (q >= 0)
+        for (int m = tid; m < a.n_mats; m += kThreads) {
+            const DagMatrix d = a.mats[m];
+            if (d.queue == q) {
+                sh_mats[d.rank] = d;
+                sh_index[d.rank] = m;
+            }
+        }
+    __syncthreads();
+    const int n_hi = q >= 0 ? a.n_hi[q] : 0, n_lo = q >= 0 ? a.n_lo[q] : 0;
     int ticket = -1;   // thread 0's ticket on the normal list
     while (q >= 0) {
         if (tid == 0) sh_task = dag_pop(a, q, n_hi, n_lo, &ticket);
         __syncthreads();
         const unsigned long long v = sh_task;
         if (v == 0) break;
-        const int type = (int)(v & 0xff) - 1;
-        const int s = (int)((v >> 8) & 0xffff);
-        const int ta = (int)((v >> 24) & 0xffff);
-        const int ti = (int)((v >> 40) & 0xffff);
+        const int type = uni((int)(v & 0xff) - 1);
+        const int rank = uni((int)((v >> 8) & 0xffff));
+        const int ta = uni((int)((v >> 24) & 0xffff));
+        const int ti = uni((int)((v >> 40) & 0xffff));
         // every wave drops stale L1 copies of the producers' tiles; all of the
         // task's loads come after the barrier
         xcd_acquire();
         __syncthreads();
-        double *A = a.K + (size_t)s * a.np * a.np;
+        const DagMatrix &Ml = sh_mats[rank];
+        const int s = uni(sh_index[rank]);
+        double *A = uni(Ml.K);
+        double *Mdinv = uni(Ml.dinv);
+        const int np = uni(Ml.np), nb = uni(Ml.nb), Mn = uni(Ml.n), dep_off = uni(Ml.dep_off), P = nb / 2;
         const bool live = ld_relaxed(a.dead + s) == 0;
         int rtype = kTaskTT, ra = 0, rlo = 0, rhi = 0;   // dependants to release
         if (type == kTaskTG) {
             const int p = ta, i = ti;
             if (live) {
                 if (i == 2 * p)   // only the lower 64 x 64 block of the diagonal row is used
-                    gemm_tile_update<kMB, kNBI>(A, a.np, i * kNBI, p * kNBO, 0, p * kNBO, smem);
+                    gemm_tile_update<kMB, kNBI>(A, np, i * kNBI, p * kNBO, 0, p * kNBO, smem);
                 else
-                    gemm_tile_update<kMB, kNBO>(A, a.np, i * kNBI, p * kNBO, 0, p * kNBO, smem);
+                    gemm_tile_update<kMB, kNBO>(A, np, i * kNBI, p * kNBO, 0, p * kNBO, smem);
             }
             if (i == 2 * p) { rtype = kTaskTP; ra = i; rlo = 0; rhi = 1; }
             else { rtype = kTaskTT; ra = 2 * p; rlo = i; rhi = i + 1; }
@@ -273,78 +327,77 @@ __global__ void __launch_bounds__(kThreads) dag_factor_kernel(DagArgs a)
             const int b = ta;
             if (live) {
                 double lg = 0.0;
-                const int fail = potrf64_block(A, a.np, a.n, b * kNBI, &lg, smem,
-                                               a.dinv + ((size_t)s * a.nb + b) * kDinv);
+                const int fail = potrf64_block(A, np, Mn, b * kNBI, &lg, smem, Mdinv + (size_t)b * kDinv);
                 if (tid == 0) {
                     if (fail) {
-                        a.info[s] = fail;
+                        *Ml.info = fail;
                         st_relaxed(a.dead + s, 1);
                     } else {
-                        a.logdet_part[(size_t)s * a.nb + b] = lg;
+                        Ml.logdet_part[b] = lg;
                     }
                 }
             }
-            rtype = kTaskTT; ra = b; rlo = b + 1; rhi = a.nb;
+            rtype = kTaskTT; ra = b; rlo = b + 1; rhi = nb;
         } else if (type == kTaskTT) {
             const int b = ta, i = ti, p = b / 2;
-            if (live) trsm64_mfma(A, a.np, b * kNBI, i * kNBI, a.dinv + ((size_t)s * a.nb + b) * kDinv);
+            if (live) trsm64_mfma(A, np, b * kNBI, i * kNBI, Mdinv + (size_t)b * kDinv);
             if ((b & 1) == 0) {
                 rtype = kTaskTG2; ra = p;
-                if (i == b + 1) { rlo = b + 1; rhi = a.nb; } else { rlo = i; rhi = i + 1; }
+                if (i == b + 1) { rlo = b + 1; rhi = nb; } else { rlo = i; rhi = i + 1; }
             } else if (p + 1 < P) {
                 rtype = kTaskTG; ra = p + 1;
-                if (i == b + 1 || i == b + 2) { rlo = b + 1; rhi = a.nb; } else { rlo = i; rhi = i + 1; }
+                if (i == b + 1 || i == b + 2) { rlo = b + 1; rhi = nb; } else { rlo = i; rhi = i + 1; }
             }
         } else {
             const int p = ta, i = ti, b0 = 2 * p, b1 = 2 * p + 1;
-            if (live) gemm_tile_update<kMB, kNBI>(A, a.np, i * kNBI, b1 * kNBI, b0 * kNBI, b1 * kNBI, smem);
+            if (live) gemm_tile_update<kMB, kNBI>(A, np, i * kNBI, b1 * kNBI, b0 * kNBI, b1 * kNBI, smem);
             if (i == b1) { rtype = kTaskTP; ra = b1; rlo = 0; rhi = 1; }
             else { rtype = kTaskTT; ra = b1; rlo = i; rhi = i + 1; }
         }
         // publish: every wave's stores reach L2, then the dependants are released
         xcd_release();
         __syncthreads();
-        dag_release(a, q, s, rtype, ra, rlo, rhi);
+        dag_release(a, q, rank, nb, dep_off, rtype, ra, rlo, rhi);
     }
     // the last work-group out checks that every queue was drained
     if (tid == 0) {
         if (atomicAdd(a.ctl + kCtlFinished, 1) == (int)gridDim.x - 1) {
-            for (int k = 0; k < a.Q; ++k) {
-                const int m = (a.S - k + a.Q - 1) / a.Q;
-                if (ld_relaxed(a.ctl + ctl_word(k, kHiHead)) < m * a.hpm ||
-                    ld_relaxed(a.ctl + ctl_word(k, kLoHead)) < m * (a.tpm - a.hpm))
+            for (int k = 0; k < a.Q; ++k)
+                if (ld_relaxed(a.ctl + ctl_word(k, kHiHead)) < a.n_hi[k] ||
+                    ld_relaxed(a.ctl + ctl_word(k, kLoHead)) < a.n_lo[k])
                     st_relaxed(a.ctl + kCtlError, 2);
-            }
         }
     }
 }
 
-// Fresh counters, empty lists holding only TP(0) of every matrix, control words.
+// Fresh counters (matrix blockIdx.y), empty lists holding only TP(0) of
+// every matrix, control words (blockIdx.y = n_mats).
 __global__ void dag_reset_kernel(DagArgs a)
 {
-    const size_t n_dep = (size_t)a.S * a.ntm;
-    const size_t n_slot = (size_t)a.Q * 2 * a.slot_cap;
-    const size_t total = n_dep + n_slot + a.S + kDagCtl;
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
-        if (e < n_dep) {
-            a.dep[e] = a.dep0[e % a.ntm];
-        } else if (e < n_dep + n_slot) {
-            const size_t k = e - n_dep;
-            const int q = (int)(k / (2 * (size_t)a.slot_cap));
-            const int l = (int)((k / a.slot_cap) % 2);
-            const int idx = (int)(k % a.slot_cap);
-            const int s = q + idx * a.Q;
-            a.slots[k] = (l == 0 && s < a.S) ? dag_encode(kTaskTP, s, 0, 0) : 0ull;
-        } else if (e < n_dep + n_slot + a.S) {
-            a.dead[e - n_dep - n_slot] = 0;
-        } else {
-            const int w = (int)(e - n_dep - n_slot - a.S);
-            const int q = w / (kCtlPerQueue * kCtlLine), k = (w / kCtlLine) % kCtlPerQueue;
-            int v = 0;
-            if (q < a.Q && k == kHiTail) v = (a.S - q + a.Q - 1) / a.Q;   // the TP(0)s
-            if (q < kDagMaxQueues && k == kOwner) v = -1;
-            a.ctl[w] = v;
+    const int y = blockIdx.y;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t e0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (y < a.n_mats) {
+        const DagMatrix M = a.mats[y];
+        const int ntm = dag_index_space(M.nb);
+        for (size_t e = e0; e < (size_t)ntm; e += stride) a.dep[M.dep_off + e] = dag_initial_count((int)e, M.nb);
+        if (e0 == 0) {
+            a.dead[y] = 0;
+            a.slots[a.slot_off[M.queue] + M.rank] = dag_encode(kTaskTP, M.rank, 0, 0);
         }
+        return;
+    }
+    for (size_t e = e0; e < (size_t)a.n_slots; e += stride) {
+        int q = 0;
+        while (q + 1 < a.Q && (size_t)a.slot_off[q + 1] <= e) ++q;
+        if (e - a.slot_off[q] >= (size_t)a.n_first[q]) a.slots[e] = 0ull;   // the TP(0) slots are the matrices'
+    }
+    for (size_t w = e0; w < (size_t)kDagCtl; w += stride) {
+        const int q = (int)(w / (kCtlPerQueue * kCtlLine)), k = (int)((w / kCtlLine) % kCtlPerQueue);
+        int v = 0;
+        if (q < a.Q && k == kHiTail) v = a.n_first[q];
+        if (q < kDagMaxQueues && k == kOwner) v = -1;
+        a.ctl[w] = v;
     }
 }
 
@@ -371,23 +424,54 @@ int dag_tasks_per_matrix(int np)
     return t;
 }
 
-std::vector<int> dag_initial_counts(int np)
-{
-    const int nb = np / kNBI, P = nb / 2;
-    std::vector<int> d(dag_index_space(nb), 0);
-    for (int p = 1; p < P; ++p)
-        for (int i = 2 * p; i < nb; ++i) d[dag_index(kTaskTG, p, i, nb)] = (i <= 2 * p + 1) ? 2 : 3;
-    for (int b = 0; b < nb; ++b) d[dag_index(kTaskTP, b, 0, nb)] = b == 0 ? 0 : 1;
-    for (int b = 0; b < nb; ++b)
-        for (int i = b + 1; i < nb; ++i) d[dag_index(kTaskTT, b, i, nb)] = b == 0 ? 1 : 2;
-    for (int p = 0; p < P; ++p)
-        for (int i = 2 * p + 1; i < nb; ++i) d[dag_index(kTaskTG2, p, i, nb)] = (i == 2 * p + 1) ? 1 : 2;
-    return d;
-}
-
 int dag_index_count(int np) { return dag_index_space(np / kNBI); }
 
-int dag_slot_capacity(int S, int np, int Q) { return ((S + Q - 1) / Q) * dag_tasks_per_matrix(np); }
+DagPlan dag_plan(std::vector<DagMatrix> &mats, int n_xcd)
+{
+    DagPlan pl;
+    const int n = (int)mats.size();
+    pl.Q = std::max(1, std::min(std::min(n_xcd, kDagMaxQueues), n));
+    bool uniform = true;
+    for (const DagMatrix &m : mats) uniform = uniform && m.np == mats[0].np;
+    std::vector<double> load(pl.Q, 0.0);
+    if (uniform) {
+        for (int s = 0; s < n; ++s) mats[s].queue = s % pl.Q;
+    } else {
+        // longest processing time first on the factorisation work ~ N^3
+        std::vector<int> order(n);
+        for (int s = 0; s < n; ++s) order[s] = s;
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return mats[x].n > mats[y].n; });
+        for (int s : order) {
+            const int q = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            mats[s].queue = q;
+            load[q] += std::pow((double)mats[s].n, 3.0);
+        }
+    }
+    int dep = 0;
+    std::vector<int> tasks(pl.Q, 0);
+    for (int q = 0; q < pl.Q; ++q) pl.n_hi[q] = pl.n_lo[q] = pl.n_first[q] = 0;
+    for (DagMatrix &m : mats) {
+        m.dep_off = dep;
+        dep += dag_index_space(m.nb);
+        m.rank = pl.n_first[m.queue]++;
+        const int t = dag_tasks_per_matrix(m.np), h = dag_high_tasks_per_matrix(m.np);
+        pl.n_hi[m.queue] += h;
+        pl.n_lo[m.queue] += t - h;
+        tasks[m.queue] += t;
+    }
+    int off = 0;
+    for (int q = 0; q < kDagMaxQueues; ++q) {
+        pl.slot_cap[q] = q < pl.Q ? tasks[q] : 0;
+        pl.slot_off[q] = off;
+        off += 2 * pl.slot_cap[q];
+        if (q >= pl.Q) pl.n_hi[q] = pl.n_lo[q] = pl.n_first[q] = 0;
+    }
+    pl.n_dep = dep;
+    pl.n_slots = off;
+    pl.n_mats = n;
+    for (int q = 0; q < pl.Q; ++q) pl.ok = pl.ok && pl.n_first[q] <= kDagMaxQueueMats;
+    return pl;
+}
 
 int dag_grid_size(int *n_xcd)
 {
@@ -401,22 +485,20 @@ int dag_grid_size(int *n_xcd)
     return cus * std::max(per_cu, 1);
 }
 
-int dag_queues(int S, int n_xcd) { return std::max(1, std::min(std::min(n_xcd, kDagMaxQueues), S)); }
-
-void launch_dag(const SwarmBuffers &b, const DagWork &w, int n_queues, int grid, hipStream_t st)
+void launch_dag(const DagPlan &pl, const DagMatrix *d_mats, const DagWork &w, int grid, hipStream_t st)
 {
     DagArgs a;
-    a.K = b.K; a.np = b.np; a.n = b.n; a.nb = b.np / kNBI; a.S = b.S;
-    a.Q = dag_queues(b.S, n_queues);
-    a.tpm = dag_tasks_per_matrix(b.np);
-    a.hpm = dag_high_tasks_per_matrix(b.np);
-    a.ntm = dag_index_space(a.nb);
-    a.slot_cap = dag_slot_capacity(b.S, b.np, a.Q);
-    a.ctl = w.ctl; a.dep = w.dep; a.dep0 = w.dep0; a.slots = w.slots; a.dead = w.dead;
-    a.info = b.info; a.logdet_part = b.logdet_part; a.dinv = b.dinv;
-    const size_t total = (size_t)a.S * a.ntm + (size_t)a.Q * 2 * a.slot_cap + a.S + kDagCtl;
-    const int rgrid = (int)std::min<size_t>(1024, (total + 255) / 256);
-    hipLaunchKernelGGL(dag_reset_kernel, dim3(rgrid), dim3(256), 0, st, a);
+    a.mats = d_mats;
+    a.n_mats = pl.n_mats;
+    a.Q = pl.Q;
+    for (int q = 0; q < kDagMaxQueues; ++q) {
+        a.n_hi[q] = pl.n_hi[q]; a.n_lo[q] = pl.n_lo[q]; a.n_first[q] = pl.n_first[q];
+        a.slot_off[q] = pl.slot_off[q]; a.slot_cap[q] = pl.slot_cap[q];
+    }
+    a.n_dep = pl.n_dep; a.n_slots = pl.n_slots;
+    a.ctl = w.ctl; a.dep = w.dep; a.slots = w.slots; a.dead = w.dead;
+    const int rgrid = std::max(1, std::min(64, (std::max(pl.n_slots, kDagCtl) + 255) / 256));
+    hipLaunchKernelGGL(dag_reset_kernel, dim3(rgrid, pl.n_mats + 1), dim3(256), 0, st, a);
     hipLaunchKernelGGL(dag_factor_kernel, dim3(std::max(1, grid)), dim3(kThreads), 0, st, a);
 }
 

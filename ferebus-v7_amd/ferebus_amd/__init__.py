@@ -7,9 +7,9 @@ inputs (kernel programs, mean vectors) and calls it.
 from .kernel_expr import (KernelConfig, KernelExpressionError, KernelProgram, compile_kernel,
                           parse)
 from .mean import MeanFit, fit_mean
-from .gpr import SwarmLikelihood
+from .gpr import MultiSwarm, SwarmLikelihood
 from ._abi import FbError, device_name, library_path, load
 from . import lbfgs
 
 __all__ = ["KernelConfig", "KernelExpressionError", "KernelProgram", "compile_kernel", "parse",
-           "MeanFit", "fit_mean", "SwarmLikelihood", "FbError", "device_name", "library_path", "load", "lbfgs"]
+           "MeanFit", "fit_mean", "SwarmLikelihood", "MultiSwarm", "FbError", "device_name", "library_path", "load", "lbfgs"]

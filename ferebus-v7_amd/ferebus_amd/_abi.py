@@ -23,6 +23,7 @@ EXPORTS = (
     "fb_create", "fb_destroy", "fb_nparams", "fb_set_stream", "fb_swarm_loglik",
     "fb_swarm_loglik_async", "fb_particle_loglik", "fb_get_alpha", "fb_swarm_loglik_grad",
     "fb_profile_enable", "fb_profile_read", "fb_device_name", "fb_last_error",
+    "fb_multi_create", "fb_multi_swarm_loglik", "fb_multi_destroy",
 )
 
 
@@ -76,6 +77,10 @@ def load() -> ctypes.CDLL:
         "fb_profile_read": (ctypes.c_int, [vp, ctypes.c_int, _P_DBL, ctypes.POINTER(ctypes.c_longlong)]),
         "fb_device_name": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int]),
         "fb_last_error": (ctypes.c_char_p, []),
+        "fb_multi_create": (ctypes.c_int, [ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(vp)]),
+        "fb_multi_swarm_loglik": (ctypes.c_int, [vp, _P_INT, ctypes.POINTER(_P_DBL), ctypes.POINTER(_P_DBL),
+                                                 ctypes.POINTER(_P_INT)]),
+        "fb_multi_destroy": (ctypes.c_int, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

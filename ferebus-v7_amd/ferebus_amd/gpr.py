@@ -172,3 +172,46 @@ class SwarmLikelihood:
             self.close()
         except Exception:
             pass
+
+
+class MultiSwarm:
+    """The swarms of several training sets (one SwarmLikelihood each, any N)
+    evaluated together: builds and likelihoods per training set, the
+    factorisations of all of them in one persistent launch
+    (fb_multi_swarm_loglik).  Results are bit-identical to each
+    SwarmLikelihood.swarm_loglik."""
+
+    def __init__(self, gps: Sequence[SwarmLikelihood]):
+        self._lib = _abi.load()
+        self.gps = list(gps)
+        arr = (ctypes.c_void_p * len(self.gps))(*[g._h.value for g in self.gps])
+        self._h = ctypes.c_void_p()
+        _abi.check(self._lib.fb_multi_create(ctypes.byref(self._h), len(self.gps), arr), "fb_multi_create")
+
+    def swarm_loglik(self, thetas):
+        """thetas: one (S_g, P_g) array per training set (S_g may be 0).
+        Returns ([ll_g], [info_g])."""
+        G = len(self.gps)
+        ts = [np.zeros((0, g.nparams)) if t is None else g._theta(t) if len(np.atleast_2d(t)) else
+              np.zeros((0, g.nparams)) for g, t in zip(self.gps, thetas)]
+        s = np.array([len(t) for t in ts], dtype=np.int32)
+        lls = [np.empty(len(t)) for t in ts]
+        infos = [np.empty(len(t), dtype=np.int32) for t in ts]
+        P_DBL, P_INT = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+        th_p = (P_DBL * G)(*[_abi.dptr(t) if len(t) else None for t in ts])
+        ll_p = (P_DBL * G)(*[_abi.dptr(v) if len(v) else None for v in lls])
+        in_p = (P_INT * G)(*[_abi.iptr(v) if len(v) else None for v in infos])
+        _abi.check(self._lib.fb_multi_swarm_loglik(self._h, _abi.iptr(s), th_p, ll_p, in_p),
+                   "fb_multi_swarm_loglik")
+        return lls, infos
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.fb_multi_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

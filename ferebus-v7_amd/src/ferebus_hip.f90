@@ -26,6 +26,7 @@ module ferebus_hip
 
   public :: fb_create, fb_destroy, fb_nparams, fb_set_stream, fb_swarm_loglik, &
             fb_swarm_loglik_async, fb_particle_loglik, fb_get_alpha, fb_swarm_loglik_grad, &
+            fb_multi_create, fb_multi_swarm_loglik, fb_multi_destroy, &
             fb_profile_enable, fb_profile_read, fb_device_name, fb_last_error, fb_error_message
 
   interface
@@ -100,6 +101,27 @@ module ferebus_hip
       real(c_double), intent(out) :: ll(*), grad(*)
       integer(c_int), intent(out) :: info(*)
     end function fb_swarm_loglik_grad
+
+    !> Several handles factorised in one persistent launch (uneven N).
+    !> gps, theta, ll, info: arrays of c_ptr (c_loc of each handle's arrays).
+    integer(c_int) function fb_multi_create(m, n_gps, gps) bind(c, name="fb_multi_create")
+      import :: c_int, c_ptr
+      type(c_ptr), intent(out) :: m
+      integer(c_int), value :: n_gps
+      type(c_ptr), intent(in) :: gps(*)
+    end function fb_multi_create
+
+    integer(c_int) function fb_multi_swarm_loglik(m, s, theta, ll, info) bind(c, name="fb_multi_swarm_loglik")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+      integer(c_int), intent(in) :: s(*)
+      type(c_ptr), intent(in) :: theta(*), ll(*), info(*)
+    end function fb_multi_swarm_loglik
+
+    integer(c_int) function fb_multi_destroy(m) bind(c, name="fb_multi_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+    end function fb_multi_destroy
 
     integer(c_int) function fb_profile_enable(h, enable) bind(c, name="fb_profile_enable")
       import :: c_int, c_ptr

@@ -145,6 +145,25 @@ int fb_get_alpha(fb_handle *h, const double *theta, double *alpha, double *out4,
 int fb_swarm_loglik_grad(fb_handle *h, int s, const double *theta, double *ll,
                          double *grad, int *info);
 
+/* ---- several training sets at once (uneven N) ------------------------------
+ * The swarms of several handles -- e.g. one GP per atom or property, each
+ * with its own N, kernel and likelihood -- are factorised together in ONE
+ * persistent launch; covariance builds and likelihood reductions stay per
+ * handle.  Results are bit-identical to fb_swarm_loglik on each handle.
+ * No reference counterpart: the reference trains its GPs one after another
+ * (main.f90 loops over atoms and properties; its kernel for one GP is the
+ * path of fb_swarm_loglik).  The handles are borrowed (not owned, not to be
+ * used concurrently while a multi evaluation runs) and must outlive m; all
+ * work runs on the first handle's stream.
+ *   s[g]       particles of handle g (0..its max_swarm; 0 skips it)
+ *   theta[g]   P_g * s[g] FP64 (layout of fb_swarm_loglik)
+ *   ll[g], info[g]  s[g] entries each */
+typedef struct fb_multi fb_multi;
+int fb_multi_create(fb_multi **m, int n_gps, fb_handle *const *gps);
+int fb_multi_swarm_loglik(fb_multi *m, const int *s, const double *const *theta, double *const *ll,
+                          int *const *info);
+int fb_multi_destroy(fb_multi *m);
+
 /* Kernel timing: when enabled, each launch of the handle is bracketed by HIP
  * events and the per-kernel totals accumulate until read.
  * kind: 0 covariance build, 1 trailing-update GEMM, 2 diagonal potrf,
