@@ -159,6 +159,7 @@ def main():
                                f"constant mean, marginal, nugget 1e-10, theta~U[0,1)",
                    "path": "batched (hipGraph replay per swarm evaluation, "
                            + ("persistent factorisation)" if persistent else "lock-step factorisation)"),
+                   "factor_path": gp.factor_path(S),
                    "failed_particles_last_step": fails},
         "roofline": {
             "kernel": fact_kernel,

@@ -161,6 +161,13 @@ struct fb_handle {
     int n_groups = 1;
     bool use_graph = true;
     bool use_dag = true;             // persistent factorisation for the batched path
+    int gemm_kc = 32;                // lock-step trailing-update chunk depth
+    // autotuner: (np, swarm size) -> 0 persistent, 1 lock-step KC 32, 2 lock-step KC 16
+    bool autotune = true;
+    int forced_mode = -1;            // FB_PERSISTENT=0/1 pins the path
+    std::map<int, int> tuned;        // per swarm size (np is fixed per handle)
+    std::string tune_file;           // FB_TUNE_CACHE: choices kept across processes
+    std::string device_key;          // device the choices were timed on
     int dag_grid = 0, n_xcd = 1;
     int *d_ctl = nullptr, *d_dep = nullptr, *d_dead = nullptr, *h_err = nullptr;
     // swarm size -> schedule plan and device matrix descriptors
@@ -205,7 +212,8 @@ fb::SwarmBuffers buffers(fb_handle *h, int g0, int S, const double *theta, doubl
     b.n_slots = h->n_slots; b.n_leaves = h->prog.n_leaves;
     b.slots = h->d_slots; b.RHS = h->d_rhs;
     b.hyp = h->d_hyp + (size_t)g0 * (h->prog.n_par_kernel + fb::kMaxTerms + 1);
-    b.theta = theta + (size_t)g0 * h->prog.n_par; b.prog = h->d_prog;
+    b.theta = theta + (size_t)g0 * h->prog.n_par; b.prog = h->d_prog; b.prog_host = &h->prog;
+    b.gemm_kc = h->gemm_kc;
     b.K = h->d_K + (size_t)g0 * h->np * h->np;
     b.info = info + g0; b.ll = ll + g0;
     b.logdet_part = h->d_logdet + (size_t)g0 * h->nsteps;
@@ -419,6 +427,63 @@ int fb_device_name(char *buf, int len)
     return FB_OK;
 }
 
+// Tuning cache file: a JSON array of {"device", "np", "swarm", "path", "ms"}
+// records, one per line; read at fb_create, rewritten (temp file + rename)
+// after each new choice.  Records of other devices or orders are kept.
+struct TuneRecord {
+    std::string device;
+    int np = 0, swarm = 0, path = -1;
+    double ms = 0.0;
+};
+
+static std::vector<TuneRecord> read_tune_file(const std::string &path)
+{
+    std::vector<TuneRecord> out;
+    FILE *f = std::fopen(path.c_str(), "r");
+    if (!f) return out;
+    char line[1024];
+    while (std::fgets(line, sizeof line, f)) {
+        TuneRecord r;
+        const char *d = std::strstr(line, "\"device\": \"");
+        const char *a = std::strstr(line, "\"np\": ");
+        const char *b = std::strstr(line, "\"swarm\": ");
+        const char *c = std::strstr(line, "\"path\": ");
+        const char *m = std::strstr(line, "\"ms\": ");
+        if (!d || !a || !b || !c) continue;
+        d += 11;
+        const char *e = std::strchr(d, '"');
+        if (!e) continue;
+        r.device.assign(d, e);
+        r.np = std::atoi(a + 6);
+        r.swarm = std::atoi(b + 9);
+        r.path = std::atoi(c + 8);
+        r.ms = m ? std::atof(m + 6) : 0.0;
+        if (r.path >= 0 && r.path <= 2) out.push_back(r);
+    }
+    std::fclose(f);
+    return out;
+}
+
+static void write_tune_file(const fb_handle *h, int s, int path, double ms)
+{
+    std::vector<TuneRecord> recs = read_tune_file(h->tune_file);
+    bool found = false;
+    for (auto &r : recs)
+        if (r.device == h->device_key && r.np == h->np && r.swarm == s) { r.path = path; r.ms = ms; found = true; }
+    if (!found) recs.push_back({h->device_key, h->np, s, path, ms});
+    const std::string tmp = h->tune_file + ".tmp" + std::to_string((long long)std::rand());
+    FILE *f = std::fopen(tmp.c_str(), "w");
+    if (!f) return;   // a cache that cannot be written only costs re-tuning
+    std::fprintf(f, "[\n");
+    for (size_t k = 0; k < recs.size(); ++k)
+        std::fprintf(f, "  {\"device\": \"%s\", \"np\": %d, \"swarm\": %d, \"path\": %d, \"ms\": %.4f}%s\n",
+                     recs[k].device.c_str(), recs[k].np, recs[k].swarm, recs[k].path, recs[k].ms,
+                     k + 1 < recs.size() ? "," : "");
+    std::fprintf(f, "]\n");
+    const bool ok = std::fclose(f) == 0;
+    if (!ok || std::rename(tmp.c_str(), h->tune_file.c_str()) != 0) std::remove(tmp.c_str());
+}
+
 int fb_create(fb_handle **out, int n, int d, const double *x, const double *y, const double *mean,
               const fb_kernel_spec *spec, double noise, int optimise_noise, int likelihood,
               int max_swarm)
@@ -506,8 +571,33 @@ int fb_create(fb_handle **out, int n, int d, const double *x, const double *y, c
     h->stream = h->own_stream;
     if (const char *env = std::getenv("FB_STREAMS")) h->n_groups = std::atoi(env);
     if (const char *env = std::getenv("FB_GRAPH")) h->use_graph = std::atoi(env) != 0;
-    if (const char *env = std::getenv("FB_PERSISTENT")) h->use_dag = std::atoi(env) != 0;
-    if (h->dag_grid <= 0) h->use_dag = false;
+    if (const char *env = std::getenv("FB_PERSISTENT")) {
+        h->use_dag = std::atoi(env) != 0;
+        h->forced_mode = h->use_dag ? 0 : 1;
+    }
+    if (const char *env = std::getenv("FB_AUTOTUNE")) h->autotune = std::atoi(env) != 0;
+    if (const char *env = std::getenv("FB_FACTOR_PATH")) {
+        const int m = std::atoi(env);
+        if (m >= 0 && m <= 2) h->forced_mode = m;
+    }
+    if (const char *env = std::getenv("FB_TUNE_CACHE")) {
+        h->tune_file = env;
+        hipDeviceProp_t prop;
+        int dev = 0;
+        if (!h->tune_file.empty() && hipGetDevice(&dev) == hipSuccess &&
+            hipGetDeviceProperties(&prop, dev) == hipSuccess) {
+            h->device_key = std::string(prop.gcnArchName) + "/" + std::to_string(prop.multiProcessorCount) + "cu";
+            for (const auto &r : read_tune_file(h->tune_file))
+                if (r.device == h->device_key && r.np == h->np && (r.path != 0 || h->dag_grid > 0))
+                    h->tuned[r.swarm] = r.path;
+        } else {
+            h->tune_file.clear();
+        }
+    }
+    if (h->dag_grid <= 0) {
+        h->use_dag = false;
+        if (h->forced_mode == 0) h->forced_mode = 1;
+    }
     h->n_groups = std::max(1, std::min(h->n_groups, fb_handle::kMaxGroups));
     bool ev_ok = hipEventCreateWithFlags(&h->fork_ev, hipEventDisableTiming) == hipSuccess;
     for (int g = 1; g < h->n_groups && ev_ok; ++g)
@@ -556,10 +646,20 @@ int fb_set_stream(fb_handle *h, void *stream)
     return FB_OK;
 }
 
+static void set_mode(fb_handle *h, int mode)
+{
+    h->use_dag = mode == 0;
+    h->gemm_kc = mode == 2 ? 16 : 32;
+}
+
 int fb_swarm_loglik_async(fb_handle *h, int s, const double *d_theta, double *d_ll, int *d_info)
 {
     if (!h || !d_theta || !d_ll || !d_info) return fail(FB_E_ARG, "fb_swarm_loglik_async: null argument");
     if (s < 1 || s > h->max_swarm) return fail(FB_E_CAPACITY, "fb_swarm_loglik_async: swarm size out of range");
+    {
+        auto it = h->tuned.find(s);
+        set_mode(h, h->forced_mode >= 0 ? h->forced_mode : it != h->tuned.end() ? it->second : (h->dag_grid > 0 ? 0 : 1));
+    }
     if (h->use_dag) {
         // the persistent schedule's descriptors name the handle's INFO buffer
         const int rc = dag_plan_for(h, s);
@@ -570,6 +670,58 @@ int fb_swarm_loglik_async(fb_handle *h, int s, const double *d_theta, double *d_
         enqueue_pipeline(h, s, d_theta, d_ll, d_info, h->stream);
     }
     FB_HIP(hipGetLastError());
+    return FB_OK;
+}
+
+// Picks the factorisation path for a swarm of s (theta already staged): the
+// persistent launch, or the lock-step launches with 32- or 16-deep GEMM
+// chunks, each timed twice on the real inputs, the fastest kept per swarm
+// size.  All three compute the same bits (the same tile routines and
+// accumulation order), so the choice changes only speed.  Not run while a
+// stream capture is active (the cached or default choice is used).
+static int choose_path(fb_handle *h, int s)
+{
+    if (h->forced_mode >= 0) { set_mode(h, h->forced_mode); return FB_OK; }
+    auto it = h->tuned.find(s);
+    if (it != h->tuned.end()) { set_mode(h, it->second); return FB_OK; }
+    if (!h->autotune || h->dag_grid <= 0) { set_mode(h, h->dag_grid > 0 ? 0 : 1); return FB_OK; }
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(h->stream, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) {
+        set_mode(h, 0);
+        return FB_OK;
+    }
+    hipStream_t st = h->stream;
+    hipEvent_t e0, e1;
+    FB_HIP(hipEventCreate(&e0));
+    FB_HIP(hipEventCreate(&e1));
+    double best = 1e300;
+    int best_mode = 0;
+    for (int mode = 0; mode < 3; ++mode) {
+        set_mode(h, mode);
+        if (mode == 0 && dag_plan_for(h, s) != FB_OK) continue;
+        float ms_best = 1e30f;
+        for (int rep = 0; rep < 2; ++rep) {
+            (void)hipEventRecord(e0, st);
+            enqueue_pipeline(h, s, h->d_theta, h->d_ll, h->d_info, st);
+            (void)hipEventRecord(e1, st);
+            if (hipEventSynchronize(e1) != hipSuccess) break;
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            ms_best = std::min(ms_best, ms);
+        }
+        if (mode == 0) {   // a persistent run that could not be served does not count
+            int err = 0;
+            (void)hipMemcpy(&err, h->d_ctl + fb::kDagErrorWord, sizeof(int), hipMemcpyDeviceToHost);
+            if (err != 0) ms_best = 1e30f;
+        }
+        if (ms_best < best) { best = ms_best; best_mode = mode; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    FB_HIP(hipGetLastError());
+    h->tuned[s] = best_mode;
+    set_mode(h, best_mode);
+    if (!h->tune_file.empty()) write_tune_file(h, s, best_mode, best);
     return FB_OK;
 }
 
@@ -616,20 +768,23 @@ int fb_swarm_loglik(fb_handle *h, int s, const double *theta, double *ll, int *i
     if (!h || !theta || !ll || !info) return fail(FB_E_ARG, "fb_swarm_loglik: null argument");
     if (s < 1 || s > h->max_swarm) return fail(FB_E_CAPACITY, "fb_swarm_loglik: swarm size out of range");
     const int P = h->prog.n_par;
-    if (h->use_dag) {
-        const int rc = dag_plan_for(h, s);
-        if (rc) return rc;
-    }
     std::memcpy(h->h_theta, theta, sizeof(double) * P * s);
     FB_HIP(hipMemcpyAsync(h->d_theta, h->h_theta, sizeof(double) * P * s, hipMemcpyHostToDevice, h->stream));
+    int rc = choose_path(h, s);
+    if (rc) return rc;
+    if (h->use_dag) {
+        rc = dag_plan_for(h, s);
+        if (rc) return rc;
+    }
     int err = 0;
-    int rc = run_swarm(h, s, &err);
+    rc = run_swarm(h, s, &err);
     if (rc) return rc;
     if (err == 1) return fail(FB_E_HIP, "fb_swarm_loglik: the persistent factorisation found no ready task for 2 s");
     if (err == 2) {
         // some XCD queue found no work-group of its own: this device does not
         // dispatch the way the persistent schedule assumes; use lock-step
-        h->use_dag = false;
+        h->forced_mode = 1;
+        set_mode(h, 1);
         drop_graphs(h);
         rc = run_swarm(h, s, &err);
         if (rc) return rc;
@@ -703,18 +858,23 @@ int fb_swarm_loglik_grad(fb_handle *h, int s, const double *theta, double *ll, d
         h->grad_cap = s;
     }
     h->grad.P = P;
-    if (h->use_dag) {
-        const int prc = dag_plan_for(h, s);
-        if (prc) return prc;
-    }
     std::memcpy(h->h_theta, theta, sizeof(double) * P * s);
     FB_HIP(hipMemcpyAsync(h->d_theta, h->h_theta, sizeof(double) * P * s, hipMemcpyHostToDevice, h->stream));
+    {
+        const int prc = choose_path(h, s);
+        if (prc) return prc;
+        if (h->use_dag) {
+            const int qrc = dag_plan_for(h, s);
+            if (qrc) return qrc;
+        }
+    }
     int err = 0;
     int rc = run_swarm(h, s, &err);
     if (rc) return rc;
     if (err == 1) return fail(FB_E_HIP, "fb_swarm_loglik_grad: the persistent factorisation found no ready task for 2 s");
     if (err == 2) {
-        h->use_dag = false;
+        h->forced_mode = 1;
+        set_mode(h, 1);
         drop_graphs(h);
         rc = run_swarm(h, s, &err);
         if (rc) return rc;
@@ -881,6 +1041,15 @@ int fb_multi_swarm_loglik(fb_multi *m, const int *s, const double *const *theta,
         std::memcpy(ll[g], m->gps[g]->h_ll, sizeof(double) * sv[g]);
         std::memcpy(info[g], m->gps[g]->h_info, sizeof(int) * sv[g]);
     }
+    return FB_OK;
+}
+
+int fb_factor_path(const fb_handle *h, int s, int *path)
+{
+    if (!h || !path) return fail(FB_E_ARG, "fb_factor_path: null argument");
+    if (h->forced_mode >= 0) { *path = h->forced_mode; return FB_OK; }
+    auto it = h->tuned.find(s);
+    *path = it != h->tuned.end() ? it->second : -1;
     return FB_OK;
 }
 

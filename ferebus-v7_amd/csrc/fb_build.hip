@@ -91,9 +91,12 @@ struct BuildArgs {
     const double *slots;   // n_slots x n: feature value of point i at slots[slot*n + i]
     const double *rhs;     // nrhs x n
     const double *hyp;
-    const KernelProgram *prog;
     double *K;
     int *info;
+    // by value: in the kernel-argument segment every field is read with
+    // scalar loads, so the loop bounds, dimension codes and hyperparameter
+    // addresses derived from it stay uniform (scalar) in the inner loops
+    KernelProgram prog;
 };
 
 // TB: tile order; MAXL: leaves held in registers (>= n_leaves); SP: particles
@@ -106,6 +109,7 @@ __global__ void __launch_bounds__(kThreads) build_kernel(BuildArgs a)
     extern __shared__ double lds[];
     double *row_f = lds;                        // [n_slots][TB]
     double *col_f = lds + (size_t)a.n_slots * TB;
+    double *hyp_s = lds + 2 * (size_t)a.n_slots * TB;   // [SP][hyper record]: broadcast reads
 
     // lower-triangular tile index: b = ti*(ti+1)/2 + tj, tj <= ti
     const int b = blockIdx.x;
@@ -115,13 +119,17 @@ __global__ void __launch_bounds__(kThreads) build_kernel(BuildArgs a)
     const int tj = b - ti * (ti + 1) / 2;
     const int s0 = blockIdx.y * SP;
     const int nsp = min(SP, a.S - s0);
-    const KernelProgram &p = *a.prog;
+    const KernelProgram &p = a.prog;
 
     for (int e = threadIdx.x; e < a.n_slots * TB; e += kThreads) {
         const int slot = e / TB, r = e % TB;
         const int gi = ti * TB + r, gj = tj * TB + r;
         row_f[e] = gi < a.n ? a.slots[(size_t)slot * a.n + gi] : 0.0;
         col_f[e] = gj < a.n ? a.slots[(size_t)slot * a.n + gj] : 0.0;
+    }
+    {
+        const int hs0 = hyper_stride(p);
+        for (int e = threadIdx.x; e < nsp * hs0; e += kThreads) hyp_s[e] = a.hyp[(size_t)s0 * hs0 + e];
     }
     if (b == 0 && threadIdx.x < nsp) a.info[s0 + threadIdx.x] = 0;
     __syncthreads();
@@ -168,13 +176,13 @@ __global__ void __launch_bounds__(kThreads) build_kernel(BuildArgs a)
                     }
 #pragma unroll
                     for (int s = 0; s < SP; ++s)
-                        if (s < nsp) e[s][l] = fma(a.hyp[(size_t)(s0 + s) * hs + poff + q], f, e[s][l]);
+                        if (s < nsp) e[s][l] = fma(hyp_s[s * hs + poff + q], f, e[s][l]);
                 }
             }
 #pragma unroll
             for (int s = 0; s < SP; ++s) {
                 if (s >= nsp) break;
-                const double *h = a.hyp + (size_t)(s0 + s) * hs;
+                const double *h = hyp_s + s * hs;
                 double v = 0.0;
                 for (int t = 0; t < p.n_terms; ++t) {
                     const uint32_t mask = p.term_mask[t];
@@ -204,13 +212,23 @@ __global__ void __launch_bounds__(kThreads) build_kernel(BuildArgs a)
     }
 }
 
+// Dynamic LDS of a build work-group: features of a row and a column tile, and
+// the hyperparameter records of its SP particles.
+static size_t build_lds_bytes(int TB, int n_slots, int SP, const KernelProgram &p)
+{
+    return ((size_t)2 * n_slots * TB + (size_t)SP * hyper_stride(p)) * sizeof(double);
+}
+
 template <int TB, int MAXL>
 static void launch_tb(const BuildArgs &a, int n_slots, hipStream_t st)
 {
     constexpr int SP = 8;
     const int nt = a.np / TB;
     dim3 grid(nt * (nt + 1) / 2, ceil_div(a.S, SP));
-    const size_t shm = (size_t)2 * n_slots * TB * sizeof(double);
+    const size_t shm = build_lds_bytes(TB, n_slots, SP, a.prog);
+    if (shm > 65536)   // beyond the default limit: gfx950 work-groups may use up to 160 KB
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&build_kernel<TB, MAXL, SP>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
     hipLaunchKernelGGL((build_kernel<TB, MAXL, SP>), grid, dim3(kThreads), shm, st, a);
 }
 
@@ -228,9 +246,9 @@ void launch_build(const SwarmBuffers &b, hipStream_t st)
     hipLaunchKernelGGL(hyper_kernel, dim3(b.S), dim3(64), 0, st, b.prog, b.theta, b.S, b.hyp);
     BuildArgs a;
     a.n = b.n; a.np = b.np; a.nrhs = b.nrhs; a.S = b.S; a.n_slots = b.n_slots;
-    a.slots = b.slots; a.rhs = b.RHS; a.hyp = b.hyp; a.prog = b.prog; a.K = b.K; a.info = b.info;
+    a.slots = b.slots; a.rhs = b.RHS; a.hyp = b.hyp; a.prog = *b.prog_host; a.K = b.K; a.info = b.info;
     // 64 x 64 tiles while the staged features fit 64 KB of LDS, else 32 x 32
-    if ((size_t)2 * b.n_slots * 64 * sizeof(double) <= 65536) launch_maxl<64>(a, b.n_leaves, st);
+    if (build_lds_bytes(64, b.n_slots, 8, a.prog) <= 65536) launch_maxl<64>(a, b.n_leaves, st);
     else launch_maxl<32>(a, b.n_leaves, st);
 }
 

@@ -75,6 +75,8 @@ struct SwarmBuffers {
     double *hyp;           // S x hyper record (see fb_build.hip)
     const double *theta;   // n_par x S
     const KernelProgram *prog;   // device copy
+    const KernelProgram *prog_host;   // host copy (passed by value to kernels whose loops read it)
+    int gemm_kc = 32;      // chunk depth of the lock-step trailing update (16 or 32)
     double *K;             // S x np x np
     int *info;             // S
     double *logdet_part;   // S x (np / kNBI)

@@ -22,17 +22,17 @@ namespace fb {
 
 // ---- lock-step kernels (one launch per step, every particle) ---------------
 
-template <int MB, int NB>
+template <int MB, int NB, int KC>
 __global__ void __launch_bounds__(kThreads) gemm_update_kernel(double *__restrict__ K, int np, int col0,
                                                                int row_start, int k0, int k1,
                                                                const int *__restrict__ info)
 {
-    __shared__ __attribute__((aligned(16))) double smem[TileShape<MB, NB>::SMEM];
+    __shared__ __attribute__((aligned(16))) double smem[TileShape<MB, NB, KC>::SMEM];
     const int rt_per = (np - row_start) / MB;
     const int s = blockIdx.x / rt_per;
     const int rt = blockIdx.x % rt_per;
     if (info[s] != 0) return;
-    gemm_tile_update<MB, NB>(K + (size_t)s * np * np, np, row_start + rt * MB, col0, k0, k1, smem);
+    gemm_tile_update<MB, NB, KC>(K + (size_t)s * np * np, np, row_start + rt * MB, col0, k0, k1, smem);
 }
 
 __global__ void __launch_bounds__(kThreads) trsm64_kernel(double *__restrict__ K, int np, int c0,
@@ -69,12 +69,22 @@ void launch_gemm_update(const SwarmBuffers &b, int col0, int width, int row_star
 {
     if (k1 <= k0) return;
     const int rt = (b.np - row_start) / kMB;
+    // chunk depth 32 (one work-group per CU, 115 KB of LDS) or 16 (two per
+    // CU): the same accumulation order, so the same bits, at different speed
     if (width == kNBO) {
-        hipLaunchKernelGGL((gemm_update_kernel<kMB, kNBO>), dim3(b.S * rt), dim3(kThreads), 0, st,
-                           b.K, b.np, col0, row_start, k0, k1, b.info);
+        if (b.gemm_kc == 16)
+            hipLaunchKernelGGL((gemm_update_kernel<kMB, kNBO, 16>), dim3(b.S * rt), dim3(kThreads), 0, st,
+                               b.K, b.np, col0, row_start, k0, k1, b.info);
+        else
+            hipLaunchKernelGGL((gemm_update_kernel<kMB, kNBO, 32>), dim3(b.S * rt), dim3(kThreads), 0, st,
+                               b.K, b.np, col0, row_start, k0, k1, b.info);
     } else {
-        hipLaunchKernelGGL((gemm_update_kernel<kMB, kNBI>), dim3(b.S * rt), dim3(kThreads), 0, st,
-                           b.K, b.np, col0, row_start, k0, k1, b.info);
+        if (b.gemm_kc == 16)
+            hipLaunchKernelGGL((gemm_update_kernel<kMB, kNBI, 16>), dim3(b.S * rt), dim3(kThreads), 0, st,
+                               b.K, b.np, col0, row_start, k0, k1, b.info);
+        else
+            hipLaunchKernelGGL((gemm_update_kernel<kMB, kNBI, 32>), dim3(b.S * rt), dim3(kThreads), 0, st,
+                               b.K, b.np, col0, row_start, k0, k1, b.info);
     }
 }
 

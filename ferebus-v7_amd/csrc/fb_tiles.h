@@ -49,9 +49,10 @@ struct TileShape {
 // C[r0 + l%16][c0 + 4v + l/16] -- sixteen consecutive rows of four columns
 // per load, which coalesces.  C is therefore read into registers before the
 // K loop (its latency hidden behind the loop) and written back straight from
-// the accumulators: no LDS epilogue, no barrier after the loop.  Each chunk
-// of KC is fetched into registers while the previous one feeds the matrix
-// cores from LDS.
+// the accumulators: no LDS epilogue, no barrier after the loop.  K is
+// streamed in chunks of KC through two LDS buffers; the global loads of a
+// chunk are issued two chunks ahead (two register sets), so each has two
+// chunks of matrix-core work to land.
 template <int MB, int NB, int KC>
 __device__ inline void tile_gemm_sub(double *__restrict__ Cg, int ldc,
                                      const double *__restrict__ Ag, int lda,
@@ -95,45 +96,35 @@ __device__ inline void tile_gemm_sub(double *__restrict__ Cg, int ldc,
             b_off[u] = col * ldb + r2;
             b_lds[u] = col * T::LDB + r2;
         }
-        // staging registers as scalar pairs: arrays of double2 are not
-        // promoted to registers by the compiler and would live in scratch
-        double rax[NA], ray[NA], rbx[NBL], rby[NBL];
-#pragma unroll
-        for (int u = 0; u < NA; ++u) {
-            const double2 t = *reinterpret_cast<const double2 *>(Ag + a_off[u]);
-            rax[u] = t.x; ray[u] = t.y;
-        }
-#pragma unroll
-        for (int u = 0; u < NBL; ++u) {
-            const double2 t = *reinterpret_cast<const double2 *>(Bg + b_off[u]);
-            rbx[u] = t.x; rby[u] = t.y;
-        }
-#pragma unroll
-        for (int u = 0; u < NA; ++u) *reinterpret_cast<double2 *>(smem + a_lds[u]) = double2{rax[u], ray[u]};
-#pragma unroll
-        for (int u = 0; u < NBL; ++u)
-            *reinterpret_cast<double2 *>(smem + 2 * T::ASZ + b_lds[u]) = double2{rbx[u], rby[u]};
-        __syncthreads();
-
-        const int nchunks = depth / KC;
-        for (int c = 0; c < nchunks; ++c) {
-            const int cur = c & 1;
-            const bool more = c + 1 < nchunks;
-            if (more) {
-                const size_t ka = (size_t)(c + 1) * KC * lda, kb = (size_t)(c + 1) * KC * ldb;
-#pragma unroll
-                for (int u = 0; u < NA; ++u) {
-                    const double2 t = *reinterpret_cast<const double2 *>(Ag + ka + a_off[u]);
-                    rax[u] = t.x; ray[u] = t.y;
-                }
-#pragma unroll
-                for (int u = 0; u < NBL; ++u) {
-                    const double2 t = *reinterpret_cast<const double2 *>(Bg + kb + b_off[u]);
-                    rbx[u] = t.x; rby[u] = t.y;
-                }
-            }
-            const double *as = smem + cur * T::ASZ;
-            const double *bs = smem + 2 * T::ASZ + cur * T::BSZ;
+        // two register staging sets (scalar pairs: arrays of double2 would
+        // live in scratch): the loads of chunk c + 2 are issued before chunk
+        // c is computed, so every load has two chunks of MFMA work to land
+        double r0ax[NA], r0ay[NA], r0bx[NBL], r0by[NBL];
+        double r1ax[NA], r1ay[NA], r1bx[NBL], r1by[NBL];
+#define FB_LOAD(SET, CH)                                                                        \
+        do {                                                                                    \
+            const size_t ka = (size_t)(CH) * KC * lda, kb = (size_t)(CH) * KC * ldb;            \
+            _Pragma("unroll") for (int u = 0; u < NA; ++u) {                                    \
+                const double2 t = *reinterpret_cast<const double2 *>(Ag + ka + a_off[u]);      \
+                r##SET##ax[u] = t.x; r##SET##ay[u] = t.y;                                       \
+            }                                                                                   \
+            _Pragma("unroll") for (int u = 0; u < NBL; ++u) {                                   \
+                const double2 t = *reinterpret_cast<const double2 *>(Bg + kb + b_off[u]);      \
+                r##SET##bx[u] = t.x; r##SET##by[u] = t.y;                                       \
+            }                                                                                   \
+        } while (0)
+#define FB_STORE(SET, BUF)                                                                      \
+        do {                                                                                    \
+            double *an = smem + (BUF) * T::ASZ;                                                 \
+            double *bn = smem + 2 * T::ASZ + (BUF) * T::BSZ;                                    \
+            _Pragma("unroll") for (int u = 0; u < NA; ++u)                                      \
+                *reinterpret_cast<double2 *>(an + a_lds[u]) = double2{r##SET##ax[u], r##SET##ay[u]}; \
+            _Pragma("unroll") for (int u = 0; u < NBL; ++u)                                     \
+                *reinterpret_cast<double2 *>(bn + b_lds[u]) = double2{r##SET##bx[u], r##SET##by[u]}; \
+        } while (0)
+        auto compute = [&](int buf) {
+            const double *as = smem + buf * T::ASZ;
+            const double *bs = smem + 2 * T::ASZ + buf * T::BSZ;
 #pragma unroll
             for (int kk = 0; kk < KC / 4; ++kk) {
                 double af[T::BM], bf[T::BN];
@@ -146,16 +137,27 @@ __device__ inline void tile_gemm_sub(double *__restrict__ Cg, int ldc,
 #pragma unroll
                     for (int j = 0; j < T::BN; ++j) acc[i][j] = mfma_f64(bf[j], af[i], acc[i][j]);
             }
-            if (more) {
-                double *an = smem + (cur ^ 1) * T::ASZ;
-                double *bn = smem + 2 * T::ASZ + (cur ^ 1) * T::BSZ;
-#pragma unroll
-                for (int u = 0; u < NA; ++u) *reinterpret_cast<double2 *>(an + a_lds[u]) = double2{rax[u], ray[u]};
-#pragma unroll
-                for (int u = 0; u < NBL; ++u) *reinterpret_cast<double2 *>(bn + b_lds[u]) = double2{rbx[u], rby[u]};
-            }
+        };
+        const int nchunks = depth / KC;
+        FB_LOAD(0, 0);
+        FB_STORE(0, 0);
+        if (nchunks > 1) FB_LOAD(1, 1);
+        __syncthreads();
+        for (int c = 0; c < nchunks; c += 2) {
+            // chunk c in LDS buffer 0, chunk c + 1 in flight in set 1
+            if (c + 2 < nchunks) FB_LOAD(0, c + 2);
+            compute(0);
+            if (c + 1 < nchunks) FB_STORE(1, 1);
+            __syncthreads();
+            if (c + 1 >= nchunks) break;
+            // chunk c + 1 in LDS buffer 1, chunk c + 2 in flight in set 0
+            if (c + 3 < nchunks) FB_LOAD(1, c + 3);
+            compute(1);
+            if (c + 2 < nchunks) FB_STORE(0, 0);
             __syncthreads();
         }
+#undef FB_LOAD
+#undef FB_STORE
     }
 #pragma unroll
     for (int i = 0; i < T::BM; ++i)

@@ -22,7 +22,7 @@ ERRORS = {-1: "FB_E_ARG", -2: "FB_E_SPEC", -3: "FB_E_HIP", -4: "FB_E_NOMEM",
 EXPORTS = (
     "fb_create", "fb_destroy", "fb_nparams", "fb_set_stream", "fb_swarm_loglik",
     "fb_swarm_loglik_async", "fb_particle_loglik", "fb_get_alpha", "fb_swarm_loglik_grad",
-    "fb_profile_enable", "fb_profile_read", "fb_device_name", "fb_last_error",
+    "fb_profile_enable", "fb_profile_read", "fb_factor_path", "fb_device_name", "fb_last_error",
     "fb_multi_create", "fb_multi_swarm_loglik", "fb_multi_destroy",
 )
 
@@ -74,6 +74,7 @@ def load() -> ctypes.CDLL:
         "fb_get_alpha": (ctypes.c_int, [vp, _P_DBL, _P_DBL, _P_DBL, _P_INT]),
         "fb_swarm_loglik_grad": (ctypes.c_int, [vp, ctypes.c_int, _P_DBL, _P_DBL, _P_DBL, _P_INT]),
         "fb_profile_enable": (ctypes.c_int, [vp, ctypes.c_int]),
+        "fb_factor_path": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
         "fb_profile_read": (ctypes.c_int, [vp, ctypes.c_int, _P_DBL, ctypes.POINTER(ctypes.c_longlong)]),
         "fb_device_name": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int]),
         "fb_last_error": (ctypes.c_char_p, []),

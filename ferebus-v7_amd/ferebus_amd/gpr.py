@@ -146,6 +146,14 @@ class SwarmLikelihood:
     def set_stream(self, stream_handle: Optional[int]) -> None:
         _abi.check(self._lib.fb_set_stream(self._h, ctypes.c_void_p(stream_handle or 0)), "fb_set_stream")
 
+    FACTOR_PATHS = {-1: "untuned", 0: "persistent", 1: "lockstep-kc32", 2: "lockstep-kc16"}
+
+    def factor_path(self, s: int) -> str:
+        """Factorisation path chosen for a swarm of s (see fb_factor_path)."""
+        p = ctypes.c_int(-1)
+        _abi.check(self._lib.fb_factor_path(self._h, int(s), ctypes.byref(p)), "fb_factor_path")
+        return self.FACTOR_PATHS[p.value]
+
     def profile(self, enable: bool) -> None:
         _abi.check(self._lib.fb_profile_enable(self._h, int(enable)), "fb_profile_enable")
 

@@ -27,7 +27,7 @@ module ferebus_hip
   public :: fb_create, fb_destroy, fb_nparams, fb_set_stream, fb_swarm_loglik, &
             fb_swarm_loglik_async, fb_particle_loglik, fb_get_alpha, fb_swarm_loglik_grad, &
             fb_multi_create, fb_multi_swarm_loglik, fb_multi_destroy, &
-            fb_profile_enable, fb_profile_read, fb_device_name, fb_last_error, fb_error_message
+            fb_profile_enable, fb_profile_read, fb_factor_path, fb_device_name, fb_last_error, fb_error_message
 
   interface
     integer(c_int) function fb_create(h, n, d, x, y, mean, spec, noise, optimise_noise, &
@@ -136,6 +136,13 @@ module ferebus_hip
       real(c_double), intent(out) :: total_ms
       integer(c_long_long), intent(out) :: launches
     end function fb_profile_read
+
+    integer(c_int) function fb_factor_path(h, s, path) bind(c, name="fb_factor_path")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: s
+      integer(c_int), intent(out) :: path
+    end function fb_factor_path
 
     integer(c_int) function fb_device_name(buf, len) bind(c, name="fb_device_name")
       import :: c_int, c_char

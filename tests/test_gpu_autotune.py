@@ -1,0 +1,79 @@
+"""Factorisation paths and the autotuner that picks among them.
+
+The batched evaluation can factorise the swarm with one persistent launch
+(path 0) or with lock-step launches whose trailing-update GEMM stages 32-deep
+(path 1) or 16-deep (path 2) chunks.  All three run the same tile routines in
+the same accumulation order, so their results must be bit-identical; the
+autotuner (first call per swarm size) may then pick any of them.  Each path
+is also held to the oracle.
+"""
+import numpy as np
+import pytest
+
+from ferebus_amd import SwarmLikelihood, compile_kernel, fit_mean, synthetic
+from test_gpu_parity import RTOL, _check, _lapack, _oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _gp(x, y, prog, mfit, S, noise=1e-6):
+    return SwarmLikelihood(x, y, prog, mean=mfit, noise=noise, max_swarm=S)
+
+
+def _problem(n, d, seed):
+    rng = np.random.default_rng(seed)
+    x = synthetic.alf_features(rng, n, d)
+    y = synthetic.bump_target(rng, x, nbumps=5)
+    prog = compile_kernel("k1+k2", synthetic.rbf_periodic_kernels(d), d)
+    return rng, x, y, prog, fit_mean("constant", x, y)
+
+
+@pytest.mark.parametrize("n,S", [(300, 5), (641, 9), (1000, 1)])
+def test_paths_bit_identical(gpu, monkeypatch, n, S):
+    rng, x, y, prog, mfit = _problem(n, 6, seed=n)
+    out = {}
+    for path in (0, 1, 2):
+        monkeypatch.setenv("FB_FACTOR_PATH", str(path))
+        gp = _gp(x, y, prog, mfit, S)
+        if path == 0:
+            th = rng.uniform(0.0, 1.0, (S, gp.nparams))
+        ll, info = gp.swarm_loglik(th)
+        ll2, info2 = gp.swarm_loglik(th)          # graph replay
+        assert np.array_equal(ll, ll2) and np.array_equal(info, info2)
+        assert gp.factor_path(S) == SwarmLikelihood.FACTOR_PATHS[path]
+        out[path] = (ll, info)
+    print({p: o[1].tolist() for p, o in out.items()})
+    for path in (1, 2):
+        assert np.array_equal(out[path][1], out[0][1]), f"path {path}: INFO differs"
+        ok = out[0][1] == 0
+        assert np.array_equal(out[path][0][ok], out[0][0][ok]), f"path {path}: ll differs"
+    oll, oinfo = _oracle(x, y, mfit.value, prog, th, 1e-6, False, "marginal")
+    _check(out[0][0], out[0][1], oll, oinfo, RTOL, f"n={n} S={S}",
+           _lapack(x, y, mfit.value, prog, th, 1e-6, False, "marginal"))
+
+
+def test_autotune_picks_and_keeps(gpu, monkeypatch):
+    monkeypatch.delenv("FB_FACTOR_PATH", raising=False)
+    monkeypatch.delenv("FB_PERSISTENT", raising=False)
+    monkeypatch.delenv("FB_AUTOTUNE", raising=False)
+    rng, x, y, prog, mfit = _problem(900, 6, seed=3)
+    gp = _gp(x, y, prog, mfit, 12)
+    th = rng.uniform(0.0, 1.0, (12, gp.nparams))
+    assert gp.factor_path(12) == "untuned"
+    ll, info = gp.swarm_loglik(th)
+    chosen = gp.factor_path(12)
+    print("autotuned path for S=12:", chosen)
+    assert chosen in ("persistent", "lockstep-kc32", "lockstep-kc16")
+    ll2, info2 = gp.swarm_loglik(th)
+    assert gp.factor_path(12) == chosen
+    assert np.array_equal(ll, ll2) and np.array_equal(info, info2)
+    # a smaller swarm is tuned on its own first call, the gradient call included
+    assert gp.factor_path(4) == "untuned"
+    l4, g4, i4 = gp.swarm_grad(th[:4])
+    assert gp.factor_path(4) != "untuned"
+    assert np.array_equal(i4, info[:4]) and np.array_equal(l4[i4 == 0], ll[:4][i4 == 0])
+    # every path gives the same bits as the one chosen
+    monkeypatch.setenv("FB_FACTOR_PATH", "2")
+    gp2 = _gp(x, y, prog, mfit, 12)
+    ll3, info3 = gp2.swarm_loglik(th)
+    assert np.array_equal(info3, info) and np.array_equal(ll3[info == 0], ll[info == 0])
