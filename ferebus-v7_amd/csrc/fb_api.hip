@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <string>
 #include <vector>
@@ -192,6 +193,7 @@ struct fb_handle {
     int *h_info = nullptr;
     // captured pipelines of the batched path, by swarm size
     std::map<int, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
+    int graph_epoch = 0;             // bumped whenever captured work goes stale
     // profiling
     bool profile = false;
     std::vector<EventPair> pending;
@@ -370,6 +372,7 @@ void drop_graphs(fb_handle *h)
         (void)hipGraphDestroy(g.second.first);
     }
     h->graphs.clear();
+    ++h->graph_epoch;
 }
 
 void free_grad(fb_handle *h)
@@ -1041,6 +1044,316 @@ int fb_multi_swarm_loglik(fb_multi *m, const int *s, const double *const *theta,
         std::memcpy(ll[g], m->gps[g]->h_ll, sizeof(double) * sv[g]);
         std::memcpy(info[g], m->gps[g]->h_info, sizeof(int) * sv[g]);
     }
+    return FB_OK;
+}
+
+
+// ---- particle swarm on the device ------------------------------------------
+
+struct fb_pso {
+    fb_handle *h = nullptr;
+    fb::PsoState st;
+    double *d_r = nullptr, *h_r = nullptr;            // r1 | r2, device and pinned staging
+    double *h_cost = nullptr, *h_state = nullptr;     // pinned read-back
+    int *h_info = nullptr, *h_gidx = nullptr;
+    int *d_idx = nullptr, *h_idx = nullptr;           // retry lists
+    double *d_xs = nullptr, *h_xs = nullptr;          // retry positions | velocities
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int graph_epoch = -1;
+    int graph_mode = -1;                              // factorisation path captured
+};
+
+namespace {
+
+void pso_drop_graph(fb_pso *p)
+{
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    p->exec = nullptr;
+    p->graph = nullptr;
+}
+
+void free_pso(fb_pso *p)
+{
+    pso_drop_graph(p);
+    for (void *q : {(void *)p->st.x, (void *)p->st.v, (void *)p->st.pbest, (void *)p->st.pcost,
+                    (void *)p->st.gbest, (void *)p->st.cost, (void *)p->st.state, (void *)p->st.info,
+                    (void *)p->st.pending, (void *)p->st.gidx, (void *)p->d_r, (void *)p->d_idx, (void *)p->d_xs})
+        if (q) (void)hipFree(q);
+    for (void *q : {(void *)p->h_r, (void *)p->h_cost, (void *)p->h_state, (void *)p->h_info, (void *)p->h_gidx,
+                    (void *)p->h_idx, (void *)p->h_xs})
+        if (q) (void)hipHostFree(q);
+}
+
+// The swarm's evaluation at its current positions (ll / INFO of the handle,
+// then costs), on the handle's chosen path for S.
+void pso_enqueue_eval(fb_pso *p, hipStream_t st)
+{
+    fb_handle *h = p->h;
+    enqueue_pipeline(h, p->st.S, p->st.x, h->d_ll, h->d_info, st);
+    fb::launch_pso_cost(p->st, h->d_ll, h->d_info, st);
+}
+
+// Path for the swarm size and its schedule; the positions are copied to the
+// handle's theta buffer first, since a first call times the candidates there.
+int pso_prepare(fb_pso *p)
+{
+    fb_handle *h = p->h;
+    const int S = p->st.S;
+    if (!h->tuned.count(S) && h->forced_mode < 0)
+        FB_HIP(hipMemcpyAsync(h->d_theta, p->st.x, sizeof(double) * S * p->st.P, hipMemcpyDeviceToDevice,
+                              h->stream));
+    int rc = choose_path(h, S);
+    if (rc) return rc;
+    if (h->use_dag) {
+        rc = dag_plan_for(h, S);
+        if (rc) return rc;
+    }
+    return FB_OK;
+}
+
+// Reads back costs / INFO (and the persistent path's error word); on a
+// device whose dispatch the persistent schedule cannot use, re-evaluates the
+// current positions on the lock-step path.
+int pso_finish(fb_pso *p, double *cost, int *info)
+{
+    fb_handle *h = p->h;
+    const int S = p->st.S;
+    hipStream_t st = h->stream;
+    FB_HIP(hipMemcpyAsync(p->h_cost, p->st.cost, sizeof(double) * S, hipMemcpyDeviceToHost, st));
+    FB_HIP(hipMemcpyAsync(p->h_info, p->st.info, sizeof(int) * S, hipMemcpyDeviceToHost, st));
+    *h->h_err = 0;
+    if (h->use_dag)
+        FB_HIP(hipMemcpyAsync(h->h_err, h->d_ctl + fb::kDagErrorWord, sizeof(int), hipMemcpyDeviceToHost, st));
+    FB_HIP(hipStreamSynchronize(st));
+    FB_HIP(hipGetLastError());
+    if (*h->h_err == 1) return fail(FB_E_HIP, "PSO: the persistent factorisation found no ready task for 2 s");
+    if (*h->h_err == 2) {
+        h->forced_mode = 1;
+        set_mode(h, 1);
+        drop_graphs(h);
+        pso_drop_graph(p);
+        pso_enqueue_eval(p, st);
+        FB_HIP(hipMemcpyAsync(p->h_cost, p->st.cost, sizeof(double) * S, hipMemcpyDeviceToHost, st));
+        FB_HIP(hipMemcpyAsync(p->h_info, p->st.info, sizeof(int) * S, hipMemcpyDeviceToHost, st));
+        FB_HIP(hipStreamSynchronize(st));
+        FB_HIP(hipGetLastError());
+    }
+    if (cost) std::memcpy(cost, p->h_cost, sizeof(double) * S);
+    if (info) std::memcpy(info, p->h_info, sizeof(int) * S);
+    return FB_OK;
+}
+
+}  // namespace
+
+int fb_pso_create(fb_pso **out, fb_handle *h, int s, double w, double c1, double c2)
+{
+    if (!out || !h) return fail(FB_E_ARG, "fb_pso_create: null argument");
+    *out = nullptr;
+    if (s < 1 || s > h->max_swarm) return fail(FB_E_CAPACITY, "fb_pso_create: swarm size out of range");
+    const int P = h->prog.n_par;
+    if (P < 1) return fail(FB_E_ARG, "fb_pso_create: the model has no parameters to optimise");
+    fb_pso *p = new fb_pso;
+    p->h = h;
+    p->st.S = s; p->st.P = P; p->st.w = w; p->st.c1 = c1; p->st.c2 = c2;
+    const size_t SP = (size_t)s * P;
+    bool ok = true;
+    auto dev = [&](void **q, size_t bytes) { ok = ok && hipMalloc(q, bytes) == hipSuccess; };
+    auto pin = [&](void **q, size_t bytes) { ok = ok && hipHostMalloc(q, bytes) == hipSuccess; };
+    dev((void **)&p->st.x, sizeof(double) * SP);
+    dev((void **)&p->st.v, sizeof(double) * SP);
+    dev((void **)&p->st.pbest, sizeof(double) * SP);
+    dev((void **)&p->st.pcost, sizeof(double) * s);
+    dev((void **)&p->st.gbest, sizeof(double) * P);
+    dev((void **)&p->st.cost, sizeof(double) * s);
+    dev((void **)&p->st.state, sizeof(double) * 2);
+    dev((void **)&p->st.info, sizeof(int) * s);
+    dev((void **)&p->st.pending, sizeof(int));
+    dev((void **)&p->st.gidx, sizeof(int));
+    dev((void **)&p->d_r, sizeof(double) * 2 * SP);
+    dev((void **)&p->d_idx, sizeof(int) * s);
+    dev((void **)&p->d_xs, sizeof(double) * 2 * SP);
+    pin((void **)&p->h_r, sizeof(double) * 2 * SP);
+    pin((void **)&p->h_cost, sizeof(double) * s);
+    pin((void **)&p->h_state, sizeof(double) * (2 + P));
+    pin((void **)&p->h_info, sizeof(int) * s);
+    pin((void **)&p->h_gidx, sizeof(int));
+    pin((void **)&p->h_idx, sizeof(int) * s);
+    pin((void **)&p->h_xs, sizeof(double) * 2 * SP);
+    if (!ok) {
+        free_pso(p);
+        delete p;
+        return fail(FB_E_NOMEM, "fb_pso_create: allocation failed");
+    }
+    *out = p;
+    return FB_OK;
+}
+
+int fb_pso_destroy(fb_pso *p)
+{
+    if (!p) return FB_OK;
+    (void)hipStreamSynchronize(p->h->stream);
+    free_pso(p);
+    delete p;
+    return FB_OK;
+}
+
+int fb_pso_init(fb_pso *p, const double *x0, const double *v0, double *cost, int *info)
+{
+    if (!p || !x0 || !v0) return fail(FB_E_ARG, "fb_pso_init: null argument");
+    fb_handle *h = p->h;
+    const int S = p->st.S, P = p->st.P;
+    const size_t SP = (size_t)S * P;
+    hipStream_t st = h->stream;
+    std::memcpy(p->h_xs, x0, sizeof(double) * SP);
+    std::memcpy(p->h_xs + SP, v0, sizeof(double) * SP);
+    FB_HIP(hipMemcpyAsync(p->st.x, p->h_xs, sizeof(double) * SP, hipMemcpyHostToDevice, st));
+    FB_HIP(hipMemcpyAsync(p->st.v, p->h_xs + SP, sizeof(double) * SP, hipMemcpyHostToDevice, st));
+    int rc = pso_prepare(p);
+    if (rc) return rc;
+    // initial swarm (pso.f90:291-308): evaluated once; pbest = x0, pbest
+    // cost = cost; gbest from +Inf in particle order
+    pso_enqueue_eval(p, st);
+    FB_HIP(hipMemcpyAsync(p->st.pbest, p->st.x, sizeof(double) * SP, hipMemcpyDeviceToDevice, st));
+    FB_HIP(hipMemcpyAsync(p->st.pcost, p->st.cost, sizeof(double) * S, hipMemcpyDeviceToDevice, st));
+    p->h_state[0] = std::numeric_limits<double>::infinity();
+    p->h_state[1] = 0.0;
+    FB_HIP(hipMemcpyAsync(p->st.state, p->h_state, sizeof(double) * 2, hipMemcpyHostToDevice, st));
+    rc = pso_finish(p, cost, info);
+    if (rc) return rc;
+    // a lock-step re-evaluation (if any) left fresh costs: seed pcost again
+    FB_HIP(hipMemcpyAsync(p->st.pcost, p->st.cost, sizeof(double) * S, hipMemcpyDeviceToDevice, st));
+    fb::launch_pso_select(p->st, st);
+    FB_HIP(hipStreamSynchronize(st));
+    FB_HIP(hipGetLastError());
+    return FB_OK;
+}
+
+int fb_pso_step(fb_pso *p, const double *r1, const double *r2, double *cost, int *info)
+{
+    if (!p || !r1 || !r2) return fail(FB_E_ARG, "fb_pso_step: null argument");
+    fb_handle *h = p->h;
+    const int S = p->st.S, P = p->st.P;
+    const size_t SP = (size_t)S * P;
+    hipStream_t st = h->stream;
+    int rc = pso_prepare(p);
+    if (rc) return rc;
+    std::memcpy(p->h_r, r1, sizeof(double) * SP);
+    std::memcpy(p->h_r + SP, r2, sizeof(double) * SP);
+    auto enqueue = [&](hipStream_t s) {
+        (void)hipMemcpyAsync(p->d_r, p->h_r, sizeof(double) * 2 * SP, hipMemcpyHostToDevice, s);
+        fb::launch_pso_select(p->st, s);
+        fb::launch_pso_update(p->st, p->d_r, p->d_r + SP, s);
+        pso_enqueue_eval(p, s);
+    };
+    if (h->profile || !h->use_graph) {
+        enqueue(st);
+    } else {
+        const int mode = h->use_dag ? 0 : (h->gemm_kc == 16 ? 2 : 1);
+        if (p->exec && (p->graph_epoch != h->graph_epoch || p->graph_mode != mode)) pso_drop_graph(p);
+        if (!p->exec) {
+            FB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            enqueue(st);
+            FB_HIP(hipStreamEndCapture(st, &p->graph));
+            FB_HIP(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
+            p->graph_epoch = h->graph_epoch;
+            p->graph_mode = mode;
+        }
+        FB_HIP(hipGraphLaunch(p->exec, st));
+    }
+    rc = pso_finish(p, cost, info);
+    if (rc) return rc;
+    return h->profile ? drain_profile(h) : FB_OK;
+}
+
+int fb_pso_retry(fb_pso *p, int k, const int *idx, const double *x, const double *v, double *cost, int *info)
+{
+    if (!p || (k > 0 && (!idx || !x || !v))) return fail(FB_E_ARG, "fb_pso_retry: null argument");
+    fb_handle *h = p->h;
+    const int S = p->st.S, P = p->st.P;
+    if (k < 0 || k > S) return fail(FB_E_CAPACITY, "fb_pso_retry: particle count out of range");
+    if (k == 0) return FB_OK;
+    for (int r = 0; r < k; ++r)
+        if (idx[r] < 0 || idx[r] >= S) return fail(FB_E_ARG, "fb_pso_retry: particle index out of range");
+    hipStream_t st = h->stream;
+    const size_t kP = (size_t)k * P;
+    std::memcpy(p->h_idx, idx, sizeof(int) * k);
+    std::memcpy(p->h_xs, x, sizeof(double) * kP);
+    std::memcpy(p->h_xs + kP, v, sizeof(double) * kP);
+    FB_HIP(hipMemcpyAsync(p->d_idx, p->h_idx, sizeof(int) * k, hipMemcpyHostToDevice, st));
+    FB_HIP(hipMemcpyAsync(p->d_xs, p->h_xs, sizeof(double) * 2 * kP, hipMemcpyHostToDevice, st));
+    fb::launch_pso_scatter(p->st, k, p->d_idx, p->d_xs, p->d_xs + kP, st);
+    // the listed particles as a swarm of k on the ordinary batched path
+    fb::launch_pso_gather(p->st, k, p->d_idx, h->d_theta, st);
+    int rc = choose_path(h, k);
+    if (rc) return rc;
+    if (h->use_dag) {
+        rc = dag_plan_for(h, k);
+        if (rc) return rc;
+    }
+    int err = 0;
+    rc = run_swarm(h, k, &err);
+    if (rc) return rc;
+    if (err == 1) return fail(FB_E_HIP, "fb_pso_retry: the persistent factorisation found no ready task for 2 s");
+    if (err == 2) {
+        h->forced_mode = 1;
+        set_mode(h, 1);
+        drop_graphs(h);
+        rc = run_swarm(h, k, &err);
+        if (rc) return rc;
+    }
+    fb::launch_pso_put_cost(p->st, k, p->d_idx, h->d_ll, h->d_info, st);
+    FB_HIP(hipGetLastError());
+    for (int r = 0; r < k; ++r) {
+        if (cost) cost[r] = h->h_info[r] == 0 ? -h->h_ll[r] : std::numeric_limits<double>::infinity();
+        if (info) info[r] = h->h_info[r];
+    }
+    return h->profile ? drain_profile(h) : FB_OK;
+}
+
+int fb_pso_select(fb_pso *p, double *gcost, double *gbest, int *gidx)
+{
+    if (!p) return fail(FB_E_ARG, "fb_pso_select: null argument");
+    hipStream_t st = p->h->stream;
+    const int P = p->st.P;
+    fb::launch_pso_select(p->st, st);
+    FB_HIP(hipMemcpyAsync(p->h_state, p->st.state, sizeof(double), hipMemcpyDeviceToHost, st));
+    FB_HIP(hipMemcpyAsync(p->h_state + 2, p->st.gbest, sizeof(double) * P, hipMemcpyDeviceToHost, st));
+    FB_HIP(hipMemcpyAsync(p->h_gidx, p->st.gidx, sizeof(int), hipMemcpyDeviceToHost, st));
+    FB_HIP(hipStreamSynchronize(st));
+    FB_HIP(hipGetLastError());
+    if (gcost) *gcost = p->h_state[0];
+    if (gbest) std::memcpy(gbest, p->h_state + 2, sizeof(double) * P);
+    if (gidx) *gidx = *p->h_gidx;
+    return FB_OK;
+}
+
+int fb_pso_set_gbest(fb_pso *p, const double *gbest, double gcost)
+{
+    if (!p || !gbest) return fail(FB_E_ARG, "fb_pso_set_gbest: null argument");
+    hipStream_t st = p->h->stream;
+    const int P = p->st.P;
+    fb::launch_pso_select(p->st, st);   // pending costs are selected first
+    p->h_state[0] = gcost;
+    std::memcpy(p->h_state + 2, gbest, sizeof(double) * P);
+    FB_HIP(hipMemcpyAsync(p->st.state, p->h_state, sizeof(double), hipMemcpyHostToDevice, st));
+    FB_HIP(hipMemcpyAsync(p->st.gbest, p->h_state + 2, sizeof(double) * P, hipMemcpyHostToDevice, st));
+    FB_HIP(hipStreamSynchronize(st));
+    FB_HIP(hipGetLastError());
+    return FB_OK;
+}
+
+int fb_pso_positions(fb_pso *p, double *x, double *v)
+{
+    if (!p || (!x && !v)) return fail(FB_E_ARG, "fb_pso_positions: null argument");
+    hipStream_t st = p->h->stream;
+    const size_t SP = (size_t)p->st.S * p->st.P;
+    if (x) FB_HIP(hipMemcpyAsync(p->h_xs, p->st.x, sizeof(double) * SP, hipMemcpyDeviceToHost, st));
+    if (v) FB_HIP(hipMemcpyAsync(p->h_xs + SP, p->st.v, sizeof(double) * SP, hipMemcpyDeviceToHost, st));
+    FB_HIP(hipStreamSynchronize(st));
+    if (x) std::memcpy(x, p->h_xs, sizeof(double) * SP);
+    if (v) std::memcpy(v, p->h_xs + SP, sizeof(double) * SP);
     return FB_OK;
 }
 

@@ -141,4 +141,22 @@ DagPlan dag_plan(std::vector<DagMatrix> &mats, int n_xcd);
 void launch_dag(const DagPlan &plan, const DagMatrix *d_mats, const DagWork &w, int grid, hipStream_t st);
 void launch_backsolve(const SwarmBuffers &b, int particle, double *alpha, hipStream_t st);
 
+// Particle swarm state on the device (fb_pso.hip).  Particle k's parameters
+// at x[k * P + j]; state[0] = gbest cost.
+struct PsoState {
+    int S = 0, P = 0;
+    double w = 0.0, c1 = 0.0, c2 = 0.0;
+    double *x = nullptr, *v = nullptr, *pbest = nullptr, *pcost = nullptr, *gbest = nullptr;
+    double *cost = nullptr, *state = nullptr;
+    int *info = nullptr, *pending = nullptr, *gidx = nullptr;
+};
+void launch_pso_cost(const PsoState &p, const double *ll, const int *info_src, hipStream_t st);
+void launch_pso_select(const PsoState &p, hipStream_t st);
+void launch_pso_update(const PsoState &p, const double *r1, const double *r2, hipStream_t st);
+void launch_pso_scatter(const PsoState &p, int k, const int *idx, const double *xs, const double *vs,
+                        hipStream_t st);
+void launch_pso_gather(const PsoState &p, int k, const int *idx, double *out, hipStream_t st);
+void launch_pso_put_cost(const PsoState &p, int k, const int *idx, const double *ll, const int *info_k,
+                         hipStream_t st);
+
 }  // namespace fb

@@ -262,9 +262,10 @@ __device__ inline void dag_release(const DagArgs &a, int q, int rank, int nb, in
     }
 }
 
-constexpr int kDagSmem = TileShape<kMB, kNBO>::SMEM > kPotrfSmem ? TileShape<kMB, kNBO>::SMEM : kPotrfSmem;
+constexpr int kDagKC = 16;
+constexpr int kDagSmem = TileShape<kMB, kNBO, kDagKC>::SMEM > kPotrfSmem ? TileShape<kMB, kNBO, kDagKC>::SMEM : kPotrfSmem;
 
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, 1))) dag_factor_kernel(DagArgs a)
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) dag_factor_kernel(DagArgs a)
 {
     __shared__ __attribute__((aligned(16))) double smem[kDagSmem];
     __shared__ unsigned long long sh_task;
@@ -317,9 +318,9 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1
             const int p = ta, i = ti;
             if (live) {
                 if (i == 2 * p)   // only the lower 64 x 64 block of the diagonal row is used
-                    gemm_tile_update<kMB, kNBI>(A, np, i * kNBI, p * kNBO, 0, p * kNBO, smem);
+                    gemm_tile_update<kMB, kNBI, kDagKC, 1>(A, np, i * kNBI, p * kNBO, 0, p * kNBO, smem);
                 else
-                    gemm_tile_update<kMB, kNBO>(A, np, i * kNBI, p * kNBO, 0, p * kNBO, smem);
+                    gemm_tile_update<kMB, kNBO, kDagKC, 1>(A, np, i * kNBI, p * kNBO, 0, p * kNBO, smem);
             }
             if (i == 2 * p) { rtype = kTaskTP; ra = i; rlo = 0; rhi = 1; }
             else { rtype = kTaskTT; ra = 2 * p; rlo = i; rhi = i + 1; }
@@ -350,7 +351,7 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1
             }
         } else {
             const int p = ta, i = ti, b0 = 2 * p, b1 = 2 * p + 1;
-            if (live) gemm_tile_update<kMB, kNBI>(A, np, i * kNBI, b1 * kNBI, b0 * kNBI, b1 * kNBI, smem);
+            if (live) gemm_tile_update<kMB, kNBI, kDagKC, 1>(A, np, i * kNBI, b1 * kNBI, b0 * kNBI, b1 * kNBI, smem);
             if (i == b1) { rtype = kTaskTP; ra = b1; rlo = 0; rhi = 1; }
             else { rtype = kTaskTT; ra = b1; rlo = i; rhi = i + 1; }
         }

@@ -53,7 +53,7 @@ struct TileShape {
 // streamed in chunks of KC through two LDS buffers; the global loads of a
 // chunk are issued two chunks ahead (two register sets), so each has two
 // chunks of matrix-core work to land.
-template <int MB, int NB, int KC>
+template <int MB, int NB, int KC, int DIST = 2>
 __device__ inline void tile_gemm_sub(double *__restrict__ Cg, int ldc,
                                      const double *__restrict__ Ag, int lda,
                                      const double *__restrict__ Bg, int ldb, int depth,
@@ -100,7 +100,7 @@ __device__ inline void tile_gemm_sub(double *__restrict__ Cg, int ldc,
         // live in scratch): the loads of chunk c + 2 are issued before chunk
         // c is computed, so every load has two chunks of MFMA work to land
         double r0ax[NA], r0ay[NA], r0bx[NBL], r0by[NBL];
-        double r1ax[NA], r1ay[NA], r1bx[NBL], r1by[NBL];
+        double r1ax[DIST > 1 ? NA : 1], r1ay[DIST > 1 ? NA : 1], r1bx[DIST > 1 ? NBL : 1], r1by[DIST > 1 ? NBL : 1];
 #define FB_LOAD(SET, CH)                                                                        \
         do {                                                                                    \
             const size_t ka = (size_t)(CH) * KC * lda, kb = (size_t)(CH) * KC * ldb;            \
@@ -125,7 +125,7 @@ __device__ inline void tile_gemm_sub(double *__restrict__ Cg, int ldc,
         auto compute = [&](int buf) {
             const double *as = smem + buf * T::ASZ;
             const double *bs = smem + 2 * T::ASZ + buf * T::BSZ;
-#pragma unroll
+#pragma unroll DIST > 1 ? KC / 4 : 2
             for (int kk = 0; kk < KC / 4; ++kk) {
                 double af[T::BM], bf[T::BN];
 #pragma unroll
@@ -141,6 +141,16 @@ __device__ inline void tile_gemm_sub(double *__restrict__ Cg, int ldc,
         const int nchunks = depth / KC;
         FB_LOAD(0, 0);
         FB_STORE(0, 0);
+        if constexpr (DIST == 1) {
+            // one register set: the loads of chunk c + 1 land during chunk c
+            __syncthreads();
+            for (int c = 0; c < nchunks; ++c) {
+                if (c + 1 < nchunks) FB_LOAD(0, c + 1);
+                compute(c & 1);
+                if (c + 1 < nchunks) FB_STORE(0, (c + 1) & 1);
+                __syncthreads();
+            }
+        } else {
         if (nchunks > 1) FB_LOAD(1, 1);
         __syncthreads();
         for (int c = 0; c < nchunks; c += 2) {
@@ -156,6 +166,7 @@ __device__ inline void tile_gemm_sub(double *__restrict__ Cg, int ldc,
             if (c + 2 < nchunks) FB_STORE(0, 0);
             __syncthreads();
         }
+        }
 #undef FB_LOAD
 #undef FB_STORE
     }
@@ -170,11 +181,11 @@ __device__ inline void tile_gemm_sub(double *__restrict__ Cg, int ldc,
 
 // C(row0:row0+MB, col0:col0+NB) -= L(row0.., k0:k1) L(col0.., k0:k1)^T
 // for the matrix at A (trailing update of one tile).
-template <int MB, int NB, int KC = kKC>
+template <int MB, int NB, int KC = kKC, int DIST = 2>
 __device__ inline void gemm_tile_update(double *__restrict__ A, int np, int row0, int col0, int k0, int k1,
                                         double *__restrict__ smem)
 {
-    tile_gemm_sub<MB, NB, KC>(A + (size_t)col0 * np + row0, np, A + (size_t)k0 * np + row0, np,
+    tile_gemm_sub<MB, NB, KC, DIST>(A + (size_t)col0 * np + row0, np, A + (size_t)k0 * np + row0, np,
                               A + (size_t)k0 * np + col0, np, k1 - k0, smem);
 }
 

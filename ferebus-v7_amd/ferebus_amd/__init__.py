@@ -10,6 +10,7 @@ from .mean import MeanFit, fit_mean
 from .gpr import MultiSwarm, SwarmLikelihood
 from ._abi import FbError, device_name, library_path, load
 from . import lbfgs
+from . import pso
 
 __all__ = ["KernelConfig", "KernelExpressionError", "KernelProgram", "compile_kernel", "parse",
-           "MeanFit", "fit_mean", "SwarmLikelihood", "MultiSwarm", "FbError", "device_name", "library_path", "load", "lbfgs"]
+           "MeanFit", "fit_mean", "SwarmLikelihood", "MultiSwarm", "FbError", "device_name", "library_path", "load", "lbfgs", "pso"]

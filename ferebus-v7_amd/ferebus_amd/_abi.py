@@ -24,6 +24,8 @@ EXPORTS = (
     "fb_swarm_loglik_async", "fb_particle_loglik", "fb_get_alpha", "fb_swarm_loglik_grad",
     "fb_profile_enable", "fb_profile_read", "fb_factor_path", "fb_device_name", "fb_last_error",
     "fb_multi_create", "fb_multi_swarm_loglik", "fb_multi_destroy",
+    "fb_pso_create", "fb_pso_destroy", "fb_pso_init", "fb_pso_step", "fb_pso_retry", "fb_pso_select",
+    "fb_pso_set_gbest", "fb_pso_positions",
 )
 
 
@@ -76,6 +78,15 @@ def load() -> ctypes.CDLL:
         "fb_profile_enable": (ctypes.c_int, [vp, ctypes.c_int]),
         "fb_factor_path": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
         "fb_profile_read": (ctypes.c_int, [vp, ctypes.c_int, _P_DBL, ctypes.POINTER(ctypes.c_longlong)]),
+        "fb_pso_create": (ctypes.c_int, [ctypes.POINTER(vp), vp, ctypes.c_int, ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_double]),
+        "fb_pso_destroy": (ctypes.c_int, [vp]),
+        "fb_pso_init": (ctypes.c_int, [vp, _P_DBL, _P_DBL, _P_DBL, _P_INT]),
+        "fb_pso_step": (ctypes.c_int, [vp, _P_DBL, _P_DBL, _P_DBL, _P_INT]),
+        "fb_pso_retry": (ctypes.c_int, [vp, ctypes.c_int, _P_INT, _P_DBL, _P_DBL, _P_DBL, _P_INT]),
+        "fb_pso_select": (ctypes.c_int, [vp, _P_DBL, _P_DBL, _P_INT]),
+        "fb_pso_set_gbest": (ctypes.c_int, [vp, _P_DBL, ctypes.c_double]),
+        "fb_pso_positions": (ctypes.c_int, [vp, _P_DBL, _P_DBL]),
         "fb_device_name": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int]),
         "fb_last_error": (ctypes.c_char_p, []),
         "fb_multi_create": (ctypes.c_int, [ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(vp)]),

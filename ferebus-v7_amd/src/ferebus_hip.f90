@@ -27,6 +27,8 @@ module ferebus_hip
   public :: fb_create, fb_destroy, fb_nparams, fb_set_stream, fb_swarm_loglik, &
             fb_swarm_loglik_async, fb_particle_loglik, fb_get_alpha, fb_swarm_loglik_grad, &
             fb_multi_create, fb_multi_swarm_loglik, fb_multi_destroy, &
+            fb_pso_create, fb_pso_destroy, fb_pso_init, fb_pso_step, fb_pso_retry, fb_pso_select, &
+            fb_pso_set_gbest, fb_pso_positions, &
             fb_profile_enable, fb_profile_read, fb_factor_path, fb_device_name, fb_last_error, fb_error_message
 
   interface
@@ -122,6 +124,68 @@ module ferebus_hip
       import :: c_int, c_ptr
       type(c_ptr), value :: m
     end function fb_multi_destroy
+
+    ! device-resident particle swarm: one graph replay per PSO iteration
+    integer(c_int) function fb_pso_create(p, h, s, w, c1, c2) bind(c, name="fb_pso_create")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), intent(out) :: p
+      type(c_ptr), value :: h
+      integer(c_int), value :: s
+      real(c_double), value :: w, c1, c2
+    end function fb_pso_create
+
+    integer(c_int) function fb_pso_destroy(p) bind(c, name="fb_pso_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: p
+    end function fb_pso_destroy
+
+    integer(c_int) function fb_pso_init(p, x0, v0, cost, info) bind(c, name="fb_pso_init")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: p
+      real(c_double), intent(in) :: x0(*), v0(*)
+      real(c_double), intent(out) :: cost(*)
+      integer(c_int), intent(out) :: info(*)
+    end function fb_pso_init
+
+    integer(c_int) function fb_pso_step(p, r1, r2, cost, info) bind(c, name="fb_pso_step")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: p
+      real(c_double), intent(in) :: r1(*), r2(*)
+      real(c_double), intent(out) :: cost(*)
+      integer(c_int), intent(out) :: info(*)
+    end function fb_pso_step
+
+    integer(c_int) function fb_pso_retry(p, k, idx, x, v, cost, info) bind(c, name="fb_pso_retry")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: p
+      integer(c_int), value :: k
+      integer(c_int), intent(in) :: idx(*)
+      real(c_double), intent(in) :: x(*), v(*)
+      real(c_double), intent(out) :: cost(*)
+      integer(c_int), intent(out) :: info(*)
+    end function fb_pso_retry
+
+    integer(c_int) function fb_pso_select(p, gcost, gbest, gidx) bind(c, name="fb_pso_select")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: p
+      real(c_double), intent(out) :: gcost
+      real(c_double), intent(out) :: gbest(*)
+      integer(c_int), intent(out) :: gidx
+    end function fb_pso_select
+
+    integer(c_int) function fb_pso_set_gbest(p, gbest, gcost) bind(c, name="fb_pso_set_gbest")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: p
+      real(c_double), intent(in) :: gbest(*)
+      real(c_double), value :: gcost
+    end function fb_pso_set_gbest
+
+    integer(c_int) function fb_pso_positions(p, x, v) bind(c, name="fb_pso_positions")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: p
+      real(c_double), intent(out) :: x(*)
+      type(c_ptr), value :: v
+    end function fb_pso_positions
 
     integer(c_int) function fb_profile_enable(h, enable) bind(c, name="fb_profile_enable")
       import :: c_int, c_ptr

@@ -13,6 +13,10 @@
 !>    definite is re-randomised (position and velocity, particle.f90:117-128)
 !>    and evaluated again until it succeeds -- here all failed particles of
 !>    an iteration are retried together in one batched call;
+!>  * the swarm lives on the device (fb_pso_*): one iteration's selection of
+!>    the previous bests, velocity / position update, build, factorisation
+!>    and likelihood are one graph replay; r1, r2 and re-randomised
+!>    particles come from this host's RNG;
 !>  * pbest with <= (pso.f90:440-443), gbest in particle order with <=
 !>    (pso.f90:454-459), best state with <= (iter_state.f90:62-82);
 !>  * stopping: relative change of the best cost below `tolerance` for
@@ -23,9 +27,10 @@
 !> the global best is polished by a few L-BFGS steps on the adjoint gradient
 !> (fb_lbfgs) and replaces gbest when the polished cost is lower.
 module fb_pso
-  use iso_c_binding, only: c_int, c_double
+  use iso_c_binding, only: c_int, c_double, c_ptr, c_null_ptr
   use ieee_arithmetic, only: ieee_value, ieee_positive_inf
-  use ferebus_hip, only: FB_OK, fb_error_message
+  use ferebus_hip, only: FB_OK, fb_error_message, fb_pso_create, fb_pso_destroy, fb_pso_init, fb_pso_step, &
+                         fb_pso_retry, fb_pso_select, fb_pso_set_gbest, fb_pso_positions
   use fb_gpr, only: GaussianProcessRegressor
   use fb_lbfgs, only: LbfgsSettings, lbfgs_polish
   implicit none
@@ -89,96 +94,106 @@ contains
     call random_seed(put=s)
   end subroutine seed_rng
 
-  !> Evaluates the listed particles; failed ones are re-randomised and
-  !> re-evaluated together until every particle succeeds (get_cost).
-  subroutine evaluate(gp, cfg, x, v, lo, hi, idx, cost, res, it, retry)
-    type(GaussianProcessRegressor), intent(inout) :: gp
+  !> Writes one trace line per listed particle.
+  subroutine trace_lines(cfg, it, round, idx, info, c, th)
     type(PsoSettings), intent(in) :: cfg
-    real(wp), intent(inout) :: x(:, :), v(:, :)
+    integer, intent(in) :: it, round, idx(:)
+    integer(c_int), intent(in) :: info(:)
+    real(wp), intent(in) :: c(:), th(:, :)
+    integer :: k
+    if (.not. cfg%trace) return
+    do k = 1, size(idx)
+      write(cfg%trace_unit, '(I0,1X,I0,1X,I0,1X,I0,*(1X,ES24.16E3))') it, round, idx(k), info(k), c(k), th(:, k)
+    end do
+  end subroutine trace_lines
+
+  !> Failed particles of this iteration: re-randomised (particle.f90:117-128)
+  !> and re-evaluated together on the device until every one succeeds
+  !> (get_cost, particle.f90:149-162).
+  subroutine retry_failed(ps, cfg, lo, hi, cost, info, res, it)
+    type(c_ptr), intent(in) :: ps
+    type(PsoSettings), intent(in) :: cfg
     real(wp), intent(in) :: lo(:), hi(:)
-    integer, intent(in) :: idx(:)
     real(wp), intent(inout) :: cost(:)
+    integer(c_int), intent(inout) :: info(:)
     type(PsoResult), intent(inout) :: res
     integer, intent(in) :: it
-    logical, intent(in) :: retry
     integer, allocatable :: todo(:)
-    real(wp), allocatable :: th(:, :), c(:), rp(:, :), rv(:, :)
-    logical, allocatable :: ok(:)
-    integer(c_int), allocatable :: info(:)
-    integer :: rc, k, round
-    todo = idx
+    integer(c_int), allocatable :: idx0(:), inf(:)
+    real(wp), allocatable :: rp(:, :), rv(:, :), c(:)
+    integer :: round, k, i, p
+    p = size(lo)
+    todo = pack([(i, i = 1, size(cost))], info /= 0)
     round = 0
     do while (size(todo) > 0)
-      th = x(:, todo)
-      allocate(c(size(todo)), ok(size(todo)), info(size(todo)))
-      call gp%swarm_cost(th, c, ok, info, rc)
-      if (rc /= FB_OK) then
-        res%status = rc
-        res%message = fb_error_message()
-        return
-      end if
-      res%evaluations = res%evaluations + size(todo)
-      do k = 1, size(todo)
-        if (cfg%trace) write(cfg%trace_unit, '(I0,1X,I0,1X,I0,1X,I0,*(1X,ES24.16E3))') &
-            it, round, todo(k), info(k), c(k), th(:, k)
-        if (ok(k)) then
-          cost(todo(k)) = c(k)
-        else
-          cost(todo(k)) = ieee_value(1.0_wp, ieee_positive_inf)
-        end if
-      end do
-      res%rejections = res%rejections + count(.not. ok)
-      if (.not. retry) exit
-      todo = pack(todo, .not. ok)
-      deallocate(c, ok, info)
-      if (size(todo) == 0) exit
+      res%rejections = res%rejections + size(todo)
       round = round + 1
       if (round > cfg%max_retries) then
         res%status = -100
         res%message = "PSO: particles stayed non-positive-definite after max_retries re-randomisations"
         return
       end if
-      ! particle.f90:117-128: new position in [min, max], velocity in +-(max - min)
-      allocate(rp(size(x, 1), size(todo)), rv(size(x, 1), size(todo)))
+      k = size(todo)
+      allocate(rp(p, k), rv(p, k), c(k), inf(k))
       call uniform(rp, lo, hi)
       call uniform(rv, lo - hi, hi - lo)
-      x(:, todo) = rp
-      v(:, todo) = rv
-      deallocate(rp, rv)
+      idx0 = int(todo - 1, c_int)
+      res%status = fb_pso_retry(ps, int(k, c_int), idx0, rp, rv, c, inf)
+      if (res%status /= FB_OK) then
+        res%message = fb_error_message()
+        return
+      end if
+      res%evaluations = res%evaluations + k
+      call trace_lines(cfg, it, round, todo, inf, c, rp)
+      cost(todo) = c
+      info(todo) = inf
+      todo = pack(todo, inf /= 0)
+      deallocate(rp, rv, c, inf)
     end do
-  end subroutine evaluate
+  end subroutine retry_failed
 
+  !> The PSO over the device-resident swarm (fb_pso_*): each iteration's
+  !> selection, update, build, factorisation and likelihood are one graph
+  !> replay; r1, r2 and re-randomised particles come from this host's RNG.
   subroutine run_pso(gp, cfg, res)
     type(GaussianProcessRegressor), intent(inout) :: gp
     type(PsoSettings), intent(in) :: cfg
     type(PsoResult), intent(out) :: res
-    real(wp), allocatable :: x(:, :), v(:, :), pbest(:, :), r1(:, :), r2(:, :)
-    real(wp), allocatable :: cost(:), pcost(:), lo(:), hi(:), gbest(:)
-    real(wp) :: gcost, stall_cost, fp
+    real(wp), allocatable :: x(:, :), v(:, :), r1(:, :), r2(:, :)
+    real(wp), allocatable :: cost(:), lo(:), hi(:), gbest(:)
+    real(wp) :: gcost, stall_cost, fp, gdev
     real(wp), allocatable :: xp(:)
-    integer :: p, s, i, it, stall_count, all_idx(cfg%swarm_size), ne, na, rc
+    integer(c_int), allocatable :: info(:)
+    integer(c_int) :: gidx
+    integer :: p, s, i, it, stall_count, ne, na, rc
     integer(8) :: t0, t1, rate
+    type(c_ptr) :: ps
     p = gp%nparams
     s = cfg%swarm_size
-    allocate(x(p, s), v(p, s), pbest(p, s), r1(p, s), r2(p, s), cost(s), pcost(s), lo(p), hi(p))
+    allocate(x(p, s), v(p, s), r1(p, s), r2(p, s), cost(s), info(s), lo(p), hi(p), gbest(p))
     call gp%search_bounds(lo, hi)
     call seed_rng(cfg%seed)
     call uniform(x, lo, hi)
     call uniform(v, lo - hi, hi - lo)
-    all_idx = [(i, i = 1, s)]
+    res%status = fb_pso_create(ps, gp%h, int(s, c_int), cfg%inertia_weight, cfg%cognitive_learning_rate, &
+                               cfg%social_learning_rate)
+    if (res%status /= FB_OK) then
+      res%message = fb_error_message()
+      return
+    end if
     ! initial swarm: evaluated once, no re-randomisation (particle.f90:84-88)
-    call evaluate(gp, cfg, x, v, lo, hi, all_idx, cost, res, 0, .false.)
-    if (res%status /= 0) return
-    pbest = x
-    pcost = cost
+    res%status = fb_pso_init(ps, x, v, cost, info)
+    if (res%status /= FB_OK) then
+      res%message = fb_error_message()
+      rc = fb_pso_destroy(ps)
+      return
+    end if
+    res%evaluations = s
+    res%rejections = count(info /= 0)
+    call trace_lines(cfg, 0, 0, [(i, i = 1, s)], info, cost, x)
     gcost = ieee_value(1.0_wp, ieee_positive_inf)
-    allocate(gbest(p))
-    gbest = x(:, 1)
     do i = 1, s                                     ! state%update in particle order
-      if (cost(i) <= gcost) then
-        gcost = cost(i)
-        gbest = x(:, i)
-      end if
+      if (cost(i) <= gcost) gcost = cost(i)
     end do
     stall_cost = ieee_value(1.0_wp, ieee_positive_inf)
     stall_count = 0
@@ -188,39 +203,39 @@ contains
       if (cfg%verbose) print '(A,I0,A,ES23.15E3)', " Iter: ", it, " Best Cost: ", gcost
       call random_number(r1)
       call random_number(r2)
-      v = cfg%inertia_weight * v + cfg%cognitive_learning_rate * r1 * (pbest - x) + &
-          cfg%social_learning_rate * r2 * (spread(gbest, 2, s) - x)
-      x = x + v
-      call evaluate(gp, cfg, x, v, lo, hi, all_idx, cost, res, it, .true.)
-      if (res%status /= 0) return
-      do i = 1, s
-        if (cost(i) <= pcost(i)) then
-          pbest(:, i) = x(:, i)
-          pcost(i) = cost(i)
-        end if
-      end do
-      do i = 1, s
+      res%status = fb_pso_step(ps, r1, r2, cost, info)
+      if (res%status /= FB_OK) exit
+      res%evaluations = res%evaluations + s
+      if (cfg%trace) then
+        res%status = fb_pso_positions(ps, x, c_null_ptr)
+        if (res%status /= FB_OK) exit
+        call trace_lines(cfg, it, 0, [(i, i = 1, s)], info, cost, x)
+      end if
+      call retry_failed(ps, cfg, lo, hi, cost, info, res, it)
+      if (res%status /= 0) exit
+      do i = 1, s                                   ! pbest / gbest are selected on the device
         if (cost(i) <= gcost) then
           gcost = cost(i)
-          gbest = x(:, i)
           res%best_iter = it
         end if
       end do
       if (cfg%lbfgs_every > 0) then
         if (mod(it, cfg%lbfgs_every) == 0) then
+          res%status = fb_pso_select(ps, gdev, gbest, gidx)
+          if (res%status /= FB_OK) exit
           xp = gbest
-          fp = gcost
+          fp = gdev
           call lbfgs_polish(gp, cfg%lbfgs, xp, fp, ne, na, rc)
           if (rc /= FB_OK) then
             res%status = rc
-            res%message = fb_error_message()
-            return
+            exit
           end if
           res%polish_calls = res%polish_calls + 1
           res%polish_evals = res%polish_evals + ne
-          if (fp < gcost) then
+          if (fp < gdev) then
+            res%status = fb_pso_set_gbest(ps, xp, fp)
+            if (res%status /= FB_OK) exit
             gcost = fp
-            gbest = xp
             res%best_iter = it
             res%polish_improvements = res%polish_improvements + 1
           end if
@@ -241,9 +256,16 @@ contains
       if (it >= cfg%iterations) exit
       it = it + 1
     end do
+    if (res%status == 0) res%status = fb_pso_select(ps, gdev, gbest, gidx)
+    if (res%status /= 0) then
+      if (.not. allocated(res%message)) res%message = fb_error_message()
+      rc = fb_pso_destroy(ps)
+      return
+    end if
     call system_clock(t1)
+    rc = fb_pso_destroy(ps)
     res%opt_seconds = real(t1 - t0, wp) / real(rate, wp)
     res%best_position = gbest
-    res%best_cost = gcost
+    res%best_cost = gdev
   end subroutine run_pso
 end module fb_pso

@@ -164,6 +164,46 @@ int fb_multi_swarm_loglik(fb_multi *m, const int *s, const double *const *theta,
                           int *const *info);
 int fb_multi_destroy(fb_multi *m);
 
+/* Particle swarm on the device: positions, velocities, personal and global
+ * bests stay in device memory, and one PSO iteration -- selection of the
+ * previous iteration's bests, velocity / position update, covariance build,
+ * factorisation, likelihood -- is one captured graph replay.  Replaces the
+ * swarm loop of ParticleSwarm (src/optimiser/pso/pso.f90:382-472) around
+ * Particle%get_cost (src/optimiser/pso/particle.f90:149-162):
+ *   update  v = w v + c1 r1 (pbest - x) + c2 r2 (gbest - x); x = x + v, no
+ *           bounds (pso.f90:422-434); r1, r2 come from the caller's RNG
+ *           (drawn before the swarm loop, pso.f90:405-409);
+ *   pbest   replaced when cost <= pbest cost (pso.f90:440-443);
+ *   gbest   particles in index order, replaced when cost <= running gbest
+ *           cost (pso.f90:454-459; iter_state.f90:62-82).
+ * cost = -logL, +Inf where INFO != 0.  Layouts as fb_swarm_loglik: particle
+ * k's P parameters at x[k * P .. k * P + P).  Selection is deferred until
+ * the next fb_pso_step (or fb_pso_select), so that the caller can first
+ * re-randomise and re-evaluate failed particles with fb_pso_retry, as
+ * get_cost does.  The handle must outlive the swarm; calls on one handle
+ * are not re-entrant.
+ *   fb_pso_create   s particles (<= the handle's max_swarm), coefficients
+ *   fb_pso_init     initial swarm x0, v0: one evaluation, no retries;
+ *                   pbest = x0, gbest selected (pso.f90:291-327)
+ *   fb_pso_step     one iteration with r1, r2 (P * s each, U[0,1)); costs
+ *                   and INFO of every particle
+ *   fb_pso_retry    new positions / velocities for the k particles idx
+ *                   (0-based), re-evaluated together; their costs / INFO
+ *   fb_pso_select   applies pending selections; gbest cost, position and
+ *                   particle index (gidx may be NULL)
+ *   fb_pso_set_gbest  replaces gbest (after polishing), pending costs
+ *                   selected first
+ *   fb_pso_positions  current positions and / or velocities (either NULL) */
+typedef struct fb_pso fb_pso;
+int fb_pso_create(fb_pso **p, fb_handle *h, int s, double w, double c1, double c2);
+int fb_pso_destroy(fb_pso *p);
+int fb_pso_init(fb_pso *p, const double *x0, const double *v0, double *cost, int *info);
+int fb_pso_step(fb_pso *p, const double *r1, const double *r2, double *cost, int *info);
+int fb_pso_retry(fb_pso *p, int k, const int *idx, const double *x, const double *v, double *cost, int *info);
+int fb_pso_select(fb_pso *p, double *gcost, double *gbest, int *gidx);
+int fb_pso_set_gbest(fb_pso *p, const double *gbest, double gcost);
+int fb_pso_positions(fb_pso *p, double *x, double *v);
+
 /* Kernel timing: when enabled, each launch of the handle is bracketed by HIP
  * events and the per-kernel totals accumulate until read.
  * kind: 0 covariance build, 1 trailing-update GEMM, 2 diagonal potrf,
