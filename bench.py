@@ -50,6 +50,46 @@ def potrf_flops(n: int) -> float:
     return n ** 3 / 3.0 + n ** 2 / 2.0 + n / 6.0
 
 
+def converge_runs(gp, S, cap):
+    """Wall time to the converged logL of the swarm search on this workload.
+
+    Plain PSO (the reference's optimiser, seed 1) runs until the relative
+    change of its best cost stays <= 1e-8 for 50 iterations (pso.f90:619-636
+    with the examples' tolerance / stall settings) or `cap` iterations; its
+    final cost is the target.  The same PSO with 5 L-BFGS steps on gbest
+    every 10 iterations (adjoint gradient) then runs until its best cost
+    reaches that target.  Both run on the device-resident swarm, one graph
+    replay per iteration, inputs resident; times exclude set-up."""
+    from ferebus_amd.lbfgs import LbfgsSettings
+    from ferebus_amd.pso import PsoSettings, run_pso
+
+    base = dict(swarm_size=S, iterations=cap, relative_change=True, tolerance=1e-8, stall_iterations=50, seed=1)
+    plain = run_pso(gp, PsoSettings(**base))
+    target = plain.best_cost
+    out = {
+        "criterion": "relative change of best cost <= 1e-8 for 50 iterations (plain PSO), "
+                     f"cap {cap} iterations; each polished run stops at the plain run's final logL",
+        "plain_pso": {"seconds": round(plain.seconds, 3), "iterations": plain.iterations,
+                      "logL": -plain.best_cost, "evaluations": plain.evaluations,
+                      "ms_per_iteration": round(1e3 * plain.seconds / max(plain.iterations, 1), 3)},
+        "lbfgs_polished_pso": [],
+    }
+    # polishing schedules: every 10 iterations, 20 or 5 L-BFGS steps (trial
+    # steps evaluated 4 at a time)
+    for every, steps in ((10, 20), (10, 5)):
+        pol = run_pso(gp, PsoSettings(**base, lbfgs_every=every, lbfgs=LbfgsSettings(steps=steps)),
+                      target_cost=target)
+        reached = next((t for it, t, c in pol.history if c <= target), None)
+        out["lbfgs_polished_pso"].append({
+            "schedule": f"{steps} L-BFGS steps on gbest every {every} iterations",
+            "seconds_to_plain_logL": None if reached is None else round(reached, 3),
+            "time_ratio_over_plain": None if reached is None else round(reached / plain.seconds, 4),
+            "iterations": pol.iterations, "logL": -pol.best_cost, "polish_calls": pol.polish_calls,
+            "gradient_evals": pol.polish_evals, "polish_improvements": pol.polish_improvements})
+    out["time_ratio_polished_over_plain"] = out["lbfgs_polished_pso"][0]["time_ratio_over_plain"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -61,6 +101,10 @@ def main():
                     help="bound on the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-converge", action="store_true",
+                    help="skip the wall-time-to-converged-logL runs (plain PSO vs L-BFGS-polished PSO)")
+    ap.add_argument("--converge-cap", type=int, default=1000,
+                    help="iteration cap of each PSO run")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run 5 device-to-device copies of this many bytes (a known-size "
                          "copy kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters)")
@@ -188,6 +232,9 @@ def main():
         plain_s = time.perf_counter() - t
         result["plain_path"] = {"evals_per_s": round(S / plain_s, 2),
                                 "batched_over_plain": round(value / (S / plain_s), 2)}
+
+    if not args.no_converge:
+        result["converged"] = converge_runs(gp, S, args.converge_cap)
 
     if not args.no_cpu:
         import oracle

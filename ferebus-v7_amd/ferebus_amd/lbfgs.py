@@ -6,6 +6,11 @@ never uses them); polishing is an optional addition, off by default.  Two-loop
 recursion (Nocedal 1980), H0 = (s'y / y'y) I from the newest pair (1/|g| on
 the first step), backtracking Armijo line search (c1 = 1e-4, halving); a
 trial point whose matrix is not positive definite counts as a failed step.
+The full step t = 1 is tried alone; after a rejection the trial steps
+1/2, 1/4, ... are evaluated `batch` at a time in one swarm gradient call (a
+swarm of a few particles costs about as much as one on the GPU).  The
+accepted step is the first of the sequence that satisfies the Armijo
+condition, exactly as in a one-at-a-time backtracking search.
 """
 from __future__ import annotations
 
@@ -22,11 +27,17 @@ class LbfgsSettings:
     memory: int = 5
     c1: float = 1e-4
     max_backtracks: int = 30
+    batch: int = 4           # trial steps per gradient call
 
 
 def _cost_grad(gp, x: np.ndarray) -> Tuple[float, np.ndarray, bool]:
     ll, g, info = gp.swarm_grad(x[None, :])
     return -float(ll[0]), -g[0], bool(info[0] == 0)
+
+
+def _cost_grad_batch(gp, xs: np.ndarray):
+    ll, g, info = gp.swarm_grad(xs)
+    return -ll, -g, info == 0
 
 
 def polish(gp, x0, cfg: LbfgsSettings = LbfgsSettings()):
@@ -60,15 +71,19 @@ def polish(gp, x0, cfg: LbfgsSettings = LbfgsSettings()):
             gd = g @ d
             if not gd < 0.0:
                 break
-        t, ok = 1.0, False
-        for _ in range(cfg.max_backtracks + 1):
-            xt = x + t * d
-            ft, gt, pd = _cost_grad(gp, xt)
-            evals += 1
-            ok = pd and ft <= f + cfg.c1 * t * gd
-            if ok:
-                break
-            t *= 0.5
+        ok, k0 = False, 0
+        nb = max(1, min(cfg.batch, getattr(gp, "max_swarm", 1)))
+        while not ok and k0 <= cfg.max_backtracks:
+            # the full step alone first (usually accepted), then `nb` at a time
+            ts = 0.5 ** np.arange(k0, min(k0 + (1 if k0 == 0 else nb), cfg.max_backtracks + 1))
+            fs, gs, pds = _cost_grad_batch(gp, x[None, :] + ts[:, None] * d[None, :])
+            evals += len(ts)
+            for j, t in enumerate(ts):
+                if pds[j] and fs[j] <= f + cfg.c1 * t * gd:
+                    ok = True
+                    xt, ft, gt = x + t * d, float(fs[j]), gs[j].copy()
+                    break
+            k0 += len(ts)
         if not ok:
             break
         sy = (xt - x) @ (gt - g)

@@ -175,7 +175,7 @@ def run_pso(gp, cfg: PsoSettings, trace: Optional[Callable] = None,
         gcost = math.inf
         for c in cost:                        # state%update in particle order
             if c <= gcost:
-                gcost = c
+                gcost = float(c)
         res.history.append((0, time.perf_counter() - t0, gcost))
         stall_cost, stall = math.inf, 0
         it = 1
@@ -208,7 +208,7 @@ def run_pso(gp, cfg: PsoSettings, trace: Optional[Callable] = None,
                 todo = todo[inf != 0]
             for c in cost:
                 if c <= gcost:
-                    gcost = c
+                    gcost = float(c)
                     res.best_iter = it
             if cfg.lbfgs_every > 0 and it % cfg.lbfgs_every == 0:
                 gdev, gbest, _ = sw.select()
@@ -217,7 +217,7 @@ def run_pso(gp, cfg: PsoSettings, trace: Optional[Callable] = None,
                 res.polish_evals += ne
                 if fp < gdev:
                     sw.set_gbest(xp, fp)
-                    gcost = fp
+                    gcost = float(fp)
                     res.best_iter = it
                     res.polish_improvements += 1
                 if trace:

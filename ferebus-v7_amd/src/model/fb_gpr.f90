@@ -15,7 +15,7 @@ module fb_gpr
 
   type, public :: GaussianProcessRegressor
     type(c_ptr) :: h = c_null_ptr
-    integer :: n = 0, d = 0, nparams = 0, nparams_kernel = 0
+    integer :: n = 0, d = 0, nparams = 0, nparams_kernel = 0, max_swarm = 0
     integer :: likelihood = FB_LIKELIHOOD_MARGINAL
     logical :: optimise_noise = .false.
     real(wp) :: noise = 1e-10_wp
@@ -51,6 +51,7 @@ contains
     type(fb_kernel_spec) :: spec
     self%n = size(y)
     self%d = size(x, 2)
+    self%max_swarm = max_swarm
     self%x = x
     self%y = y
     self%prog = prog

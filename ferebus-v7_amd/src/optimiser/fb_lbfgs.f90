@@ -9,8 +9,11 @@
 !> backtracking Armijo line search (sufficient decrease c1 = 1e-4, halving).
 !> A trial point whose matrix is not positive definite (INFO > 0) counts as
 !> a failed step and is halved like any other; a pair with s'y <= 0 is not
-!> stored.  Each trial costs one gradient evaluation (cost and gradient come
-!> together).
+!> stored.  The full step t = 1 is tried alone; after a rejection the trial
+!> steps 1/2, 1/4, ... are evaluated `batch` at a time as one swarm gradient
+!> call (a few particles cost about as much as one on the GPU); the
+!> accepted step is the first of the sequence that satisfies the Armijo
+!> condition, as in a one-at-a-time search.
 module fb_lbfgs
   use iso_c_binding, only: c_int, c_double
   use ferebus_hip, only: FB_OK, fb_error_message
@@ -24,6 +27,7 @@ module fb_lbfgs
     integer :: memory = 5           ! stored (s, y) pairs
     real(wp) :: c1 = 1e-4_wp        ! Armijo constant
     integer :: max_backtracks = 30
+    integer :: batch = 4            ! trial steps per gradient call
   end type LbfgsSettings
 
   public :: lbfgs_polish
@@ -46,6 +50,18 @@ contains
     g = gg(:, 1)
   end subroutine cost_grad
 
+  !> Costs and gradients at the columns of xs; ok = positive definite.
+  subroutine cost_grad_batch(gp, xs, f, g, ok, rc)
+    type(GaussianProcessRegressor), intent(inout) :: gp
+    real(wp), intent(in) :: xs(:, :)
+    real(wp), intent(out) :: f(:), g(:, :)
+    logical, intent(out) :: ok(:)
+    integer, intent(out) :: rc
+    integer(c_int) :: info(size(xs, 2))
+    call gp%swarm_grad(xs, f, g, info, rc)
+    ok = rc == FB_OK .and. info == 0
+  end subroutine cost_grad_batch
+
   !> Polishes x (cost f on entry) in place.  evals: gradient evaluations
   !> used; accepted: steps taken.  rc /= 0 only on a library error.
   subroutine lbfgs_polish(gp, cfg, x, f, evals, accepted, rc)
@@ -54,8 +70,10 @@ contains
     real(wp), intent(inout) :: x(:), f
     integer, intent(out) :: evals, accepted, rc
     real(wp), allocatable :: s(:, :), y(:, :), rho(:), g(:), d(:), q(:), xt(:), gt(:), a(:)
+    real(wp), allocatable :: xs(:, :), fs(:), gs(:, :), ts(:)
+    logical, allocatable :: oks(:)
     real(wp) :: ft, t, gd, gamma, b, sy
-    integer :: n, m, k, j, i, bt, newest
+    integer :: n, m, k, j, i, k0, nb, nt, newest
     logical :: ok
     n = size(x)
     m = max(cfg%memory, 1)
@@ -94,16 +112,35 @@ contains
         gd = dot_product(g, d)
         if (.not. (gd < 0.0_wp)) exit
       end if
-      t = 1.0_wp
+      ! trial steps 0.5**k, k = k0 .. k0 + nt - 1, in one gradient call
       ok = .false.
-      do bt = 0, cfg%max_backtracks
-        xt = x + t * d
-        call cost_grad(gp, xt, ft, gt, ok, rc)
-        evals = evals + 1
+      nb = max(1, min(cfg%batch, gp%max_swarm))
+      k0 = 0
+      do while (.not. ok .and. k0 <= cfg%max_backtracks)
+        nt = 1                                   ! the full step alone, then nb at a time
+        if (k0 > 0) nt = min(nb, cfg%max_backtracks + 1 - k0)
+        allocate(xs(n, nt), fs(nt), gs(n, nt), oks(nt), ts(nt))
+        do j = 1, nt
+          ts(j) = 0.5_wp ** (k0 + j - 1)
+          xs(:, j) = x + ts(j) * d
+        end do
+        call cost_grad_batch(gp, xs, fs, gs, oks, rc)
+        evals = evals + nt
         if (rc /= FB_OK) return
-        if (ok) ok = ft <= f + cfg%c1 * t * gd
-        if (ok) exit
-        t = 0.5_wp * t
+        do j = 1, nt
+          if (oks(j)) then
+            if (fs(j) <= f + cfg%c1 * ts(j) * gd) then
+              ok = .true.
+              t = ts(j)
+              xt = xs(:, j)
+              ft = fs(j)
+              gt = gs(:, j)
+              exit
+            end if
+          end if
+        end do
+        deallocate(xs, fs, gs, oks, ts)
+        k0 = k0 + nt
       end do
       if (.not. ok) exit                         ! no acceptable step: stop polishing
       sy = dot_product(xt - x, gt - g)

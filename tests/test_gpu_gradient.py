@@ -185,3 +185,17 @@ def test_lbfgs_polish_improves(gpu):
     assert np.array_equal(xp, xp2) and fp == fp2
     ll, info = gp.swarm_loglik(xp)
     assert info[0] == 0 and -ll[0] == fp
+
+
+def test_lbfgs_batched_trials_match_sequential(gpu):
+    """Trial steps evaluated four at a time accept the same step as the
+    one-at-a-time backtracking search."""
+    from ferebus_amd import lbfgs
+    x, y, prog, m, gp, th = _setup(180, 6, seed=71, S=4)
+    a = lbfgs.polish(gp, th[0], lbfgs.LbfgsSettings(steps=6, batch=1))
+    b = lbfgs.polish(gp, th[0], lbfgs.LbfgsSettings(steps=6, batch=4))
+    print(f"sequential: cost {a[1]:.12e}, {a[3]} steps, {a[2]} evaluations; "
+          f"batched: cost {b[1]:.12e}, {b[3]} steps, {b[2]} evaluations")
+    assert a[3] == b[3]
+    np.testing.assert_allclose(b[0], a[0], rtol=1e-12, atol=0.0)
+    assert abs(b[1] - a[1]) <= 1e-12 * abs(a[1])
