@@ -252,14 +252,14 @@ void timed(fb_handle *h, int kind, hipStream_t st, F fn)
 void enqueue_group(fb_handle *h, const fb::SwarmBuffers &b, hipStream_t st)
 {
     timed(h, 0, st, [&] { fb::launch_build(b, st); });
+    // per 128-column panel: update + first block's factorisation (kind 1),
+    // panel solve + second block's factorisation (kind 2), second block's
+    // update + panel solve (kind 3)
     for (int c0 = 0; c0 < h->np; c0 += fb::kNBO) {
         const int c1 = c0 + fb::kNBI;
-        if (c0 > 0) timed(h, 1, st, [&] { fb::launch_gemm_update(b, c0, fb::kNBO, c0, 0, c0, st); });
-        timed(h, 2, st, [&] { fb::launch_potrf64(b, c0, st); });
-        timed(h, 3, st, [&] { fb::launch_trsm64(b, c0, c1, st); });
-        timed(h, 1, st, [&] { fb::launch_gemm_update(b, c1, fb::kNBI, c1, c0, c1, st); });
-        timed(h, 2, st, [&] { fb::launch_potrf64(b, c1, st); });
-        if (c0 + fb::kNBO < h->np) timed(h, 3, st, [&] { fb::launch_trsm64(b, c1, c0 + fb::kNBO, st); });
+        timed(h, 1, st, [&] { fb::launch_panel_update_factor(b, c0, st); });
+        timed(h, 2, st, [&] { fb::launch_trsm_factor(b, c0, st); });
+        if (c1 + fb::kNBI < h->np) timed(h, 3, st, [&] { fb::launch_update_trsm(b, c1, st); });
     }
     timed(h, 4, st, [&] { fb::launch_likelihood(b, st); });
 }

@@ -87,10 +87,9 @@ struct SwarmBuffers {
 };
 
 void launch_build(const SwarmBuffers &b, hipStream_t st);
-void launch_gemm_update(const SwarmBuffers &b, int col0, int width, int row_start, int k0, int k1,
-                        hipStream_t st);
-void launch_potrf64(const SwarmBuffers &b, int c0, hipStream_t st);
-void launch_trsm64(const SwarmBuffers &b, int c0, int row_start, hipStream_t st);
+void launch_panel_update_factor(const SwarmBuffers &b, int c0, hipStream_t st);
+void launch_trsm_factor(const SwarmBuffers &b, int c0, hipStream_t st);
+void launch_update_trsm(const SwarmBuffers &b, int c1, hipStream_t st);
 void launch_likelihood(const SwarmBuffers &b, hipStream_t st);
 
 // Adjoint gradient (fb_grad.hip).

@@ -102,7 +102,7 @@ __global__ void __launch_bounds__(kThreads) uinv_step_kernel(const double *__res
     double *tile = Us + (size_t)64 * i * np + 64 * j;   // rows 64j.., columns 64i..
     const double *Lii = L + (size_t)64 * i * np + 64 * i;
     if (j == i) {
-        trsm64_solve<true>(tile, np, Lii, np, di);
+        trsm64_solve<true>(tile, np, tile, np, Lii, np, di);
         return;
     }
     // tile = -sum_{k in [64j, 64i)} U(64j + r, k) L(64i + c, k)  (the tile starts at 0)
@@ -113,7 +113,7 @@ __global__ void __launch_bounds__(kThreads) uinv_step_kernel(const double *__res
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     asm volatile("buffer_inv sc1" ::: "memory");
-    trsm64_solve<false>(tile, np, Lii, np, di);
+    trsm64_solve<false>(tile, np, tile, np, Lii, np, di);
 }
 
 // Columns [n, np) of U are not part of L^{-T}: zero them before U U^T.

@@ -6,100 +6,160 @@
 //
 // Left-looking blocked algorithm with two block levels.  Panels of width
 // kNBO = 128 carry the dominant trailing update; inside a panel two blocks of
-// kNBI = 64 are factored.  For the panel at column c0:
-//   gemm_update  C(c0:, c0:c0+128) -= L(c0:, 0:c0) L(c0:c0+128, 0:c0)^T
-//                (FP64 MFMA 16x16x4, LDS-staged, double-buffered; the wide
-//                panel halves how often each row panel is re-read from HBM)
-//   potrf64      factor the 64x64 block at c0 in LDS (INFO, partial log-det)
-//   trsm64       L(c0+64:, c0:c0+64) = C L(c0, c0)^{-T} by forward
-//                substitution (backward stable, no explicit inverse)
-//   gemm_update  C(c0+64:, c0+64:c0+128) -= L(c0+64:, c0:c0+64) L(...)^T
-//   potrf64, trsm64 for the block at c0 + 64.
-// The whole swarm advances in lock step: every launch covers all particles.
+// kNBI = 64 (b0 = c0, b1 = c0 + 64) are factored.  Three launches per panel:
+//   1. C(c0:, c0:c0+128) -= L(c0:, 0:c0) L(c0:c0+128, 0:c0)^T (FP64 MFMA
+//      16x16x4, LDS-staged); the first work-groups update the diagonal block
+//      b0 on chip and factor it (INFO, partial log-det, 16x16 inverses)
+//      while the other tiles are still being updated;
+//   2. L(c0+64:, b0) = C L(b0, b0)^{-T} (block forward substitution on the
+//      matrix cores); the first work-groups also update block (b1, b1) with
+//      their solved tile on chip and factor it;
+//   3. C(c0+128:, b1) -= L(c0+128:, b0) L(b1, b0)^T on chip, then solved
+//      with L(b1, b1).
+// Every step runs the tile routines of fb_tiles.h in the order the
+// persistent schedule does, so both paths give the same bits.  The whole
+// swarm advances in lock step: every launch covers all particles.
 #include "fb_tiles.h"
 
 namespace fb {
 
-// ---- lock-step kernels (one launch per step, every particle) ---------------
+// ---- lock-step kernels (three launches per 128-column panel) ---------------
 
-template <int MB, int NB, int KC>
-__global__ void __launch_bounds__(kThreads) gemm_update_kernel(double *__restrict__ K, int np, int col0,
-                                                               int row_start, int k0, int k1,
-                                                               const int *__restrict__ info)
+template <int KC>
+constexpr int panel_smem()
 {
-    __shared__ __attribute__((aligned(16))) double smem[TileShape<MB, NB, KC>::SMEM];
-    const int rt_per = (np - row_start) / MB;
-    const int s = blockIdx.x / rt_per;
-    const int rt = blockIdx.x % rt_per;
-    if (info[s] != 0) return;
-    gemm_tile_update<MB, NB, KC>(K + (size_t)s * np * np, np, row_start + rt * MB, col0, k0, k1, smem);
+    constexpr int a = TileShape<kMB, kNBO, KC>::SMEM, b = TileShape<kMB, kNBI, KC>::SMEM;
+    constexpr int m = a > b ? a : b;
+    return m > kPotrfSmem ? m : kPotrfSmem;
 }
 
-__global__ void __launch_bounds__(kThreads) trsm64_kernel(double *__restrict__ K, int np, int c0,
-                                                          int row_start, const int *__restrict__ info,
-                                                          const double *__restrict__ dinv)
+// Launch 1 of panel c0: the trailing update C(c0:, c0:c0+128) -= L(c0:, 0:c0)
+// L(c0:c0+128, 0:c0)^T.  Work-groups 0 .. S-1 take the diagonal block of
+// each matrix (its lower 64x64 only), keep the updated block on chip and
+// factor it (potrf64_block); they come first, so the factorisation runs
+// while the other row tiles are still being updated.
+template <int KC>
+__global__ void __launch_bounds__(kThreads)
+__attribute__((amdgpu_waves_per_eu(KC == 16 ? 2 : 1, KC == 16 ? 2 : 1))) panel_update_factor_kernel(double *__restrict__ K, int np, int n,
+                                                                       int c0, int S, int *__restrict__ info,
+                                                                       double *__restrict__ logdet_part,
+                                                                       double *__restrict__ dinv)
 {
-    const int rt_per = (np - row_start) / kNBI;
-    const int s = blockIdx.x / rt_per;
-    const int rt = blockIdx.x % rt_per;
+    __shared__ __attribute__((aligned(16))) double smem[panel_smem<KC>()];
+    const int b = blockIdx.x;
+    if (b < S) {
+        if (info[b] != 0) return;
+        double *A = K + (size_t)b * np * np;
+        gemm_tile_update<kMB, kNBI, KC, 2, true>(A, np, c0, c0, 0, c0, smem);
+        __syncthreads();
+        double lg = 0.0;
+        const int fail = potrf64_block<true>(A, np, n, c0, &lg, smem,
+                                             dinv + ((size_t)b * (np / kNBI) + c0 / kNBI) * kDinv);
+        if (threadIdx.x == 0) {
+            if (fail) info[b] = fail;
+            else logdet_part[(size_t)b * (np / kNBI) + c0 / kNBI] = lg;
+        }
+        return;
+    }
+    const int rt = (np - c0) / kMB - 1;     // row tiles below the diagonal block
+    const int t = b - S;
+    const int s = t / rt, r = 1 + t % rt;
     if (info[s] != 0) return;
-    trsm64_mfma(K + (size_t)s * np * np, np, c0, row_start + rt * kNBI,
-                dinv + ((size_t)s * (np / kNBI) + c0 / kNBI) * kDinv);
+    gemm_tile_update<kMB, kNBO, KC>(K + (size_t)s * np * np, np, c0 + r * kMB, c0, 0, c0, smem);
 }
 
-__global__ void __launch_bounds__(kThreads) potrf64_kernel(double *__restrict__ K, int np, int n, int c0,
-                                                           int *__restrict__ info,
-                                                           double *__restrict__ logdet_part,
-                                                           double *__restrict__ dinv)
+// Launch 2 of panel c0: every row tile below block b0 = c0 is solved with
+// L(b0, b0) (trsm64).  Work-groups 0 .. S-1 take the tile of block
+// b1 = c0 + 64, keep the solved tile on chip, apply it to the diagonal block
+// (b1, b1) (syrk64_lds) and factor that block.
+__global__ void __launch_bounds__(kThreads) trsm_factor_kernel(double *__restrict__ K, int np, int n, int c0,
+                                                               int S, int *__restrict__ info,
+                                                               double *__restrict__ logdet_part,
+                                                               double *__restrict__ dinv)
 {
-    __shared__ __attribute__((aligned(16))) double smem[kPotrfSmem];
-    const int s = blockIdx.x;
+    __shared__ __attribute__((aligned(16))) double smem[kNBI * kPLD + kPotrfSmem];
+    const int b = blockIdx.x;
+    const int rt = (np - c0) / kNBI - 1;    // row tiles below block b0
+    const int s = b < S ? b : (b - S) / (rt - 1);
+    const int r = b < S ? 1 : 2 + (b - S) % (rt - 1);
     if (info[s] != 0) return;
+    double *A = K + (size_t)s * np * np;
+    const double *d0 = dinv + ((size_t)s * (np / kNBI) + c0 / kNBI) * kDinv;
+    const int row0 = c0 + r * kNBI;
+    double *tile = A + (size_t)c0 * np + row0;
+    if (b >= S) {
+        trsm64_solve<false>(tile, np, tile, np, A + (size_t)c0 * np + c0, np, d0);
+        return;
+    }
+    double *Xs = smem;                       // L(b1, b0) on chip
+    double *Ls = smem + kNBI * kPLD;         // block (b1, b1), then its factorisation
+    trsm64_solve<false, true>(tile, np, tile, np, A + (size_t)c0 * np + c0, np, d0, Xs);
+    const int c1 = c0 + kNBI;
+    const double *Cg = A + (size_t)c1 * np + c1;
+    for (int e = threadIdx.x; e < kNBI * kNBI / 2; e += kThreads) {
+        const int col = e / (kNBI / 2), r2 = (e % (kNBI / 2)) * 2;
+        *reinterpret_cast<double2 *>(Ls + col * kPLD + r2) = *reinterpret_cast<const double2 *>(Cg + (size_t)col * np + r2);
+    }
+    __syncthreads();
+    syrk64_lds(Ls, Xs);
+    __syncthreads();
     double lg = 0.0;
-    const int fail = potrf64_block(K + (size_t)s * np * np, np, n, c0, &lg, smem,
-                                   dinv + ((size_t)s * (np / kNBI) + c0 / kNBI) * kDinv);
+    const int fail = potrf64_block<true>(A, np, n, c1, &lg, Ls,
+                                         dinv + ((size_t)s * (np / kNBI) + c1 / kNBI) * kDinv);
     if (threadIdx.x == 0) {
         if (fail) info[s] = fail;
-        else logdet_part[(size_t)s * (np / kNBI) + c0 / kNBI] = lg;
+        else logdet_part[(size_t)s * (np / kNBI) + c1 / kNBI] = lg;
     }
 }
 
-void launch_gemm_update(const SwarmBuffers &b, int col0, int width, int row_start, int k0, int k1,
-                        hipStream_t st)
+// Launch 3 of panel c0 (block b1 = c1 = c0 + 64): every row tile below b1
+// takes the update C(i, b1) -= L(i, b0) L(b1, b0)^T (64 deep, kept on chip)
+// and is solved with L(b1, b1).
+template <int KC>
+__global__ void __launch_bounds__(kThreads) update_trsm_kernel(double *__restrict__ K, int np, int c1,
+                                                               const int *__restrict__ info,
+                                                               const double *__restrict__ dinv)
 {
-    if (k1 <= k0) return;
-    const int rt = (b.np - row_start) / kMB;
-    // chunk depth 32 (one work-group per CU, 115 KB of LDS) or 16 (two per
-    // CU): the same accumulation order, so the same bits, at different speed
-    if (width == kNBO) {
-        if (b.gemm_kc == 16)
-            hipLaunchKernelGGL((gemm_update_kernel<kMB, kNBO, 16>), dim3(b.S * rt), dim3(kThreads), 0, st,
-                               b.K, b.np, col0, row_start, k0, k1, b.info);
-        else
-            hipLaunchKernelGGL((gemm_update_kernel<kMB, kNBO, 32>), dim3(b.S * rt), dim3(kThreads), 0, st,
-                               b.K, b.np, col0, row_start, k0, k1, b.info);
-    } else {
-        if (b.gemm_kc == 16)
-            hipLaunchKernelGGL((gemm_update_kernel<kMB, kNBI, 16>), dim3(b.S * rt), dim3(kThreads), 0, st,
-                               b.K, b.np, col0, row_start, k0, k1, b.info);
-        else
-            hipLaunchKernelGGL((gemm_update_kernel<kMB, kNBI, 32>), dim3(b.S * rt), dim3(kThreads), 0, st,
-                               b.K, b.np, col0, row_start, k0, k1, b.info);
-    }
+    __shared__ __attribute__((aligned(16))) double smem[TileShape<kMB, kNBI, KC>::SMEM];
+    const int rt = (np - c1) / kNBI - 1;
+    const int s = blockIdx.x / rt;
+    const int row0 = c1 + (1 + blockIdx.x % rt) * kNBI;
+    if (info[s] != 0) return;
+    double *A = K + (size_t)s * np * np;
+    gemm_tile_update<kMB, kNBI, KC, 2, true>(A, np, row0, c1, c1 - kNBI, c1, smem);
+    __syncthreads();
+    trsm64_solve<false>(smem, kPLD, A + (size_t)c1 * np + row0, np, A + (size_t)c1 * np + c1, np,
+                        dinv + ((size_t)s * (np / kNBI) + c1 / kNBI) * kDinv);
 }
 
-void launch_potrf64(const SwarmBuffers &b, int c0, hipStream_t st)
+void launch_panel_update_factor(const SwarmBuffers &b, int c0, hipStream_t st)
 {
-    hipLaunchKernelGGL(potrf64_kernel, dim3(b.S), dim3(kThreads), 0, st, b.K, b.np, b.n, c0, b.info,
-                       b.logdet_part, b.dinv);
+    const int grid = b.S * (c0 > 0 ? (b.np - c0) / kMB : 1);   // the first panel has nothing to update
+    if (b.gemm_kc == 16)
+        hipLaunchKernelGGL((panel_update_factor_kernel<16>), dim3(grid), dim3(kThreads), 0, st, b.K, b.np, b.n,
+                           c0, b.S, b.info, b.logdet_part, b.dinv);
+    else
+        hipLaunchKernelGGL((panel_update_factor_kernel<32>), dim3(grid), dim3(kThreads), 0, st, b.K, b.np, b.n,
+                           c0, b.S, b.info, b.logdet_part, b.dinv);
 }
 
-void launch_trsm64(const SwarmBuffers &b, int c0, int row_start, hipStream_t st)
+void launch_trsm_factor(const SwarmBuffers &b, int c0, hipStream_t st)
 {
-    const int rt = (b.np - row_start) / kNBI;
+    const int rt = (b.np - c0) / kNBI - 1;   // >= 1: the panel holds two blocks
+    hipLaunchKernelGGL(trsm_factor_kernel, dim3(b.S * rt), dim3(kThreads), 0, st, b.K, b.np, b.n, c0, b.S,
+                       b.info, b.logdet_part, b.dinv);
+}
+
+void launch_update_trsm(const SwarmBuffers &b, int c1, hipStream_t st)
+{
+    const int rt = (b.np - c1) / kNBI - 1;
     if (rt <= 0) return;
-    hipLaunchKernelGGL(trsm64_kernel, dim3(b.S * rt), dim3(kThreads), 0, st, b.K, b.np, c0, row_start, b.info,
-                       b.dinv);
+    if (b.gemm_kc == 16)
+        hipLaunchKernelGGL((update_trsm_kernel<16>), dim3(b.S * rt), dim3(kThreads), 0, st, b.K, b.np, c1, b.info,
+                           b.dinv);
+    else
+        hipLaunchKernelGGL((update_trsm_kernel<32>), dim3(b.S * rt), dim3(kThreads), 0, st, b.K, b.np, c1, b.info,
+                           b.dinv);
 }
 
 }  // namespace fb

@@ -20,6 +20,8 @@ __device__ inline d4 mfma_f64(double a, double b, d4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+constexpr int kPLD = 68;                        // LDS stride (doubles) of a 64x64 block kept on chip
+
 // Work-group tile of MB rows x NB columns computed by 4 waves, K streamed in
 // chunks of KC through two LDS stages.
 template <int MB, int NB, int KC = kKC>
@@ -53,7 +55,7 @@ struct TileShape {
 // streamed in chunks of KC through two LDS buffers; the global loads of a
 // chunk are issued two chunks ahead (two register sets), so each has two
 // chunks of matrix-core work to land.
-template <int MB, int NB, int KC, int DIST = 2>
+template <int MB, int NB, int KC, int DIST = 2, bool TO_LDS = false>
 __device__ inline void tile_gemm_sub(double *__restrict__ Cg, int ldc,
                                      const double *__restrict__ Ag, int lda,
                                      const double *__restrict__ Bg, int ldb, int depth,
@@ -170,27 +172,40 @@ __device__ inline void tile_gemm_sub(double *__restrict__ Cg, int ldc,
 #undef FB_LOAD
 #undef FB_STORE
     }
+    if constexpr (TO_LDS) {
+        // the tile stays on chip for the caller: smem[c * kPLD + r], the
+        // layout of the diagonal-block routines (the K loop ended with a
+        // barrier, so the staging buffers are free)
+        static_assert(MB <= 64 && NB <= 64, "LDS tile is at most 64 x 64");
 #pragma unroll
-    for (int i = 0; i < T::BM; ++i)
+        for (int i = 0; i < T::BM; ++i)
 #pragma unroll
-        for (int j = 0; j < T::BN; ++j)
+            for (int j = 0; j < T::BN; ++j)
 #pragma unroll
-            for (int v = 0; v < 4; ++v)
-                Cg[(size_t)(wcol + 16 * j + 4 * v + lq) * ldc + wrow + 16 * i + lr] = acc[i][j][v];
+                for (int v = 0; v < 4; ++v)
+                    smem[(wcol + 16 * j + 4 * v + lq) * kPLD + wrow + 16 * i + lr] = acc[i][j][v];
+    } else {
+#pragma unroll
+        for (int i = 0; i < T::BM; ++i)
+#pragma unroll
+            for (int j = 0; j < T::BN; ++j)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    Cg[(size_t)(wcol + 16 * j + 4 * v + lq) * ldc + wrow + 16 * i + lr] = acc[i][j][v];
+    }
 }
 
 // C(row0:row0+MB, col0:col0+NB) -= L(row0.., k0:k1) L(col0.., k0:k1)^T
 // for the matrix at A (trailing update of one tile).
-template <int MB, int NB, int KC = kKC, int DIST = 2>
+template <int MB, int NB, int KC = kKC, int DIST = 2, bool TO_LDS = false>
 __device__ inline void gemm_tile_update(double *__restrict__ A, int np, int row0, int col0, int k0, int k1,
                                         double *__restrict__ smem)
 {
-    tile_gemm_sub<MB, NB, KC, DIST>(A + (size_t)col0 * np + row0, np, A + (size_t)k0 * np + row0, np,
+    tile_gemm_sub<MB, NB, KC, DIST, TO_LDS>(A + (size_t)col0 * np + row0, np, A + (size_t)k0 * np + row0, np,
                               A + (size_t)k0 * np + col0, np, k1 - k0, smem);
 }
 
-constexpr int kPLD = 68;                        // LDS stride of the staged 64x64 block (doubles)
-constexpr int kPotrfSmem = kNBI * kPLD + kNBI + 8;   // doubles: block, 1 / L_ii, flags
+constexpr int kPotrfSmem = kNBI * kPLD + kNBI + 8;   // doubles: block, 1 / L_ii, flag, log sum
 
 // sqrt(x) and 1 / sqrt(x) for x > 0: the hardware reciprocal square root
 // estimate refined by two steps of Goldschmidt's coupled iteration
@@ -222,21 +237,29 @@ __device__ inline double readlane_f64(double x, int l)
 
 // Factors the 64x64 diagonal block at (c0, c0) of the matrix at A,
 // right-looking in four 16-column panels, the block staged in LDS:
-//  * one wave factors panel j (rows 16j..63, lane = row): per column the
-//    pivot and the column entries come from other lanes by v_readlane, so
-//    the 16 dependent steps need no barrier (LAPACK dpotf2 test: a pivot
-//    that is not positive, or NaN, fails; L = a * (1 / sqrt(pivot)), the
-//    square root and its reciprocal from sqrt_rsqrt; the logarithms of the
-//    diagonal are taken afterwards, one per lane);
+//  * wave 0 factors panel j (lane = row): per column the pivot comes from
+//    its lane by v_readlane and the column's multipliers from LDS as
+//    broadcast reads of the column just written (one wave: its LDS
+//    operations complete in order), so the 16 dependent steps need no
+//    barrier (LAPACK dpotf2 test: a pivot that is not positive, or NaN,
+//    fails; L = a * (1 / sqrt(pivot)), the square root and its reciprocal
+//    from sqrt_rsqrt).  Every lane updates every column
+//    of the panel: lanes above the diagonal hold upper-triangle entries,
+//    which no lower entry ever reads, so no per-update lane mask is needed;
 //  * the four waves then apply the panel to the trailing 16x16 blocks on
-//    the matrix cores (accumulators hold C^T, as in tile_gemm_sub).
+//    the matrix cores (accumulators hold C^T, as in tile_gemm_sub);
+//  * while wave 0 factors panel j, the finished panel j - 1 is completed by
+//    the other waves: logarithms of its diagonal (wave 1), the inverse of
+//    its 16x16 diagonal block (wave 2), its store to global memory (wave 3).
 // Columns at or beyond n (bordered right-hand-side rows and padding) are not
 // factored: they become the identity, leaving the right-hand-side rows
-// holding L^{-1} rhs.  Returns the 1-based global failing column (0 =
-// success); on success writes the block's lower triangle back, sum(log L_ii)
-// of its real columns to *logdet (thread 0 of wave 0) and the inverses of its four
-// 16x16 diagonal blocks to dinv (block q column major at dinv[256 q]), for
-// trsm64_mfma.
+// holding L^{-1} rhs (such a block is completed after the loop).  Returns
+// the 1-based global failing column (0 = success); on success writes the
+// block's lower triangle back, sum(log L_ii) of its real columns to *logdet
+// (thread 0) and the inverses of its four 16x16 diagonal blocks to dinv
+// (block q column major at dinv[256 q]), for trsm64_mfma.  STAGED: the
+// block is already in smem (Ls layout), left there by the caller's update.
+template <bool STAGED = false>
 __device__ inline int potrf64_block(double *__restrict__ A, int np, int n, int c0, double *logdet,
                                     double *__restrict__ smem, double *__restrict__ dinv)
 {
@@ -244,18 +267,52 @@ __device__ inline int potrf64_block(double *__restrict__ A, int np, int n, int c
     double *Ls = smem;                               // A(r, c) at Ls[c * kPLD + r]
     double *rdiag = smem + B * kPLD;                 // 1 / L_ii
     int *flag = reinterpret_cast<int *>(smem + B * kPLD + B);
+    double *lgsum = smem + B * kPLD + B + 2;         // reduced log-determinant
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int lr = lane & 15, lq = lane >> 4;
     double *Ag = A + (size_t)c0 * np + c0;
+    if constexpr (!STAGED) {
 #pragma unroll
-    for (int u = 0; u < B * B / 2 / kThreads; ++u) {
-        const int e = tid + u * kThreads;
-        const int col = e / (B / 2), r2 = (e % (B / 2)) * 2;
-        *reinterpret_cast<double2 *>(Ls + col * kPLD + r2) = *reinterpret_cast<const double2 *>(Ag + (size_t)col * np + r2);
+        for (int u = 0; u < B * B / 2 / kThreads; ++u) {
+            const int e = tid + u * kThreads;
+            const int col = e / (B / 2), r2 = (e % (B / 2)) * 2;
+            *reinterpret_cast<double2 *>(Ls + col * kPLD + r2) = *reinterpret_cast<const double2 *>(Ag + (size_t)col * np + r2);
+        }
     }
     if (tid == 0) *flag = 0;
     const int ncol = min(B, n - c0);
+    const bool full = ncol == B;                     // no identity columns in this block
+    double lg = 0.0;                                 // wave 1, lanes < 16: logs of its diagonal entries
+    // completion of panel q (columns 16q .. 16q + 15, all real): one wave each
+    auto finish = [&](int q) {
+        if (wave == 1) {
+            if (lane < 16) lg += log(Ls[(16 * q + lane) * kPLD + 16 * q + lane]);
+        } else if (wave == 2) {
+            // column-oriented forward substitution of L_qq x = e_c, lane c < 16
+            if (lane < 16) {
+                const double *D = Ls + 16 * q * kPLD + 16 * q;   // D(i, k) at D[k * kPLD + i]
+                const double *rd = rdiag + 16 * q;
+                double x[16];
+#pragma unroll
+                for (int i = 0; i < 16; ++i) x[i] = i == lane ? 1.0 : 0.0;
+#pragma unroll
+                for (int k = 0; k < 16; ++k) {
+                    x[k] = x[k] * rd[k];
+#pragma unroll
+                    for (int i = k + 1; i < 16; ++i) x[i] -= D[k * kPLD + i] * x[k];
+                }
+                double *W = dinv + q * 256 + lane * 16;
+#pragma unroll
+                for (int i = 0; i < 16; ++i) W[i] = x[i];
+            }
+        } else if (wave == 3) {
+            for (int e = lane; e < 16 * B; e += 64) {
+                const int col = 16 * q + e / B, r = e % B;
+                if (r >= col) Ag[(size_t)col * np + r] = Ls[col * kPLD + r];
+            }
+        }
+    };
     __syncthreads();
     for (int j = 0; j < 4; ++j) {
         if (wave == 0) {
@@ -266,28 +323,30 @@ __device__ inline int potrf64_block(double *__restrict__ A, int np, int n, int c
 #pragma unroll
             for (int k = 0; k < 16; ++k) {
                 const int c = 16 * j + k;
-                if (c < ncol && fail == 0) {
+                if (c < ncol) {
+                    // after a failed pivot the remaining columns carry
+                    // garbage; the block is abandoned at the barrier
                     const double piv = readlane_f64(p[k], c);
-                    if (!(piv > 0.0)) {
-                        fail = c0 + c + 1;
-                    } else {
-                        double d, rinv;
-                        sqrt_rsqrt(piv, d, rinv);
-                        if (lane == 0) rdiag[c] = rinv;
-                        p[k] = lane > c ? p[k] * rinv : (lane == c ? d : p[k]);
+                    if (fail == 0 && !(piv > 0.0)) fail = c0 + c + 1;
+                    double d, rinv;
+                    sqrt_rsqrt(piv, d, rinv);
+                    if (lane == 0) rdiag[c] = rinv;
+                    p[k] = lane == c ? d : p[k] * rinv;
+                    // the finished column goes to LDS, where the multipliers
+                    // L(16j + k2, c) of the later columns are read back as
+                    // broadcasts; the next column's one comes by v_readlane
+                    // (it gates the next pivot)
+                    double *col = Ls + c * kPLD;
+                    col[lane] = p[k];
+                    __builtin_amdgcn_wave_barrier();
+                    if (k + 1 < 16) p[k + 1] = fma(-p[k], readlane_f64(p[k], c + 1), p[k + 1]);
 #pragma unroll
-                        for (int k2 = k + 1; k2 < 16; ++k2) {
-                            const double lc = readlane_f64(p[k], 16 * j + k2);
-                            if (lane >= 16 * j + k2) p[k2] -= p[k] * lc;
-                        }
-                    }
+                    for (int k2 = k + 2; k2 < 16; ++k2) p[k2] = fma(-p[k], col[16 * j + k2], p[k2]);
                 }
             }
-            if (lane >= 16 * j) {
-#pragma unroll
-                for (int k = 0; k < 16; ++k) Ls[(16 * j + k) * kPLD + lane] = p[k];
-            }
             if (lane == 0) *flag = fail;
+        } else if (full && j > 0) {
+            finish(j - 1);
         }
         __syncthreads();
         if (*flag != 0) return *flag;
@@ -308,43 +367,59 @@ __device__ inline int potrf64_block(double *__restrict__ A, int np, int n, int c
 #pragma unroll
                 for (int v = 0; v < 4; ++v) Ls[(16 * bj + 4 * v + lq) * kPLD + 16 * bi + lr] = acc[v];
             }
-        __syncthreads();
+        if (j < 3) __syncthreads();
     }
-    // log-determinant of the real columns: one log per lane
-    if (wave == 0) {
-        double lg = lane < ncol ? log(Ls[lane * kPLD + lane]) : 0.0;
-        for (int off = 32; off > 0; off >>= 1) lg += __shfl_xor(lg, off, 64);
-        if (lane == 0) *logdet = lg;
-    }
-    // columns at or beyond n become the identity
-    for (int e = tid; e < B * B; e += kThreads) {
-        const int col = e / B, r = e % B;
-        if (col >= ncol) Ls[col * kPLD + r] = r == col ? 1.0 : 0.0;
-    }
-    if (tid >= ncol && tid < B) rdiag[tid] = 1.0;
-    __syncthreads();
-    for (int e = tid; e < B * B; e += kThreads) {
-        const int col = e / B, r = e % B;
-        if (r >= col) Ag[(size_t)col * np + r] = Ls[col * kPLD + r];
-    }
-    // wave w inverts diagonal block w, lane c < 16 one column (column-oriented
-    // forward substitution of L_ww x = e_c)
-    if (lane < 16) {
-        const double *D = Ls + 16 * wave * kPLD + 16 * wave;   // D(i, k) at D[k * kPLD + i]
-        const double *rd = rdiag + 16 * wave;
-        double x[16];
-#pragma unroll
-        for (int i = 0; i < 16; ++i) x[i] = i == lane ? 1.0 : 0.0;
-#pragma unroll
-        for (int k = 0; k < 16; ++k) {
-            x[k] = x[k] * rd[k];
-#pragma unroll
-            for (int i = k + 1; i < 16; ++i) x[i] -= D[k * kPLD + i] * x[k];
+    if (full) {
+        finish(3);
+    } else {
+        // columns at or beyond n become the identity; then every panel is
+        // completed here
+        for (int e = tid; e < B * B; e += kThreads) {
+            const int col = e / B, r = e % B;
+            if (col >= ncol) Ls[col * kPLD + r] = r == col ? 1.0 : 0.0;
         }
-        double *W = dinv + wave * 256 + lane * 16;
+        if (tid >= ncol && tid < B) rdiag[tid] = 1.0;
+        __syncthreads();
+        if (wave == 1) {
+            // logs of the real columns only
+            for (int q = 0; q < 4; ++q)
+                if (lane < 16 && 16 * q + lane < ncol) lg += log(Ls[(16 * q + lane) * kPLD + 16 * q + lane]);
+        } else {
+            for (int q = 0; q < 4; ++q) {
+                const int w = wave;   // waves 0, 2, 3 share the stores and inverses
+                if (w == 2) {
+                    if (lane < 16) {
+                        const double *D = Ls + 16 * q * kPLD + 16 * q;
+                        const double *rd = rdiag + 16 * q;
+                        double x[16];
 #pragma unroll
-        for (int i = 0; i < 16; ++i) W[i] = x[i];
+                        for (int i = 0; i < 16; ++i) x[i] = i == lane ? 1.0 : 0.0;
+#pragma unroll
+                        for (int k = 0; k < 16; ++k) {
+                            x[k] = x[k] * rd[k];
+#pragma unroll
+                            for (int i = k + 1; i < 16; ++i) x[i] -= D[k * kPLD + i] * x[k];
+                        }
+                        double *W = dinv + q * 256 + lane * 16;
+#pragma unroll
+                        for (int i = 0; i < 16; ++i) W[i] = x[i];
+                    }
+                } else {
+                    for (int e = lane + (w == 3 ? 64 : 0); e < 16 * B; e += 128) {
+                        const int col = 16 * q + e / B, r = e % B;
+                        if (r >= col) Ag[(size_t)col * np + r] = Ls[col * kPLD + r];
+                    }
+                }
+            }
+        }
     }
+    // sum of the logs: wave 1, lanes 0..15
+    if (wave == 1) {
+        for (int off = 8; off > 0; off >>= 1) lg += __shfl_xor(lg, off, 16);
+        if (lane == 0) *lgsum = lg;
+    }
+    __syncthreads();
+    if (tid == 0) *logdet = *lgsum;
     return 0;
 }
 
@@ -359,12 +434,14 @@ __device__ inline int potrf64_block(double *__restrict__ A, int np, int n, int c
 // in lane l, component v: the B-operand layout of four consecutive k-steps,
 // so each Y_k feeds the later products straight from registers.  No LDS, no
 // barrier; the tile is read completely before it is overwritten.
-template <bool IDENT>
-__device__ inline void trsm64_solve(double *__restrict__ Ct, int ldc, const double *__restrict__ Lb, int ldl,
-                                    const double *__restrict__ dinv)
+template <bool IDENT, bool COPY = false>
+__device__ inline void trsm64_solve(const double *Cin, int ldi, double *Ct, int ldc,
+                                    const double *__restrict__ Lb, int ldl, const double *__restrict__ dinv,
+                                    double *__restrict__ copy = nullptr)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int lr = lane & 15, lq = lane >> 4;
+    const double *Ciw = Cin + 16 * wave + lr;
     double *Cw = Ct + 16 * wave + lr;
     d4 t[4];
 #pragma unroll
@@ -372,7 +449,7 @@ __device__ inline void trsm64_solve(double *__restrict__ Ct, int ldc, const doub
 #pragma unroll
         for (int v = 0; v < 4; ++v)
             t[j][v] = IDENT ? ((16 * j + 4 * v + lq == 16 * wave + lr) ? 1.0 : 0.0)
-                            : Cw[(size_t)(16 * j + 4 * v + lq) * ldc];
+                            : Ciw[(size_t)(16 * j + 4 * v + lq) * ldi];
     // L_jk (j > k) in A layout: lane l holds L_jk[l%16][4kk + l/16]
     double lf[6][4];
 #pragma unroll
@@ -403,6 +480,39 @@ __device__ inline void trsm64_solve(double *__restrict__ Ct, int ldc, const doub
     for (int j = 0; j < 4; ++j)
 #pragma unroll
         for (int v = 0; v < 4; ++v) Cw[(size_t)(16 * j + 4 * v + lq) * ldc] = y[j][v];
+    if constexpr (COPY) {
+        // the solved tile also on chip: copy[c * kPLD + r]
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) copy[(16 * j + 4 * v + lq) * kPLD + 16 * wave + lr] = y[j][v];
+    }
+}
+
+// Ls -= X X^T on the lower 16x16 blocks of a 64x64 block, both on chip
+// (Ls[c * kPLD + r], X likewise), 64 deep on the matrix cores with the
+// accumulation order of tile_gemm_sub (so the same bits as that routine's
+// depth-64 update of the same block).
+__device__ inline void syrk64_lds(double *__restrict__ Ls, const double *__restrict__ Xs)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lr = lane & 15, lq = lane >> 4;
+    int t = 0;
+    for (int bj = 0; bj < 4; ++bj)
+        for (int bi = bj; bi < 4; ++bi, ++t) {
+            if ((t & 3) != wave) continue;
+            d4 acc;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[v] = Ls[(16 * bj + 4 * v + lq) * kPLD + 16 * bi + lr];
+#pragma unroll
+            for (int kk = 0; kk < 16; ++kk) {
+                const double af = Xs[(4 * kk + lq) * kPLD + 16 * bi + lr];
+                const double bf = Xs[(4 * kk + lq) * kPLD + 16 * bj + lr];
+                acc = mfma_f64(bf, -af, acc);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v) Ls[(16 * bj + 4 * v + lq) * kPLD + 16 * bi + lr] = acc[v];
+        }
 }
 
 // L(row0:row0+64, c0:c0+64) = C L_bb^{-T} in place, C the tile and L_bb the
@@ -410,7 +520,8 @@ __device__ inline void trsm64_solve(double *__restrict__ Ct, int ldc, const doub
 __device__ inline void trsm64_mfma(double *__restrict__ A, int np, int c0, int row0,
                                    const double *__restrict__ dinv)
 {
-    trsm64_solve<false>(A + (size_t)c0 * np + row0, np, A + (size_t)c0 * np + c0, np, dinv);
+    trsm64_solve<false>(A + (size_t)c0 * np + row0, np, A + (size_t)c0 * np + row0, np, A + (size_t)c0 * np + c0, np,
+                        dinv);
 }
 
 }  // namespace fb
