@@ -27,22 +27,29 @@ REPO = Path(__file__).resolve().parent
 sys.path.insert(0, str(REPO / "ferebus-v7_amd"))
 sys.path.insert(0, str(REPO))
 
-NBO, NBI = 128, 64  # panel width / diagonal block of the factorisation (fb_common.h)
 
 
-PROFILE = REPO / "profiles" / "dag_factor_kernel_methanol_n2000_s50.md"
+# committed rocprofv3 summaries of this workload, per factorisation path
+PROFILES = {
+    "persistent": (REPO / "profiles" / "dag_factor_kernel_methanol_n2000_s50.md", "dag_factor_kernel"),
+    "lockstep-kc16": (REPO / "profiles" / "lockstep_factor_methanol_n2000_s50.md", "lockstep_factor"),
+}
 
 
-def profiled_traffic(kernel: str):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
-    summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), or None."""
+def profiled_traffic(path: str):
+    """(HBM bytes per factorisation, source) from the committed rocprofv3
+    counter summary of that path (FETCH_SIZE + WRITE_SIZE, copy-kernel
+    calibrated), or (None, None)."""
+    if path not in PROFILES:
+        return None, None
+    f, key = PROFILES[path]
     try:
-        for line in PROFILE.read_text().splitlines():
-            if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
-                return float(line.split(":")[1])
+        for line in f.read_text().splitlines():
+            if line.startswith(f"traffic_bytes_per_launch {key}:"):
+                return float(line.split(":")[1]), str(f.relative_to(REPO))
     except OSError:
         pass
-    return None
+    return None, None
 
 
 def potrf_flops(n: int) -> float:
@@ -165,23 +172,22 @@ def main():
     prof_steps = 3
     for k in range(prof_steps):
         gp.swarm_loglik(thetas[k % 4])
-    prof = {k: gp.profile_read(k) for k in ("build", "factor", "gemm", "potrf", "trsm", "likelihood", "swarm")}
+    prof = {k: gp.profile_read(k) for k in ("build", "factor", "update", "solve_factor", "update_solve",
+                                            "likelihood", "swarm")}
     gp.profile(False)
-    n, np_ = args.n, ((args.n + 1 + NBO - 1) // NBO) * NBO
+    n = args.n
     persistent = prof["factor"][1] > 0
     if persistent:
         # one persistent launch (dag_factor_kernel, after a counter reset) per evaluation
         fact_ms = prof["factor"][0] / prof_steps
         fact_kernel = "dag_factor_kernel (whole batched Cholesky of the swarm, one persistent launch)"
     else:
-        fact_ms = (prof["gemm"][0] + prof["potrf"][0] + prof["trsm"][0]) / prof_steps
-        fact_kernel = "lock-step batched Cholesky (gemm_update + potrf64 + trsm64 launches)"
+        fact_ms = (prof["update"][0] + prof["solve_factor"][0] + prof["update_solve"][0]) / prof_steps
+        fact_kernel = ("lock-step batched Cholesky: per 128-column panel panel_update_factor_kernel, "
+                       "trsm_factor_kernel, update_trsm_kernel (47 launches per evaluation, durations summed)")
     fact_flops = S * potrf_flops(n)
-    gemm_ms = prof["gemm"][0] / prof_steps
-    # update flops actually issued: panel c0 updates (np - c0) rows x 128 cols at depth c0,
-    # then (np - c0 - 64) rows x 64 cols at depth 64
-    gemm_flops = S * sum(2.0 * (np_ - c0) * NBO * c0 + 2.0 * (np_ - c0 - NBI) * NBI * NBI
-                         for c0 in range(0, np_, NBO))
+    factor_path = gp.factor_path(S)
+    traffic, traffic_src = profiled_traffic(factor_path) if (n == 2000 and S == 50) else (None, None)
     build_ms = prof["build"][0] / prof_steps
     build_bytes = S * 8.0 * n * (n + 1) / 2.0 + 8.0 * n * d
     achieved = fact_flops / (fact_ms * 1e-3) / 1e12
@@ -203,7 +209,7 @@ def main():
                                f"constant mean, marginal, nugget 1e-10, theta~U[0,1)",
                    "path": "batched (hipGraph replay per swarm evaluation, "
                            + ("persistent factorisation)" if persistent else "lock-step factorisation)"),
-                   "factor_path": gp.factor_path(S),
+                   "factor_path": factor_path,
                    "failed_particles_last_step": fails},
         "roofline": {
             "kernel": fact_kernel,
@@ -212,15 +218,13 @@ def main():
             "peak": 78.6,
             "unit": "TFLOP/s",
             "frac": round(achieved / 78.6, 4),
-            "traffic": (profiled_traffic("dag_factor_kernel")
-                        if persistent and n == 2000 and S == 50 else None),
+            "traffic": traffic,
             "algorithmic_flops_per_launch": fact_flops,
-            "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE, " + str(PROFILE.relative_to(REPO)),
+            "traffic_source": None if traffic is None else "rocprofv3 FETCH_SIZE + WRITE_SIZE, " + traffic_src,
             "avg_duration_ms": round(fact_ms, 4),
         },
         "kernels_ms_per_step": {k: round(v[0] / prof_steps, 4) for k, v in prof.items()},
         "kernel_launches_per_step": {k: v[1] // prof_steps for k, v in prof.items()},
-        "gemm_tflops": round(gemm_flops / (gemm_ms * 1e-3) / 1e12, 3) if gemm_ms > 0 else None,
         "build_hbm_gbs": round(build_bytes / (build_ms * 1e-3) / 1e9, 1) if build_ms > 0 else None,
         "wall_s": round(wall, 3),
         "device": device_name(),

@@ -17,7 +17,11 @@ from .kernel_expr import KernelConfig, KernelProgram, compile_kernel
 from .mean import MeanFit, fit_mean
 
 LIKELIHOODS = {"marginal": 0, "concentrated": 1}
-PROFILE_KINDS = {"build": 0, "gemm": 1, "potrf": 2, "trsm": 3, "likelihood": 4, "swarm": 5, "factor": 6}
+# lock-step factorisation per 128-column panel: "update" = trailing update +
+# first diagonal block, "solve_factor" = panel solve + second diagonal block,
+# "update_solve" = second block's update + solve
+PROFILE_KINDS = {"build": 0, "update": 1, "solve_factor": 2, "update_solve": 3, "likelihood": 4, "swarm": 5,
+                 "factor": 6}
 
 
 class SwarmLikelihood:

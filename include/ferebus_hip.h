@@ -206,9 +206,11 @@ int fb_pso_positions(fb_pso *p, double *x, double *v);
 
 /* Kernel timing: when enabled, each launch of the handle is bracketed by HIP
  * events and the per-kernel totals accumulate until read.
- * kind: 0 covariance build, 1 trailing-update GEMM, 2 diagonal potrf,
- *       3 panel trsm, 4 likelihood reduction, 5 whole swarm evaluation,
- *       6 persistent (single-launch) factorisation of the batched path. */
+ * kind: 0 covariance build; lock-step factorisation, per 128-column panel:
+ *       1 trailing update + first diagonal block, 2 panel solve + second
+ *       diagonal block, 3 second block's update + solve; 4 likelihood
+ *       reduction, 5 whole swarm evaluation, 6 persistent (single-launch)
+ *       factorisation. */
 int fb_profile_enable(fb_handle *h, int enable);
 int fb_profile_read(fb_handle *h, int kind, double *total_ms, long long *launches);
 
