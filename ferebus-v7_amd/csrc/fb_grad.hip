@@ -11,16 +11,17 @@
 //
 // After the batched factorisation (L in the lower triangle of each K, the
 // inverses of the 16x16 diagonal blocks in dinv, w = L^{-1}(y - ...) in zbuf):
-//   1. alpha = L^{-T} w (backsolve_swarm_kernel, one work-group per particle);
-//   2. U = L^{-T}, block row after block row: U_jj = I L_jj^{-T} and, for
+//   1. U = L^{-T}, block row after block row: U_jj = I L_jj^{-T} and, for
 //      j < i, U_ji = -(sum_{k=j}^{i-1} U_jk L_ik^T) L_ii^{-T} -- the tile GEMM
 //      and the MFMA panel solve of the factorisation (uinv_step_kernel);
-//   3. -K^{-1} = -U U^T on the lower tiles (nkinv_kernel);
-//   4. one fused pass over the lower triangle recomputes every leaf of K_ij
+//   2. columns [n, np) of U cleared (zero_tail_columns_kernel);
+//   3. alpha = L^{-T} w = U w, a coalesced product (ualpha_kernel);
+//   4. -K^{-1} = -U U^T on the lower tiles (nkinv_kernel);
+//   5. one fused pass over the lower triangle recomputes every leaf of K_ij
 //      from the staged features, as the build does, and accumulates
 //      sum_ij W_ij dK_ij/dtheta_p per parameter -- dK/dtheta is never formed
 //      (grad_pair_kernel, a chunk of parameters per pass);
-//   5. per-parameter sums over the tiles in a fixed order, times the
+//   6. per-parameter sums over the tiles in a fixed order, times the
 //      derivative of each hyperparameter map (grad_reduce_kernel).
 //
 // Leaf derivatives (K_ij = sum_t c_t exp(sum_{l in t} e_l) + nugget delta_ij,
@@ -37,48 +38,6 @@
 #include "fb_tiles.h"
 
 namespace fb {
-
-// ---- 1. alpha for every particle -----------------------------------------
-
-__global__ void __launch_bounds__(kThreads) backsolve_swarm_kernel(const double *__restrict__ K, int np, int n,
-                                                                  const double *__restrict__ wbuf,
-                                                                  const int *__restrict__ info,
-                                                                  double *__restrict__ alpha)
-{
-    extern __shared__ double al[];   // n entries
-    __shared__ double part[kThreads / 64][64];
-    const int s = blockIdx.x;
-    if (info[s] != 0) return;
-    const double *L = K + (size_t)s * np * np;
-    const double *w = wbuf + (size_t)s * n;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int c = tid; c < n; c += kThreads) al[c] = 0.0;
-    __syncthreads();
-    const int nb = ceil_div(n, 64);
-    for (int b = nb - 1; b >= 0; --b) {
-        const int c0 = b * 64;
-        const int cn = min(64, n - c0);
-        double acc = 0.0;
-        if (lane < cn) {
-            const double *col = L + (size_t)(c0 + lane) * np;
-            for (int i = c0 + cn + wave; i < n; i += kThreads / 64) acc += col[i] * al[i];
-        }
-        part[wave][lane] = acc;
-        __syncthreads();
-        if (wave == 0) {
-            double t = (lane < cn) ? w[c0 + lane] - ((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane])) : 0.0;
-            for (int c = cn - 1; c >= 0; --c) {
-                const double lcc = L[(size_t)(c0 + c) * np + c0 + c];
-                const double v = __shfl(t, c, 64) / lcc;
-                if (lane == c) t = v;
-                if (lane < c) t -= L[(size_t)(c0 + lane) * np + c0 + c] * v;
-                if (lane == c) al[c0 + c] = v;
-            }
-        }
-        __syncthreads();
-    }
-    for (int c = tid; c < n; c += kThreads) alpha[(size_t)s * n + c] = al[c];
-}
 
 // ---- 2. U = L^{-T} -------------------------------------------------------
 
@@ -126,7 +85,31 @@ __global__ void zero_tail_columns_kernel(double *__restrict__ U, int np, int n, 
     }
 }
 
-// ---- 3. -K^{-1} = -U U^T on the lower tiles ------------------------------
+// ---- 3. alpha = U w for every particle -------------------------------------
+
+// Rows [64 R, 64 R + 64) of alpha = U w (U upper triangular, column major):
+// lane = row, the four waves take every fourth column, so each wave reads
+// 512 contiguous bytes per column; the wave sums are added in a fixed order.
+__global__ void __launch_bounds__(kThreads) ualpha_kernel(const double *__restrict__ U, int np, int n,
+                                                         const double *__restrict__ wbuf,
+                                                         const int *__restrict__ info, double *__restrict__ alpha)
+{
+    __shared__ double part[kThreads / 64][64];
+    const int R = blockIdx.x, s = blockIdx.y;
+    if (info[s] != 0) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = 64 * R + lane;
+    const double *Us = U + (size_t)s * np * np;
+    const double *w = wbuf + (size_t)s * n;
+    double acc = 0.0;
+    for (int c = 64 * R + wave; c < n; c += kThreads / 64) acc += Us[(size_t)c * np + r] * w[c];
+    part[wave][lane] = acc;
+    __syncthreads();
+    if (wave == 0 && r < n)
+        alpha[(size_t)s * n + r] = (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
+}
+
+// ---- 4. -K^{-1} = -U U^T on the lower tiles ------------------------------
 
 __global__ void __launch_bounds__(kThreads) nkinv_kernel(const double *__restrict__ U, double *__restrict__ NK,
                                                         int np, int nbn, const int *__restrict__ info)
@@ -145,7 +128,7 @@ __global__ void __launch_bounds__(kThreads) nkinv_kernel(const double *__restric
                                64 * (nbn - I), smem);
 }
 
-// ---- 4. fused pass: sum_ij W_ij dK_ij/dtheta_p ----------------------------
+// ---- 5. fused pass: sum_ij W_ij dK_ij/dtheta_p ----------------------------
 
 constexpr int kGradChunk = 16;   // parameters per pass
 
@@ -359,13 +342,18 @@ __global__ void grad_reduce_kernel(const double *__restrict__ partial, int ntile
 {
     const int s = blockIdx.x;
     const KernelProgram &p = *prog;
-    for (int q = threadIdx.x; q < P; q += blockDim.x) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    // one wave per parameter: lane k sums tiles k, k + 64, ... in order, then
+    // a fixed butterfly over the 64 lanes
+    for (int q = wave; q < P; q += nw) {
         if (info[s] != 0) {
-            grad[(size_t)s * P + q] = __builtin_nan("");
+            if (lane == 0) grad[(size_t)s * P + q] = __builtin_nan("");
             continue;
         }
         double sum = 0.0;
-        for (int t = 0; t < ntiles; ++t) sum += partial[((size_t)s * ntiles + t) * P + q];
+        for (int t = lane; t < ntiles; t += 64) sum += partial[((size_t)s * ntiles + t) * P + q];
+        for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
+        if (lane != 0) continue;
         double mult = 1.0;   // nugget and constant leaves
         const double th = theta[(size_t)s * p.n_par + q];
         for (int l = 0; l < p.n_leaves; ++l) {
@@ -392,8 +380,6 @@ void launch_gradient(const SwarmBuffers &b, const GradWork &w, double *grad, hip
     const int TB = gradient_tile(b.n_slots);
     const int nbt = ceil_div(b.n, TB);
     const size_t mat = (size_t)b.np * b.np;
-    hipLaunchKernelGGL(backsolve_swarm_kernel, dim3(b.S), dim3(kThreads), (size_t)b.n * sizeof(double), st, b.K,
-                       b.np, b.n, b.zbuf, b.info, w.alpha);
     const size_t zn = mat * b.S;
     hipLaunchKernelGGL(fill_zero_kernel, dim3(1024), dim3(256), 0, st, w.U, zn);
     hipLaunchKernelGGL(fill_zero_kernel, dim3(1024), dim3(256), 0, st, w.NK, zn);
@@ -402,6 +388,8 @@ void launch_gradient(const SwarmBuffers &b, const GradWork &w, double *grad, hip
                            b.info);
     if (b.np > b.n)
         hipLaunchKernelGGL(zero_tail_columns_kernel, dim3(1024), dim3(256), 0, st, w.U, b.np, b.n, b.S);
+    hipLaunchKernelGGL(ualpha_kernel, dim3(nbn, b.S), dim3(kThreads), 0, st, w.U, b.np, b.n, b.zbuf, b.info,
+                       w.alpha);
     hipLaunchKernelGGL(nkinv_kernel, dim3(nbn * (nbn + 1) / 2, b.S), dim3(kThreads), 0, st, w.U, w.NK, b.np, nbn,
                        b.info);
     const int ntiles = nbt * (nbt + 1) / 2;
@@ -415,7 +403,7 @@ void launch_gradient(const SwarmBuffers &b, const GradWork &w, double *grad, hip
         hipLaunchKernelGGL((grad_pair_kernel<64, kMaxLeaves>), dim3(ntiles, b.S, passes), dim3(kThreads), lds, st, g);
     else
         hipLaunchKernelGGL((grad_pair_kernel<32, kMaxLeaves>), dim3(ntiles, b.S, passes), dim3(kThreads), lds, st, g);
-    hipLaunchKernelGGL(grad_reduce_kernel, dim3(b.S), dim3(64), 0, st, w.partial, ntiles, w.P, b.prog, b.theta,
+    hipLaunchKernelGGL(grad_reduce_kernel, dim3(b.S), dim3(256), 0, st, w.partial, ntiles, w.P, b.prog, b.theta,
                        b.info, grad);
 }
 
