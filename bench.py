@@ -33,6 +33,8 @@ sys.path.insert(0, str(REPO))
 PROFILES = {
     "persistent": (REPO / "profiles" / "dag_factor_kernel_methanol_n2000_s50.md", "dag_factor_kernel"),
     "lockstep-kc16": (REPO / "profiles" / "lockstep_factor_methanol_n2000_s50.md", "lockstep_factor"),
+    "lockstep-kc16-32x128": (REPO / "profiles" / "lockstep_factor_32x128_methanol_n2000_s50.md", "lockstep_factor"),
+    "lockstep-kc16-32x64": (REPO / "profiles" / "lockstep_factor_32x64_methanol_n2000_s50.md", "lockstep_factor"),
 }
 
 

@@ -150,6 +150,11 @@ struct EventPair {
 
 }  // namespace
 
+// factorisation paths the autotuner chooses from: 0 persistent single launch;
+// lock-step launches with 1: 32-deep GEMM chunks, 64x128 update tiles,
+// 2: 16-deep, 64x128, 3: 16-deep, 32x128, 4: 16-deep, 32x64
+constexpr int kNumModes = 5;
+
 struct fb_handle {
     int n = 0, d = 0, np = 0, nrhs = 0, nsteps = 0, likelihood = 0, max_swarm = 0;
     int n_slots = 0;
@@ -163,6 +168,8 @@ struct fb_handle {
     bool use_graph = true;
     bool use_dag = true;             // persistent factorisation for the batched path
     int gemm_kc = 32;                // lock-step trailing-update chunk depth
+    int tile = 0;                    // lock-step panel-update tile shape (16-deep chunks)
+    int mode = -1;                   // factorisation path in use (kNumModes below)
     // autotuner: (np, swarm size) -> 0 persistent, 1 lock-step KC 32, 2 lock-step KC 16
     bool autotune = true;
     int forced_mode = -1;            // FB_PERSISTENT=0/1 pins the path
@@ -216,6 +223,7 @@ fb::SwarmBuffers buffers(fb_handle *h, int g0, int S, const double *theta, doubl
     b.hyp = h->d_hyp + (size_t)g0 * (h->prog.n_par_kernel + fb::kMaxTerms + 1);
     b.theta = theta + (size_t)g0 * h->prog.n_par; b.prog = h->d_prog; b.prog_host = &h->prog;
     b.gemm_kc = h->gemm_kc;
+    b.tile = h->tile;
     b.K = h->d_K + (size_t)g0 * h->np * h->np;
     b.info = info + g0; b.ll = ll + g0;
     b.logdet_part = h->d_logdet + (size_t)g0 * h->nsteps;
@@ -461,7 +469,7 @@ static std::vector<TuneRecord> read_tune_file(const std::string &path)
         r.swarm = std::atoi(b + 9);
         r.path = std::atoi(c + 8);
         r.ms = m ? std::atof(m + 6) : 0.0;
-        if (r.path >= 0 && r.path <= 2) out.push_back(r);
+        if (r.path >= 0 && r.path < kNumModes) out.push_back(r);
     }
     std::fclose(f);
     return out;
@@ -581,7 +589,7 @@ int fb_create(fb_handle **out, int n, int d, const double *x, const double *y, c
     if (const char *env = std::getenv("FB_AUTOTUNE")) h->autotune = std::atoi(env) != 0;
     if (const char *env = std::getenv("FB_FACTOR_PATH")) {
         const int m = std::atoi(env);
-        if (m >= 0 && m <= 2) h->forced_mode = m;
+        if (m >= 0 && m < kNumModes) h->forced_mode = m;
     }
     if (const char *env = std::getenv("FB_TUNE_CACHE")) {
         h->tune_file = env;
@@ -651,8 +659,10 @@ int fb_set_stream(fb_handle *h, void *stream)
 
 static void set_mode(fb_handle *h, int mode)
 {
+    h->mode = mode;
     h->use_dag = mode == 0;
-    h->gemm_kc = mode == 2 ? 16 : 32;
+    h->gemm_kc = mode == 1 ? 32 : 16;
+    h->tile = mode >= 3 ? mode - 2 : 0;
 }
 
 int fb_swarm_loglik_async(fb_handle *h, int s, const double *d_theta, double *d_ll, int *d_info)
@@ -677,11 +687,12 @@ int fb_swarm_loglik_async(fb_handle *h, int s, const double *d_theta, double *d_
 }
 
 // Picks the factorisation path for a swarm of s (theta already staged): the
-// persistent launch, or the lock-step launches with 32- or 16-deep GEMM
-// chunks, each timed twice on the real inputs, the fastest kept per swarm
-// size.  All three compute the same bits (the same tile routines and
-// accumulation order), so the choice changes only speed.  Not run while a
-// stream capture is active (the cached or default choice is used).
+// persistent launch, or the lock-step launches with their GEMM chunk depth
+// and panel-update tile shape (kNumModes), each timed twice on the real
+// inputs, the fastest kept per swarm size.  All of them compute the same
+// bits (the same tile routines and per-element accumulation order), so the
+// choice changes only speed.  Not run while a stream capture is active (the
+// cached or default choice is used).
 static int choose_path(fb_handle *h, int s)
 {
     if (h->forced_mode >= 0) { set_mode(h, h->forced_mode); return FB_OK; }
@@ -699,7 +710,7 @@ static int choose_path(fb_handle *h, int s)
     FB_HIP(hipEventCreate(&e1));
     double best = 1e300;
     int best_mode = 0;
-    for (int mode = 0; mode < 3; ++mode) {
+    for (int mode = 0; mode < kNumModes; ++mode) {
         set_mode(h, mode);
         if (mode == 0 && dag_plan_for(h, s) != FB_OK) continue;
         float ms_best = 1e30f;
@@ -1250,7 +1261,7 @@ int fb_pso_step(fb_pso *p, const double *r1, const double *r2, double *cost, int
     if (h->profile || !h->use_graph) {
         enqueue(st);
     } else {
-        const int mode = h->use_dag ? 0 : (h->gemm_kc == 16 ? 2 : 1);
+        const int mode = h->mode;
         if (p->exec && (p->graph_epoch != h->graph_epoch || p->graph_mode != mode)) pso_drop_graph(p);
         if (!p->exec) {
             FB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));

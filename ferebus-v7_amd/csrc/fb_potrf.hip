@@ -25,10 +25,10 @@ namespace fb {
 
 // ---- lock-step kernels (three launches per 128-column panel) ---------------
 
-template <int KC>
+template <int KC, int MB, int NB>
 constexpr int panel_smem()
 {
-    constexpr int a = TileShape<kMB, kNBO, KC>::SMEM, b = TileShape<kMB, kNBI, KC>::SMEM;
+    constexpr int a = TileShape<MB, NB, KC>::SMEM, b = TileShape<kMB, kNBI, KC>::SMEM;
     constexpr int m = a > b ? a : b;
     return m > kPotrfSmem ? m : kPotrfSmem;
 }
@@ -37,15 +37,18 @@ constexpr int panel_smem()
 // L(c0:c0+128, 0:c0)^T.  Work-groups 0 .. S-1 take the diagonal block of
 // each matrix (its lower 64x64 only), keep the updated block on chip and
 // factor it (potrf64_block); they come first, so the factorisation runs
-// while the other row tiles are still being updated.
-template <int KC>
+// while the other row tiles are still being updated.  The other tiles are
+// MB x NB (64 x 128, 32 x 128 or 32 x 64: smaller tiles give small swarms
+// more work-groups; every shape accumulates each element in the same
+// order, so all give the same bits).
+template <int KC, int MB, int NB>
 __global__ void __launch_bounds__(kThreads)
 __attribute__((amdgpu_waves_per_eu(KC == 16 ? 2 : 1, KC == 16 ? 2 : 1))) panel_update_factor_kernel(double *__restrict__ K, int np, int n,
                                                                        int c0, int S, int *__restrict__ info,
                                                                        double *__restrict__ logdet_part,
                                                                        double *__restrict__ dinv)
 {
-    __shared__ __attribute__((aligned(16))) double smem[panel_smem<KC>()];
+    __shared__ __attribute__((aligned(16))) double smem[panel_smem<KC, MB, NB>()];
     const int b = blockIdx.x;
     if (b < S) {
         if (info[b] != 0) return;
@@ -61,11 +64,13 @@ __attribute__((amdgpu_waves_per_eu(KC == 16 ? 2 : 1, KC == 16 ? 2 : 1))) panel_u
         }
         return;
     }
-    const int rt = (np - c0) / kMB - 1;     // row tiles below the diagonal block
+    constexpr int CT = kNBO / NB;            // column tiles of the panel
+    const int rt = (np - c0 - kNBI) / MB;    // row tiles below the diagonal block
     const int t = b - S;
-    const int s = t / rt, r = 1 + t % rt;
+    const int s = t / (rt * CT), u = t % (rt * CT);
     if (info[s] != 0) return;
-    gemm_tile_update<kMB, kNBO, KC>(K + (size_t)s * np * np, np, c0 + r * kMB, c0, 0, c0, smem);
+    gemm_tile_update<MB, NB, KC>(K + (size_t)s * np * np, np, c0 + kNBI + (u / CT) * MB, c0 + (u % CT) * NB, 0, c0,
+                                 smem);
 }
 
 // Launch 2 of panel c0: every row tile below block b0 = c0 is solved with
@@ -132,15 +137,21 @@ __global__ void __launch_bounds__(kThreads) update_trsm_kernel(double *__restric
                         dinv + ((size_t)s * (np / kNBI) + c1 / kNBI) * kDinv);
 }
 
+template <int KC, int MB, int NB>
+static void panel_launch(const SwarmBuffers &b, int c0, hipStream_t st)
+{
+    // the first panel has nothing to update: only the diagonal work-groups
+    const int tiles = c0 > 0 ? ((b.np - c0 - kNBI) / MB) * (kNBO / NB) : 0;
+    hipLaunchKernelGGL((panel_update_factor_kernel<KC, MB, NB>), dim3(b.S * (1 + tiles)), dim3(kThreads), 0, st,
+                       b.K, b.np, b.n, c0, b.S, b.info, b.logdet_part, b.dinv);
+}
+
 void launch_panel_update_factor(const SwarmBuffers &b, int c0, hipStream_t st)
 {
-    const int grid = b.S * (c0 > 0 ? (b.np - c0) / kMB : 1);   // the first panel has nothing to update
-    if (b.gemm_kc == 16)
-        hipLaunchKernelGGL((panel_update_factor_kernel<16>), dim3(grid), dim3(kThreads), 0, st, b.K, b.np, b.n,
-                           c0, b.S, b.info, b.logdet_part, b.dinv);
-    else
-        hipLaunchKernelGGL((panel_update_factor_kernel<32>), dim3(grid), dim3(kThreads), 0, st, b.K, b.np, b.n,
-                           c0, b.S, b.info, b.logdet_part, b.dinv);
+    if (b.gemm_kc == 32) panel_launch<32, 64, 128>(b, c0, st);
+    else if (b.tile == 1) panel_launch<16, 32, 128>(b, c0, st);
+    else if (b.tile == 2) panel_launch<16, 32, 64>(b, c0, st);
+    else panel_launch<16, 64, 128>(b, c0, st);
 }
 
 void launch_trsm_factor(const SwarmBuffers &b, int c0, hipStream_t st)

@@ -216,13 +216,15 @@ int fb_profile_read(fb_handle *h, int kind, double *total_ms, long long *launche
 
 /* The factorisation path the batched evaluation uses for a swarm of s on
  * this handle (no reference counterpart: a diagnostic of the autotuner).
- * The first fb_swarm_loglik / fb_swarm_loglik_grad call for a swarm size
- * times the candidates on its inputs and keeps the fastest; all of them
- * compute the same bits.  path: -1 not chosen yet, 0 persistent single
- * launch, 1 lock-step launches with 32-deep GEMM chunks, 2 lock-step with
- * 16-deep chunks.  Environment, read at fb_create: FB_FACTOR_PATH=0/1/2
- * pins a path, FB_PERSISTENT=0/1 pins path 1/0, FB_AUTOTUNE=0 keeps path 0
- * without timing. */
+ * The first fb_swarm_loglik / fb_swarm_loglik_grad / fb_pso call for a swarm
+ * size times the candidates on its inputs and keeps the fastest; all of
+ * them compute the same bits.  path: -1 not chosen yet, 0 persistent single
+ * launch; lock-step launches with 1: 32-deep GEMM chunks and 64x128 update
+ * tiles, 2: 16-deep chunks, 64x128 tiles, 3: 16-deep, 32x128, 4: 16-deep,
+ * 32x64.  Environment, read at fb_create: FB_FACTOR_PATH=0..4 pins a path,
+ * FB_PERSISTENT=0/1 pins path 1/0, FB_AUTOTUNE=0 keeps path 0 without
+ * timing, FB_TUNE_CACHE=<file> keeps the choices across processes (JSON
+ * records keyed on device, padded order and swarm size). */
 int fb_factor_path(const fb_handle *h, int s, int *path);
 
 /* Library-wide information: device name into buf (len bytes), 0 on success. */

@@ -2,10 +2,11 @@
 
 The batched evaluation can factorise the swarm with one persistent launch
 (path 0) or with lock-step launches whose trailing-update GEMM stages 32-deep
-(path 1) or 16-deep (path 2) chunks.  All three run the same tile routines in
-the same accumulation order, so their results must be bit-identical; the
-autotuner (first call per swarm size) may then pick any of them.  Each path
-is also held to the oracle.
+(path 1) or 16-deep chunks (paths 2-4, with 64x128, 32x128 or 32x64 update
+tiles).  All of them run the same tile routines in the same per-element
+accumulation order, so their results must be bit-identical; the autotuner
+(first call per swarm size) may then pick any of them.  Each path is also
+held to the oracle.
 """
 import numpy as np
 import pytest
@@ -32,7 +33,7 @@ def _problem(n, d, seed):
 def test_paths_bit_identical(gpu, monkeypatch, n, S):
     rng, x, y, prog, mfit = _problem(n, 6, seed=n)
     out = {}
-    for path in (0, 1, 2):
+    for path in range(5):
         monkeypatch.setenv("FB_FACTOR_PATH", str(path))
         gp = _gp(x, y, prog, mfit, S)
         if path == 0:
@@ -43,7 +44,7 @@ def test_paths_bit_identical(gpu, monkeypatch, n, S):
         assert gp.factor_path(S) == SwarmLikelihood.FACTOR_PATHS[path]
         out[path] = (ll, info)
     print({p: o[1].tolist() for p, o in out.items()})
-    for path in (1, 2):
+    for path in range(1, 5):
         assert np.array_equal(out[path][1], out[0][1]), f"path {path}: INFO differs"
         ok = out[0][1] == 0
         assert np.array_equal(out[path][0][ok], out[0][0][ok]), f"path {path}: ll differs"
@@ -63,7 +64,7 @@ def test_autotune_picks_and_keeps(gpu, monkeypatch):
     ll, info = gp.swarm_loglik(th)
     chosen = gp.factor_path(12)
     print("autotuned path for S=12:", chosen)
-    assert chosen in ("persistent", "lockstep-kc32", "lockstep-kc16")
+    assert chosen in SwarmLikelihood.FACTOR_PATHS.values() and chosen != "untuned"
     ll2, info2 = gp.swarm_loglik(th)
     assert gp.factor_path(12) == chosen
     assert np.array_equal(ll, ll2) and np.array_equal(info, info2)
@@ -77,3 +78,26 @@ def test_autotune_picks_and_keeps(gpu, monkeypatch):
     gp2 = _gp(x, y, prog, mfit, 12)
     ll3, info3 = gp2.swarm_loglik(th)
     assert np.array_equal(info3, info) and np.array_equal(ll3[info == 0], ll[info == 0])
+
+
+def test_tuning_cache_file(gpu, monkeypatch, tmp_path):
+    """FB_TUNE_CACHE: the first handle's choice is written as a JSON record
+    and a new handle of the same order starts with it."""
+    import json
+    monkeypatch.delenv("FB_FACTOR_PATH", raising=False)
+    monkeypatch.delenv("FB_PERSISTENT", raising=False)
+    monkeypatch.delenv("FB_AUTOTUNE", raising=False)
+    cache = tmp_path / "tune.json"
+    monkeypatch.setenv("FB_TUNE_CACHE", str(cache))
+    rng, x, y, prog, mfit = _problem(500, 6, seed=8)
+    gp = _gp(x, y, prog, mfit, 6)
+    th = rng.uniform(0.0, 1.0, (6, gp.nparams))
+    ll, info = gp.swarm_loglik(th)
+    chosen = gp.factor_path(6)
+    recs = json.loads(cache.read_text())
+    print(recs)
+    assert any(r["swarm"] == 6 and SwarmLikelihood.FACTOR_PATHS[r["path"]] == chosen for r in recs)
+    gp2 = _gp(x, y, prog, mfit, 6)
+    assert gp2.factor_path(6) == chosen          # known before any evaluation
+    ll2, info2 = gp2.swarm_loglik(th)
+    assert np.array_equal(ll, ll2) and np.array_equal(info, info2)
