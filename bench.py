@@ -59,6 +59,47 @@ def potrf_flops(n: int) -> float:
     return n ** 3 / 3.0 + n ** 2 / 2.0 + n / 6.0
 
 
+def uneven_run(steps: int = 3):
+    """Batched uneven training sets: 10 glycine-like GPs (d=24, N ~ U[1000,
+    4000]), 40 particles each -- 400 factorisations of mixed size per PSO
+    iteration.  All of them in one persistent launch (fb_multi_swarm_loglik)
+    against the ten handles' own batched evaluations one after another
+    (each on its autotuned path); the results are bit-identical."""
+    import torch
+    from ferebus_amd import MultiSwarm, SwarmLikelihood, compile_kernel, fit_mean, synthetic
+    ws = synthetic.glycine_gps(seed=2)
+    rng = np.random.default_rng(9)
+    gps, thetas = [], []
+    for w in ws:
+        prog = compile_kernel(w.kernel, w.kernels, w.x.shape[1])
+        gps.append(SwarmLikelihood(w.x, w.y, prog, mean=fit_mean(w.mean, w.x, w.y), noise=w.noise,
+                                   likelihood=w.likelihood, max_swarm=40))
+        thetas.append(rng.uniform(0.0, 1.0, (40, gps[-1].nparams)))
+    multi = MultiSwarm(gps)
+    multi.swarm_loglik(thetas)
+    for gp, th in zip(gps, thetas):
+        gp.swarm_loglik(th)                  # tunes each handle's path
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(steps):
+        lls, infos = multi.swarm_loglik(thetas)
+    t_multi = (time.perf_counter() - t) / steps
+    t = time.perf_counter()
+    for _ in range(steps):
+        for gp, th in zip(gps, thetas):
+            gp.swarm_loglik(th)
+    t_seq = (time.perf_counter() - t) / steps
+    same = all(np.array_equal(gp.swarm_loglik(th)[0], ll) for gp, th, ll in zip(gps, thetas, lls))
+    out = {"gps": len(gps), "N": [gp.n for gp in gps], "particles": 40 * len(gps),
+           "ms_one_launch": round(1e3 * t_multi, 3), "ms_per_gp_evaluations": round(1e3 * t_seq, 3),
+           "evals_per_s_one_launch": round(40 * len(gps) / t_multi, 1),
+           "per_gp_paths": sorted({gp.factor_path(40) for gp in gps}), "bit_identical": bool(same)}
+    multi.close()
+    for gp in gps:
+        gp.close()
+    return out
+
+
 def converge_runs(gp, S, cap):
     """Wall time to the converged logL of the swarm search on this workload.
 
@@ -114,6 +155,8 @@ def main():
                     help="skip the wall-time-to-converged-logL runs (plain PSO vs L-BFGS-polished PSO)")
     ap.add_argument("--converge-cap", type=int, default=1000,
                     help="iteration cap of each PSO run")
+    ap.add_argument("--no-uneven", action="store_true",
+                    help="skip the batched-uneven run (10 glycine-like GPs, N in [1000, 4000], swarm 40 each)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run 5 device-to-device copies of this many bytes (a known-size "
                          "copy kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters)")
@@ -241,6 +284,9 @@ def main():
 
     if not args.no_converge:
         result["converged"] = converge_runs(gp, S, args.converge_cap)
+
+    if not args.no_uneven:
+        result["batched_uneven"] = uneven_run()
 
     if not args.no_cpu:
         import oracle
