@@ -35,6 +35,8 @@ PROFILES = {
     "lockstep-kc16": (REPO / "profiles" / "lockstep_factor_methanol_n2000_s50.md", "lockstep_factor"),
     "lockstep-kc16-32x128": (REPO / "profiles" / "lockstep_factor_32x128_methanol_n2000_s50.md", "lockstep_factor"),
     "lockstep-kc16-32x64": (REPO / "profiles" / "lockstep_factor_32x64_methanol_n2000_s50.md", "lockstep_factor"),
+    "lockstep-kc16-per-panel": (REPO / "profiles" / "lockstep_factor_per_panel_methanol_n2000_s50.md",
+                                "lockstep_factor"),
 }
 
 

@@ -152,8 +152,11 @@ struct EventPair {
 
 // factorisation paths the autotuner chooses from: 0 persistent single launch;
 // lock-step launches with 1: 32-deep GEMM chunks, 64x128 update tiles,
-// 2: 16-deep, 64x128, 3: 16-deep, 32x128, 4: 16-deep, 32x64
+// 2: 16-deep, 64x128, 3: 16-deep, 32x128, 4: 16-deep, 32x64 (kNumModes, each
+// can be pinned); 5: 16-deep with each panel's fastest of those three tile
+// shapes (measured per panel, so only reachable by tuning)
 constexpr int kNumModes = 5;
+constexpr int kModePerPanel = 5;
 
 struct fb_handle {
     int n = 0, d = 0, np = 0, nrhs = 0, nsteps = 0, likelihood = 0, max_swarm = 0;
@@ -169,6 +172,10 @@ struct fb_handle {
     bool use_dag = true;             // persistent factorisation for the batched path
     int gemm_kc = 32;                // lock-step trailing-update chunk depth
     int tile = 0;                    // lock-step panel-update tile shape (16-deep chunks)
+    const signed char *panel_tile = nullptr;   // per-panel shapes of mode kModePerPanel
+    std::map<int, std::vector<signed char>> panel_tiles;   // per swarm size
+    std::vector<signed char> pinned_tiles;   // FB_PANEL_TILES with FB_FACTOR_PATH=5: every swarm size
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> *k1_events = nullptr;   // tuning: times each panel update
     int mode = -1;                   // factorisation path in use (kNumModes below)
     // autotuner: (np, swarm size) -> 0 persistent, 1 lock-step KC 32, 2 lock-step KC 16
     bool autotune = true;
@@ -224,6 +231,7 @@ fb::SwarmBuffers buffers(fb_handle *h, int g0, int S, const double *theta, doubl
     b.theta = theta + (size_t)g0 * h->prog.n_par; b.prog = h->d_prog; b.prog_host = &h->prog;
     b.gemm_kc = h->gemm_kc;
     b.tile = h->tile;
+    b.panel_tile = h->panel_tile;
     b.K = h->d_K + (size_t)g0 * h->np * h->np;
     b.info = info + g0; b.ll = ll + g0;
     b.logdet_part = h->d_logdet + (size_t)g0 * h->nsteps;
@@ -265,7 +273,15 @@ void enqueue_group(fb_handle *h, const fb::SwarmBuffers &b, hipStream_t st)
     // update + panel solve (kind 3)
     for (int c0 = 0; c0 < h->np; c0 += fb::kNBO) {
         const int c1 = c0 + fb::kNBI;
-        timed(h, 1, st, [&] { fb::launch_panel_update_factor(b, c0, st); });
+        if (h->k1_events) {
+            std::pair<hipEvent_t, hipEvent_t> ev{take_event(h), take_event(h)};
+            (void)hipEventRecord(ev.first, st);
+            fb::launch_panel_update_factor(b, c0, st);
+            (void)hipEventRecord(ev.second, st);
+            h->k1_events->push_back(ev);
+        } else {
+            timed(h, 1, st, [&] { fb::launch_panel_update_factor(b, c0, st); });
+        }
         timed(h, 2, st, [&] { fb::launch_trsm_factor(b, c0, st); });
         if (c1 + fb::kNBI < h->np) timed(h, 3, st, [&] { fb::launch_update_trsm(b, c1, st); });
     }
@@ -445,6 +461,7 @@ struct TuneRecord {
     std::string device;
     int np = 0, swarm = 0, path = -1;
     double ms = 0.0;
+    std::string tiles;     // per-panel shapes of the per-panel path, "0,2,1,..."
 };
 
 static std::vector<TuneRecord> read_tune_file(const std::string &path)
@@ -469,7 +486,12 @@ static std::vector<TuneRecord> read_tune_file(const std::string &path)
         r.swarm = std::atoi(b + 9);
         r.path = std::atoi(c + 8);
         r.ms = m ? std::atof(m + 6) : 0.0;
-        if (r.path >= 0 && r.path < kNumModes) out.push_back(r);
+        if (const char *t = std::strstr(line, "\"tiles\": \"")) {
+            t += 10;
+            const char *te = std::strchr(t, '"');
+            if (te) r.tiles.assign(t, te);
+        }
+        if (r.path >= 0 && (r.path < kNumModes || (r.path == kModePerPanel && !r.tiles.empty()))) out.push_back(r);
     }
     std::fclose(f);
     return out;
@@ -481,15 +503,20 @@ static void write_tune_file(const fb_handle *h, int s, int path, double ms)
     bool found = false;
     for (auto &r : recs)
         if (r.device == h->device_key && r.np == h->np && r.swarm == s) { r.path = path; r.ms = ms; found = true; }
-    if (!found) recs.push_back({h->device_key, h->np, s, path, ms});
+    std::string tiles;
+    if (path == kModePerPanel && h->panel_tiles.count(s))
+        for (signed char t : h->panel_tiles.at(s)) tiles += (tiles.empty() ? "" : ",") + std::to_string((int)t);
+    for (auto &r : recs)
+        if (r.device == h->device_key && r.np == h->np && r.swarm == s) r.tiles = tiles;
+    if (!found) recs.push_back({h->device_key, h->np, s, path, ms, tiles});
     const std::string tmp = h->tune_file + ".tmp" + std::to_string((long long)std::rand());
     FILE *f = std::fopen(tmp.c_str(), "w");
     if (!f) return;   // a cache that cannot be written only costs re-tuning
     std::fprintf(f, "[\n");
     for (size_t k = 0; k < recs.size(); ++k)
-        std::fprintf(f, "  {\"device\": \"%s\", \"np\": %d, \"swarm\": %d, \"path\": %d, \"ms\": %.4f}%s\n",
+        std::fprintf(f, "  {\"device\": \"%s\", \"np\": %d, \"swarm\": %d, \"path\": %d, \"ms\": %.4f, \"tiles\": \"%s\"}%s\n",
                      recs[k].device.c_str(), recs[k].np, recs[k].swarm, recs[k].path, recs[k].ms,
-                     k + 1 < recs.size() ? "," : "");
+                     recs[k].tiles.c_str(), k + 1 < recs.size() ? "," : "");
     std::fprintf(f, "]\n");
     const bool ok = std::fclose(f) == 0;
     if (!ok || std::rename(tmp.c_str(), h->tune_file.c_str()) != 0) std::remove(tmp.c_str());
@@ -590,6 +617,23 @@ int fb_create(fb_handle **out, int n, int d, const double *x, const double *y, c
     if (const char *env = std::getenv("FB_FACTOR_PATH")) {
         const int m = std::atoi(env);
         if (m >= 0 && m < kNumModes) h->forced_mode = m;
+        // the per-panel path with its shapes given: FB_PANEL_TILES=t0,t1,...
+        // (one of 0 64x128, 1 32x128, 2 32x64 per 128-column panel)
+        const char *tl = std::getenv("FB_PANEL_TILES");
+        if (m == kModePerPanel && tl) {
+            std::vector<signed char> v;
+            for (const char *c = tl; *c;) {
+                v.push_back((signed char)std::atoi(c));
+                while (*c && *c != ',') ++c;
+                if (*c == ',') ++c;
+            }
+            bool ok = (int)v.size() == h->np / fb::kNBO;
+            for (signed char t : v) ok = ok && t >= 0 && t <= 2;
+            if (ok) {
+                h->pinned_tiles = v;
+                h->forced_mode = kModePerPanel;
+            }
+        }
     }
     if (const char *env = std::getenv("FB_TUNE_CACHE")) {
         h->tune_file = env;
@@ -598,9 +642,22 @@ int fb_create(fb_handle **out, int n, int d, const double *x, const double *y, c
         if (!h->tune_file.empty() && hipGetDevice(&dev) == hipSuccess &&
             hipGetDeviceProperties(&prop, dev) == hipSuccess) {
             h->device_key = std::string(prop.gcnArchName) + "/" + std::to_string(prop.multiProcessorCount) + "cu";
-            for (const auto &r : read_tune_file(h->tune_file))
-                if (r.device == h->device_key && r.np == h->np && (r.path != 0 || h->dag_grid > 0))
-                    h->tuned[r.swarm] = r.path;
+            for (const auto &r : read_tune_file(h->tune_file)) {
+                if (r.device != h->device_key || r.np != h->np || (r.path == 0 && h->dag_grid <= 0)) continue;
+                if (r.path == kModePerPanel) {
+                    std::vector<signed char> v;
+                    for (const char *c = r.tiles.c_str(); *c;) {
+                        v.push_back((signed char)std::atoi(c));
+                        while (*c && *c != ',') ++c;
+                        if (*c == ',') ++c;
+                    }
+                    bool ok = (int)v.size() == h->np / fb::kNBO;
+                    for (signed char t : v) ok = ok && t >= 0 && t <= 2;
+                    if (!ok) continue;
+                    h->panel_tiles[r.swarm] = v;
+                }
+                h->tuned[r.swarm] = r.path;
+            }
         } else {
             h->tune_file.clear();
         }
@@ -657,12 +714,18 @@ int fb_set_stream(fb_handle *h, void *stream)
     return FB_OK;
 }
 
-static void set_mode(fb_handle *h, int mode)
+static void set_mode(fb_handle *h, int mode, int s = 0)
 {
     h->mode = mode;
     h->use_dag = mode == 0;
     h->gemm_kc = mode == 1 ? 32 : 16;
-    h->tile = mode >= 3 ? mode - 2 : 0;
+    h->tile = (mode >= 3 && mode <= 4) ? mode - 2 : 0;
+    h->panel_tile = nullptr;
+    if (mode == kModePerPanel) {
+        auto it = h->panel_tiles.find(s);
+        if (it != h->panel_tiles.end()) h->panel_tile = it->second.data();
+        else if (!h->pinned_tiles.empty()) h->panel_tile = h->pinned_tiles.data();
+    }
 }
 
 int fb_swarm_loglik_async(fb_handle *h, int s, const double *d_theta, double *d_ll, int *d_info)
@@ -671,7 +734,8 @@ int fb_swarm_loglik_async(fb_handle *h, int s, const double *d_theta, double *d_
     if (s < 1 || s > h->max_swarm) return fail(FB_E_CAPACITY, "fb_swarm_loglik_async: swarm size out of range");
     {
         auto it = h->tuned.find(s);
-        set_mode(h, h->forced_mode >= 0 ? h->forced_mode : it != h->tuned.end() ? it->second : (h->dag_grid > 0 ? 0 : 1));
+        set_mode(h, h->forced_mode >= 0 ? h->forced_mode : it != h->tuned.end() ? it->second : (h->dag_grid > 0 ? 0 : 1),
+                 s);
     }
     if (h->use_dag) {
         // the persistent schedule's descriptors name the handle's INFO buffer
@@ -697,7 +761,7 @@ static int choose_path(fb_handle *h, int s)
 {
     if (h->forced_mode >= 0) { set_mode(h, h->forced_mode); return FB_OK; }
     auto it = h->tuned.find(s);
-    if (it != h->tuned.end()) { set_mode(h, it->second); return FB_OK; }
+    if (it != h->tuned.end()) { set_mode(h, it->second, s); return FB_OK; }
     if (!h->autotune || h->dag_grid <= 0) { set_mode(h, h->dag_grid > 0 ? 0 : 1); return FB_OK; }
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(h->stream, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) {
@@ -730,11 +794,54 @@ static int choose_path(fb_handle *h, int s)
         }
         if (ms_best < best) { best = ms_best; best_mode = mode; }
     }
+    // the panel updates of the three 16-deep shapes, timed panel by panel: a
+    // mix of the fastest shape of each panel is one more candidate
+    const int npan = h->np / fb::kNBO;
+    std::vector<std::vector<float>> tpan(3, std::vector<float>(npan, 1e30f));
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> evs;
+    for (int t = 0; t < 3; ++t) {
+        set_mode(h, 2 + t, s);
+        evs.clear();
+        h->k1_events = &evs;
+        enqueue_pipeline(h, s, h->d_theta, h->d_ll, h->d_info, st);
+        h->k1_events = nullptr;
+        if (hipStreamSynchronize(st) != hipSuccess) break;
+        for (size_t k = 0; k < evs.size() && (int)k < npan; ++k)
+            (void)hipEventElapsedTime(&tpan[t][k], evs[k].first, evs[k].second);
+        for (auto &e : evs) {
+            h->event_pool.push_back(e.first);
+            h->event_pool.push_back(e.second);
+        }
+    }
+    std::vector<signed char> mix(npan, 0);
+    bool mixed = false;
+    for (int k = 0; k < npan; ++k) {
+        int bt = 0;
+        for (int t = 1; t < 3; ++t) if (tpan[t][k] < tpan[bt][k]) bt = t;
+        mix[k] = (signed char)bt;
+        mixed = mixed || bt != mix[0];
+    }
+    if (mixed) {
+        h->panel_tiles[s] = mix;
+        set_mode(h, kModePerPanel, s);
+        float ms_best = 1e30f;
+        for (int rep = 0; rep < 2; ++rep) {
+            (void)hipEventRecord(e0, st);
+            enqueue_pipeline(h, s, h->d_theta, h->d_ll, h->d_info, st);
+            (void)hipEventRecord(e1, st);
+            if (hipEventSynchronize(e1) != hipSuccess) break;
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            ms_best = std::min(ms_best, ms);
+        }
+        if (ms_best < best) { best = ms_best; best_mode = kModePerPanel; }
+        else h->panel_tiles.erase(s);
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     FB_HIP(hipGetLastError());
     h->tuned[s] = best_mode;
-    set_mode(h, best_mode);
+    set_mode(h, best_mode, s);
     if (!h->tune_file.empty()) write_tune_file(h, s, best_mode, best);
     return FB_OK;
 }

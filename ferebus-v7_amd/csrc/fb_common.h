@@ -78,6 +78,7 @@ struct SwarmBuffers {
     const KernelProgram *prog_host;   // host copy (passed by value to kernels whose loops read it)
     int gemm_kc = 32;      // chunk depth of the lock-step trailing update (16 or 32)
     int tile = 0;          // lock-step panel-update tiles: 0 64x128, 1 32x128, 2 32x64 (16-deep chunks)
+    const signed char *panel_tile = nullptr;   // host: per-panel tile shape (overrides `tile`), or none
     double *K;             // S x np x np
     int *info;             // S
     double *logdet_part;   // S x (np / kNBI)

@@ -148,9 +148,10 @@ static void panel_launch(const SwarmBuffers &b, int c0, hipStream_t st)
 
 void launch_panel_update_factor(const SwarmBuffers &b, int c0, hipStream_t st)
 {
+    const int tile = b.panel_tile ? b.panel_tile[c0 / kNBO] : b.tile;
     if (b.gemm_kc == 32) panel_launch<32, 64, 128>(b, c0, st);
-    else if (b.tile == 1) panel_launch<16, 32, 128>(b, c0, st);
-    else if (b.tile == 2) panel_launch<16, 32, 64>(b, c0, st);
+    else if (tile == 1) panel_launch<16, 32, 128>(b, c0, st);
+    else if (tile == 2) panel_launch<16, 32, 64>(b, c0, st);
     else panel_launch<16, 64, 128>(b, c0, st);
 }
 

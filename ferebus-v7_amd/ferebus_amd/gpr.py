@@ -151,7 +151,7 @@ class SwarmLikelihood:
         _abi.check(self._lib.fb_set_stream(self._h, ctypes.c_void_p(stream_handle or 0)), "fb_set_stream")
 
     FACTOR_PATHS = {-1: "untuned", 0: "persistent", 1: "lockstep-kc32", 2: "lockstep-kc16",
-                    3: "lockstep-kc16-32x128", 4: "lockstep-kc16-32x64"}
+                    3: "lockstep-kc16-32x128", 4: "lockstep-kc16-32x64", 5: "lockstep-kc16-per-panel"}
 
     def factor_path(self, s: int) -> str:
         """Factorisation path chosen for a swarm of s (see fb_factor_path)."""

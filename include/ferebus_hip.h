@@ -221,10 +221,14 @@ int fb_profile_read(fb_handle *h, int kind, double *total_ms, long long *launche
  * them compute the same bits.  path: -1 not chosen yet, 0 persistent single
  * launch; lock-step launches with 1: 32-deep GEMM chunks and 64x128 update
  * tiles, 2: 16-deep chunks, 64x128 tiles, 3: 16-deep, 32x128, 4: 16-deep,
- * 32x64.  Environment, read at fb_create: FB_FACTOR_PATH=0..4 pins a path,
- * FB_PERSISTENT=0/1 pins path 1/0, FB_AUTOTUNE=0 keeps path 0 without
- * timing, FB_TUNE_CACHE=<file> keeps the choices across processes (JSON
- * records keyed on device, padded order and swarm size). */
+ * 32x64, 5: 16-deep with each panel's fastest of those three tile shapes
+ * (timed panel by panel).  Environment, read at fb_create:
+ * FB_FACTOR_PATH=0..4 pins a path; FB_FACTOR_PATH=5 with
+ * FB_PANEL_TILES=t0,t1,... (one shape per 128-column panel, 0..2 as in
+ * paths 2..4) pins a per-panel mix; FB_PERSISTENT=0/1 pins path 1/0;
+ * FB_AUTOTUNE=0 keeps path 0 without timing; FB_TUNE_CACHE=<file> keeps the
+ * choices across processes (JSON records keyed on device, padded order and
+ * swarm size, with the per-panel shapes). */
 int fb_factor_path(const fb_handle *h, int s, int *path);
 
 /* Library-wide information: device name into buf (len bytes), 0 on success. */
