@@ -401,8 +401,8 @@ void drop_graphs(fb_handle *h)
 
 void free_grad(fb_handle *h)
 {
-    for (void *p : {(void *)h->grad.U, (void *)h->grad.NK, (void *)h->grad.alpha, (void *)h->grad.partial,
-                    (void *)h->d_grad})
+    for (void *p : {(void *)h->grad.U, (void *)h->grad.NK, (void *)h->grad.X, (void *)h->grad.alpha,
+                    (void *)h->grad.partial, (void *)h->d_grad})
         if (p) (void)hipFree(p);
     if (h->h_grad) (void)hipHostFree(h->h_grad);
     h->grad = fb::GradWork();
@@ -968,6 +968,7 @@ int fb_swarm_loglik_grad(fb_handle *h, int s, const double *theta, double *ll, d
         const size_t npart = fb::gradient_partial_count(h->n, h->n_slots, s, P);
         if (hipMalloc((void **)&h->grad.U, sizeof(double) * mat) != hipSuccess ||
             hipMalloc((void **)&h->grad.NK, sizeof(double) * mat) != hipSuccess ||
+            hipMalloc((void **)&h->grad.X, sizeof(double) * mat) != hipSuccess ||
             hipMalloc((void **)&h->grad.alpha, sizeof(double) * (size_t)h->n * s) != hipSuccess ||
             hipMalloc((void **)&h->grad.partial, sizeof(double) * npart) != hipSuccess ||
             hipMalloc((void **)&h->d_grad, sizeof(double) * (size_t)P * s) != hipSuccess ||

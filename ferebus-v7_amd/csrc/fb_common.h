@@ -97,7 +97,8 @@ void launch_likelihood(const SwarmBuffers &b, hipStream_t st);
 // Adjoint gradient (fb_grad.hip).
 struct GradWork {
     double *U = nullptr;        // S x np x np: L^{-T}
-    double *NK = nullptr;       // S x np x np: -K^{-1} (lower tiles)
+    double *NK = nullptr;       // S x np x np: -K^{-1} (lower tiles); scratch of the inverse before that
+    double *X = nullptr;        // S x np x np: L^{-1} (lower blocks), built alongside U
     double *alpha = nullptr;    // S x n
     double *partial = nullptr;  // gradient_partial_count(n, n_slots, S, P)
     int P = 0;

@@ -11,9 +11,9 @@
 //
 // After the batched factorisation (L in the lower triangle of each K, the
 // inverses of the 16x16 diagonal blocks in dinv, w = L^{-1}(y - ...) in zbuf):
-//   1. U = L^{-T}, block row after block row: U_jj = I L_jj^{-T} and, for
-//      j < i, U_ji = -(sum_{k=j}^{i-1} U_jk L_ik^T) L_ii^{-T} -- the tile GEMM
-//      and the MFMA panel solve of the factorisation (uinv_step_kernel);
+//   1. U = L^{-T} by recursive doubling over block sizes 64, 128, ... (the
+//      diagonal blocks by the identity panel solve, then two tile-GEMM
+//      launches per level, uinv_diag / uinv_t / uinv_ux kernels);
 //   2. columns [n, np) of U cleared (zero_tail_columns_kernel);
 //   3. alpha = L^{-T} w = U w, a coalesced product (ualpha_kernel);
 //   4. -K^{-1} = -U U^T on the lower tiles (nkinv_kernel);
@@ -39,40 +39,101 @@
 
 namespace fb {
 
-// ---- 2. U = L^{-T} -------------------------------------------------------
+// ---- 1. U = L^{-T} by recursive doubling ----------------------------------
+//
+// With L = [[A, 0], [B, C]] (A, C square blocks), L^{-1} = [[A^{-1}, 0],
+// [-C^{-1} B A^{-1}, C^{-1}]].  Bottom up over block sizes m = 64, 128, ...:
+// the diagonal 64x64 blocks first (U_kk = L_kk^{-T} by the identity panel
+// solve, X_kk = L_kk^{-1} its transpose), then for each pair of m-blocks
+//   T   =  U_A B^T                (m x mc)
+//   U12 = -T X_C^T                (the pair's off-diagonal block of U)
+//   X21 = -X_C T^T                (the same block of X = L^{-1})
+// with both factors always read column major (U and X are kept so that no
+// product needs a transposed operand).  U_A is upper and X_C lower
+// triangular, so each tile's depth stops at the triangle.  Two launches per
+// level, log2(Np / 64) levels: the chain of block-row steps it replaces was
+// Np / 64 dependent launches.
 
-__global__ void fill_zero_kernel(double *__restrict__ p, size_t n)
-{
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
-        p[e] = 0.0;
-}
-
-// Block row i of U for the tiles j <= i (grid: i + 1 tiles x S particles).
-__global__ void __launch_bounds__(kThreads) uinv_step_kernel(const double *__restrict__ K, double *__restrict__ U,
-                                                            int np, int i, const double *__restrict__ dinv,
+// Level 0: diagonal blocks (grid: blocks x particles).
+__global__ void __launch_bounds__(kThreads) uinv_diag_kernel(const double *__restrict__ K, double *__restrict__ U,
+                                                            double *__restrict__ X, int np,
+                                                            const double *__restrict__ dinv,
                                                             const int *__restrict__ info)
 {
-    __shared__ __attribute__((aligned(16))) double smem[TileShape<64, 64>::SMEM];
-    const int j = blockIdx.x, s = blockIdx.y;
+    __shared__ double ys[kNBI * kPLD];
+    const int k = blockIdx.x, s = blockIdx.y;
     if (info[s] != 0) return;
-    const double *L = K + (size_t)s * np * np;
-    double *Us = U + (size_t)s * np * np;
-    const double *di = dinv + ((size_t)s * (np / kNBI) + i) * kDinv;
-    double *tile = Us + (size_t)64 * i * np + 64 * j;   // rows 64j.., columns 64i..
-    const double *Lii = L + (size_t)64 * i * np + 64 * i;
-    if (j == i) {
-        trsm64_solve<true>(tile, np, tile, np, Lii, np, di);
-        return;
-    }
-    // tile = -sum_{k in [64j, 64i)} U(64j + r, k) L(64i + c, k)  (the tile starts at 0)
-    tile_gemm_sub<64, 64, kKC>(tile, np, Us + (size_t)64 * j * np + 64 * j, np, L + (size_t)64 * j * np + 64 * i, np,
-                               64 * (i - j), smem);
-    // the panel solve re-reads the tile this work-group just wrote: stores
-    // complete, then every wave drops its L1 copy (loaded as the GEMM's C)
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    const size_t mat = (size_t)s * np * np;
+    const double *Lkk = K + mat + (size_t)64 * k * np + 64 * k;
+    double *Ukk = U + mat + (size_t)64 * k * np + 64 * k;
+    trsm64_solve<true, true>(Ukk, np, Ukk, np, Lkk, np, dinv + ((size_t)s * (np / kNBI) + k) * kDinv, ys);
     __syncthreads();
-    asm volatile("buffer_inv sc1" ::: "memory");
-    trsm64_solve<false>(tile, np, tile, np, Lii, np, di);
+    // X_kk(r, c) = U_kk(c, r): ys[c' * kPLD + r'] = U_kk(r', c')
+    double *Xkk = X + mat + (size_t)64 * k * np + 64 * k;
+    for (int e = threadIdx.x; e < kNBI * kNBI; e += kThreads) {
+        const int r = e % kNBI, c = e / kNBI;
+        Xkk[(size_t)c * np + r] = ys[r * kPLD + c];
+    }
+}
+
+// T = U_A B^T for every pair of level m (grid: pairs * (m/64)^2 tiles x
+// particles; T is kept in the scratch buffer at the position of U12).
+__global__ void __launch_bounds__(kThreads) uinv_t_kernel(const double *__restrict__ K,
+                                                         const double *__restrict__ U, double *__restrict__ Tb,
+                                                         int np, int m, const int *__restrict__ info)
+{
+    __shared__ __attribute__((aligned(16))) double smem[TileShape<64, 64>::SMEM];
+    const int s = blockIdx.y;
+    if (info[s] != 0) return;
+    const int tm = m / 64, per = tm * tm;
+    const int q = blockIdx.x / per, u = blockIdx.x % per;
+    const int a0 = 2 * m * q, mc = min(m, np - a0 - m);
+    const int R = u / tm, Ct = u % tm;
+    if (mc <= 0 || Ct * 64 >= mc) return;
+    const size_t mat = (size_t)s * np * np;
+    const int k0 = 64 * R;     // U_A(r, k) = 0 for k < r
+    // T(r, c) = sum_k U_A(r, k) B(c, k);  U_A(r, k) at U[(a0 + k) np + a0 + r],
+    // B(c, k) = L(a0 + m + c, a0 + k)
+    tile_gemm_sub<64, 64, kKC, 2, false, true, true>(Tb + mat + (size_t)(a0 + m + 64 * Ct) * np + a0 + 64 * R, np,
+                                                     U + mat + (size_t)(a0 + k0) * np + a0 + 64 * R, np,
+                                                     K + mat + (size_t)(a0 + k0) * np + a0 + m + 64 * Ct, np,
+                                                     m - k0, smem);
+}
+
+// U12 = -T X_C^T and X21 = -X_C T^T for every pair of level m (grid:
+// 2 * pairs * (m/64)^2 tiles x particles).
+__global__ void __launch_bounds__(kThreads) uinv_ux_kernel(double *__restrict__ U, double *__restrict__ X,
+                                                          const double *__restrict__ Tb, int np, int m,
+                                                          const int *__restrict__ info)
+{
+    __shared__ __attribute__((aligned(16))) double smem[TileShape<64, 64>::SMEM];
+    const int s = blockIdx.y;
+    if (info[s] != 0) return;
+    const int tm = m / 64, per = tm * tm;
+    const int which = blockIdx.x % 2, t = blockIdx.x / 2;
+    const int q = t / per, u = t % per;
+    const int a0 = 2 * m * q, mc = min(m, np - a0 - m);
+    if (mc <= 0) return;
+    const size_t mat = (size_t)s * np * np;
+    // T(r, c) at Tb[(a0 + m + c) np + a0 + r] (r < m, c < mc);
+    // X_C(r, c) at X[(a0 + m + c) np + a0 + m + r]
+    const double *Tp = Tb + mat + (size_t)(a0 + m) * np + a0;
+    const double *XC = X + mat + (size_t)(a0 + m) * np + a0 + m;
+    if (which == 0) {
+        // U12(r, c) = -sum_{k <= c} T(r, k) X_C(c, k): rows r < m, columns c < mc
+        const int R = u / tm, Ct = u % tm;
+        if (Ct * 64 >= mc) return;
+        const int depth = min(64 * (Ct + 1), mc);
+        tile_gemm_sub<64, 64, kKC, 2, false, true, false>(U + mat + (size_t)(a0 + m + 64 * Ct) * np + a0 + 64 * R, np,
+                                                          Tp + 64 * R, np, XC + 64 * Ct, np, depth, smem);
+    } else {
+        // X21(r, c) = -sum_{k <= r} X_C(r, k) T(c, k): rows r < mc, columns c < m
+        const int Rt = u / tm, C = u % tm;
+        if (Rt * 64 >= mc) return;
+        const int depth = min(64 * (Rt + 1), mc);
+        tile_gemm_sub<64, 64, kKC, 2, false, true, false>(X + mat + (size_t)(a0 + 64 * C) * np + a0 + m + 64 * Rt, np,
+                                                          XC + 64 * Rt, np, Tp + 64 * C, np, depth, smem);
+    }
 }
 
 // Columns [n, np) of U are not part of L^{-T}: zero them before U U^T.
@@ -85,7 +146,7 @@ __global__ void zero_tail_columns_kernel(double *__restrict__ U, int np, int n, 
     }
 }
 
-// ---- 3. alpha = U w for every particle -------------------------------------
+// ---- 2. alpha = U w for every particle -------------------------------------
 
 // Rows [64 R, 64 R + 64) of alpha = U w (U upper triangular, column major):
 // lane = row, the four waves take every fourth column, so each wave reads
@@ -109,7 +170,7 @@ __global__ void __launch_bounds__(kThreads) ualpha_kernel(const double *__restri
         alpha[(size_t)s * n + r] = (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
 }
 
-// ---- 4. -K^{-1} = -U U^T on the lower tiles ------------------------------
+// ---- 3. -K^{-1} = -U U^T on the lower tiles ------------------------------
 
 __global__ void __launch_bounds__(kThreads) nkinv_kernel(const double *__restrict__ U, double *__restrict__ NK,
                                                         int np, int nbn, const int *__restrict__ info)
@@ -123,12 +184,12 @@ __global__ void __launch_bounds__(kThreads) nkinv_kernel(const double *__restric
     const int J = b - I * (I + 1) / 2;
     const double *Us = U + (size_t)s * np * np;
     // NK(I, J) = -sum_{k >= 64I} U(64I + r, k) U(64J + c, k)  (U is upper triangular)
-    tile_gemm_sub<64, 64, kKC>(NK + (size_t)s * np * np + (size_t)64 * J * np + 64 * I, np,
+    tile_gemm_sub<64, 64, kKC, 2, false, true>(NK + (size_t)s * np * np + (size_t)64 * J * np + 64 * I, np,
                                Us + (size_t)64 * I * np + 64 * I, np, Us + (size_t)64 * I * np + 64 * J, np,
                                64 * (nbn - I), smem);
 }
 
-// ---- 5. fused pass: sum_ij W_ij dK_ij/dtheta_p ----------------------------
+// ---- 4. fused pass: sum_ij W_ij dK_ij/dtheta_p ----------------------------
 
 constexpr int kGradChunk = 16;   // parameters per pass
 
@@ -380,12 +441,15 @@ void launch_gradient(const SwarmBuffers &b, const GradWork &w, double *grad, hip
     const int TB = gradient_tile(b.n_slots);
     const int nbt = ceil_div(b.n, TB);
     const size_t mat = (size_t)b.np * b.np;
-    const size_t zn = mat * b.S;
-    hipLaunchKernelGGL(fill_zero_kernel, dim3(1024), dim3(256), 0, st, w.U, zn);
-    hipLaunchKernelGGL(fill_zero_kernel, dim3(1024), dim3(256), 0, st, w.NK, zn);
-    for (int i = 0; i < nbn; ++i)
-        hipLaunchKernelGGL(uinv_step_kernel, dim3(i + 1, b.S), dim3(kThreads), 0, st, b.K, w.U, b.np, i, b.dinv,
+    (void)mat;
+    const int nbp = b.np / 64;
+    hipLaunchKernelGGL(uinv_diag_kernel, dim3(nbp, b.S), dim3(kThreads), 0, st, b.K, w.U, w.X, b.np, b.dinv, b.info);
+    for (int m = 64; m < b.np; m *= 2) {
+        const int pairs = ceil_div(b.np, 2 * m), tiles = pairs * (m / 64) * (m / 64);
+        hipLaunchKernelGGL(uinv_t_kernel, dim3(tiles, b.S), dim3(kThreads), 0, st, b.K, w.U, w.NK, b.np, m, b.info);
+        hipLaunchKernelGGL(uinv_ux_kernel, dim3(2 * tiles, b.S), dim3(kThreads), 0, st, w.U, w.X, w.NK, b.np, m,
                            b.info);
+    }
     if (b.np > b.n)
         hipLaunchKernelGGL(zero_tail_columns_kernel, dim3(1024), dim3(256), 0, st, w.U, b.np, b.n, b.S);
     hipLaunchKernelGGL(ualpha_kernel, dim3(nbn, b.S), dim3(kThreads), 0, st, w.U, b.np, b.n, b.zbuf, b.info,

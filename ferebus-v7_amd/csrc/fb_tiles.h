@@ -55,7 +55,8 @@ struct TileShape {
 // streamed in chunks of KC through two LDS buffers; the global loads of a
 // chunk are issued two chunks ahead (two register sets), so each has two
 // chunks of matrix-core work to land.
-template <int MB, int NB, int KC, int DIST = 2, bool TO_LDS = false>
+// ZERO_C: C starts at zero (not loaded); PLUS: C += A B^T instead of -=.
+template <int MB, int NB, int KC, int DIST = 2, bool TO_LDS = false, bool ZERO_C = false, bool PLUS = false>
 __device__ inline void tile_gemm_sub(double *__restrict__ Cg, int ldc,
                                      const double *__restrict__ Ag, int lda,
                                      const double *__restrict__ Bg, int ldb, int depth,
@@ -79,7 +80,7 @@ __device__ inline void tile_gemm_sub(double *__restrict__ Cg, int ldc,
         for (int j = 0; j < T::BN; ++j)
 #pragma unroll
             for (int v = 0; v < 4; ++v)
-                acc[i][j][v] = Cg[(size_t)(wcol + 16 * j + 4 * v + lq) * ldc + wrow + 16 * i + lr];
+                acc[i][j][v] = ZERO_C ? 0.0 : Cg[(size_t)(wcol + 16 * j + 4 * v + lq) * ldc + wrow + 16 * i + lr];
 
     if (depth > 0) {
         // per-thread load coordinates (fixed for every stage)
@@ -131,7 +132,10 @@ __device__ inline void tile_gemm_sub(double *__restrict__ Cg, int ldc,
             for (int kk = 0; kk < KC / 4; ++kk) {
                 double af[T::BM], bf[T::BN];
 #pragma unroll
-                for (int i = 0; i < T::BM; ++i) af[i] = -as[(kk * 4 + lq) * T::LDA + wrow + i * 16 + lr];
+                for (int i = 0; i < T::BM; ++i) {
+                    const double a = as[(kk * 4 + lq) * T::LDA + wrow + i * 16 + lr];
+                    af[i] = PLUS ? a : -a;
+                }
 #pragma unroll
                 for (int j = 0; j < T::BN; ++j) bf[j] = bs[(kk * 4 + lq) * T::LDB + wcol + j * 16 + lr];
 #pragma unroll
