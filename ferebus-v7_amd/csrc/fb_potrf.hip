@@ -77,10 +77,9 @@ __attribute__((amdgpu_waves_per_eu(KC == 16 ? 2 : 1, KC == 16 ? 2 : 1))) panel_u
 // L(b0, b0) (trsm64).  Work-groups 0 .. S-1 take the tile of block
 // b1 = c0 + 64, keep the solved tile on chip, apply it to the diagonal block
 // (b1, b1) (syrk64_lds) and factor that block.
-__global__ void __launch_bounds__(kThreads) trsm_factor_kernel(double *__restrict__ K, int np, int n, int c0,
-                                                               int S, int *__restrict__ info,
-                                                               double *__restrict__ logdet_part,
-                                                               double *__restrict__ dinv)
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2)))
+trsm_factor_kernel(double *__restrict__ K, int np, int n, int c0, int S, int *__restrict__ info,
+                   double *__restrict__ logdet_part, double *__restrict__ dinv)
 {
     __shared__ __attribute__((aligned(16))) double smem[kNBI * kPLD + kPotrfSmem];
     const int b = blockIdx.x;

@@ -6,6 +6,8 @@
 // slice of a 64- or 128-column block.
 #pragma once
 
+#include <type_traits>
+
 #include "fb_common.h"
 
 namespace fb {
@@ -241,13 +243,13 @@ __device__ inline double readlane_f64(double x, int l)
 
 // Factors the 64x64 diagonal block at (c0, c0) of the matrix at A,
 // right-looking in four 16-column panels, the block staged in LDS:
-//  * wave 0 factors panel j (lane = row): per column the pivot comes from
-//    its lane by v_readlane and the column's multipliers from LDS as
-//    broadcast reads of the column just written (one wave: its LDS
-//    operations complete in order), so the 16 dependent steps need no
-//    barrier (LAPACK dpotf2 test: a pivot that is not positive, or NaN,
-//    fails; L = a * (1 / sqrt(pivot)), the square root and its reciprocal
-//    from sqrt_rsqrt).  Every lane updates every column
+//  * wave 0 factors panel j (lane = row) in groups of four columns: inside
+//    a group the pivots and multipliers come from other lanes by
+//    v_readlane, and a finished group's multipliers reach the panel's later
+//    columns as LDS broadcast reads (one wave: its LDS operations complete
+//    in order), so the 16 dependent steps need no barrier (LAPACK dpotf2
+//    test: a pivot that is not positive, or NaN, fails; L = a * (1 /
+//    sqrt(pivot)), the square root and its reciprocal from sqrt_rsqrt).  Every lane updates every column
 //    of the panel: lanes above the diagonal hold upper-triangle entries,
 //    which no lower entry ever reads, so no per-update lane mask is needed;
 //  * the four waves then apply the panel to the trailing 16x16 blocks on
@@ -324,30 +326,58 @@ __device__ inline int potrf64_block(double *__restrict__ A, int np, int n, int c
 #pragma unroll
             for (int k = 0; k < 16; ++k) p[k] = Ls[(16 * j + k) * kPLD + lane];
             int fail = 0;
+            // four columns at a time: inside a group each column first takes
+            // the updates of the group's earlier columns (multipliers by
+            // v_readlane) and is then factored; the finished group goes to
+            // LDS once and the panel's later columns take its four updates,
+            // the multipliers read back as broadcasts (one wave: its LDS
+            // operations complete in order).  Every entry still receives
+            // the updates of columns 0, 1, 2, ... in that order.  After a
+            // failed pivot the remaining columns carry garbage; the block is
+            // abandoned at the barrier.  REAL: every column of the panel is
+            // below n (no per-column tests).
+            auto panel = [&](auto real_tag) {
+                constexpr bool REAL = decltype(real_tag)::value;
 #pragma unroll
-            for (int k = 0; k < 16; ++k) {
-                const int c = 16 * j + k;
-                if (c < ncol) {
-                    // after a failed pivot the remaining columns carry
-                    // garbage; the block is abandoned at the barrier
-                    const double piv = readlane_f64(p[k], c);
-                    if (fail == 0 && !(piv > 0.0)) fail = c0 + c + 1;
-                    double d, rinv;
-                    sqrt_rsqrt(piv, d, rinv);
-                    if (lane == 0) rdiag[c] = rinv;
-                    p[k] = lane == c ? d : p[k] * rinv;
-                    // the finished column goes to LDS, where the multipliers
-                    // L(16j + k2, c) of the later columns are read back as
-                    // broadcasts; the next column's one comes by v_readlane
-                    // (it gates the next pivot)
-                    double *col = Ls + c * kPLD;
-                    col[lane] = p[k];
+                for (int kb = 0; kb < 16; kb += 4) {
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        const int k = kb + t, c = 16 * j + k;
+                        if (REAL || c < ncol) {
+#pragma unroll
+                            for (int t2 = 0; t2 < t; ++t2)
+                                p[k] = fma(-p[kb + t2], readlane_f64(p[kb + t2], c), p[k]);
+                            const double piv = readlane_f64(p[k], c);
+                            if (fail == 0 && !(piv > 0.0)) fail = c0 + c + 1;
+                            double d, rinv;
+                            sqrt_rsqrt(piv, d, rinv);
+                            if (lane == 0) rdiag[c] = rinv;
+                            p[k] = lane == c ? d : p[k] * rinv;
+                        }
+                    }
+#pragma unroll
+                    for (int t = 0; t < 4; ++t)
+                        if (REAL || 16 * j + kb + t < ncol) Ls[(16 * j + kb + t) * kPLD + lane] = p[kb + t];
                     __builtin_amdgcn_wave_barrier();
-                    if (k + 1 < 16) p[k + 1] = fma(-p[k], readlane_f64(p[k], c + 1), p[k + 1]);
+                    if (REAL) {
 #pragma unroll
-                    for (int k2 = k + 2; k2 < 16; ++k2) p[k2] = fma(-p[k], col[16 * j + k2], p[k2]);
+                        for (int k2 = kb + 4; k2 < 16; ++k2)
+#pragma unroll
+                            for (int t = 0; t < 4; ++t)
+                                p[k2] = fma(-p[kb + t], Ls[(16 * j + kb + t) * kPLD + 16 * j + k2], p[k2]);
+                    } else {
+                        const int tn = min(4, ncol - (16 * j + kb));   // real columns of the group
+#pragma unroll
+                        for (int k2 = kb + 4; k2 < 16; ++k2)
+#pragma unroll
+                            for (int t = 0; t < 4; ++t)
+                                if (t < tn)
+                                    p[k2] = fma(-p[kb + t], Ls[(16 * j + kb + t) * kPLD + 16 * j + k2], p[k2]);
+                    }
                 }
-            }
+            };
+            if (16 * j + 16 <= ncol) panel(std::true_type{});
+            else panel(std::false_type{});
             if (lane == 0) *flag = fail;
         } else if (full && j > 0) {
             finish(j - 1);
