@@ -97,6 +97,7 @@ struct BuildArgs {
     // scalar loads, so the loop bounds, dimension codes and hyperparameter
     // addresses derived from it stay uniform (scalar) in the inner loops
     KernelProgram prog;
+    double expc[kExpConsts];   // constants of exp_nonpos
 };
 
 // TB: tile order; MAXL: leaves held in registers (>= n_leaves); SP: particles
@@ -192,7 +193,7 @@ __global__ void __launch_bounds__(kThreads) build_kernel(BuildArgs a)
                     for (int l = 0; l < MAXL; ++l)
                         if ((mask >> l) & 1u) { ex += e[s][l]; any = true; }
                     const double c = h[p.n_par_kernel + t];
-                    v += any ? c * exp(ex) : c;
+                    v += any ? c * exp_nonpos(ex, a.expc) : c;
                 }
                 if (i == j) v += h[p.n_par_kernel + kMaxTerms];
                 kcol[(size_t)(s0 + s) * mstride] = v;
@@ -247,6 +248,7 @@ void launch_build(const SwarmBuffers &b, hipStream_t st)
     BuildArgs a;
     a.n = b.n; a.np = b.np; a.nrhs = b.nrhs; a.S = b.S; a.n_slots = b.n_slots;
     a.slots = b.slots; a.rhs = b.RHS; a.hyp = b.hyp; a.prog = *b.prog_host; a.K = b.K; a.info = b.info;
+    exp_constants(a.expc);
     // 64 x 64 tiles while the staged features fit 64 KB of LDS, else 32 x 32
     if (build_lds_bytes(64, b.n_slots, 8, a.prog) <= 65536) launch_maxl<64>(a, b.n_leaves, st);
     else launch_maxl<32>(a, b.n_leaves, st);

@@ -24,6 +24,47 @@
 
 namespace fb {
 
+// exp(x) for the exponent of a stationary kernel term (x <= 0, or NaN).
+// Argument reduction after Cody and Waite, as in Tang (ACM TOMS 15, 1989):
+// x = k ln2 + r, |r| <= ln2/2, with ln2 split into a high part of 32
+// significant bits (k ln2_hi and x - k ln2_hi exact) and the rounded
+// remainder; exp(r) by the degree-13 Taylor polynomial in Horner form
+// (truncation below 5e-18 relative on |r| <= ln2/2; within 1 ulp of the C
+// library's exp on the CPU restatement's test points); 2^k by ldexp, which
+// flushes k < -1074 to zero.  x below -1000 is clamped there (exp = 0
+// either way), so -inf does not become NaN; NaN stays NaN.  The constants
+// c (kExpConsts of them, exp_constants() on the host) come through the
+// kernel arguments: in scalar registers they are FMA operands, where
+// literals would each need a vector-register copy per use.
+constexpr int kExpConsts = 15;   // 1/ln2, ln2 hi, ln2 lo, 1/13! .. 1/2!
+
+__host__ __device__ inline void exp_constants(double *c)
+{
+    c[0] = 1.4426950408889634074;
+    c[1] = 6.93147180369123816490e-01;
+    c[2] = 1.90821492927058770002e-10;
+    double f = 1.0;
+    for (int j = 2; j <= 13; ++j) f *= j;               // 13! (exact in a double)
+    for (int j = 13; j >= 2; --j) {
+        c[3 + 13 - j] = 1.0 / f;                       // correctly rounded 1/j!
+        f /= j;
+    }
+}
+
+__device__ inline double exp_nonpos(double x, const double *c)
+{
+    x = x < -1000.0 ? -1000.0 : x;
+    const double k = __builtin_rint(x * c[0]);
+    double r = __builtin_fma(-k, c[1], x);
+    r = __builtin_fma(-k, c[2], r);
+    double p = c[3];
+#pragma unroll
+    for (int j = 4; j < kExpConsts; ++j) p = __builtin_fma(p, r, c[j]);
+    p = __builtin_fma(p, r, 1.0);
+    p = __builtin_fma(p, r, 1.0);
+    return __builtin_ldexp(p, (int)k);
+}
+
 constexpr int kMaxLeaves = 8;     // stationary / constant leaves of the kernel tree
 constexpr int kMaxTerms = 16;     // sum-of-products terms after expansion
 constexpr int kMaxDims = 512;     // total active dimensions over all leaves

@@ -205,6 +205,7 @@ struct GradArgs {
     const int *info;
     int likelihood;
     double *partial;        // S x ntiles x P
+    double expc[kExpConsts];   // constants of exp_nonpos (the build's exp)
 };
 
 template <int MAXL>
@@ -339,7 +340,7 @@ __global__ void __launch_bounds__(kThreads) grad_pair_kernel(GradArgs a)
 #pragma unroll
             for (int l = 0; l < MAXL; ++l)
                 if ((mask >> l) & 1u) ex += e[l];
-            const double X = exp(ex);
+            const double X = exp_nonpos(ex, a.expc);
             const double T = h[p.n_par_kernel + t] * X;
 #pragma unroll
             for (int l = 0; l < MAXL; ++l) {
@@ -461,6 +462,7 @@ void launch_gradient(const SwarmBuffers &b, const GradWork &w, double *grad, hip
     g.n = b.n; g.np = b.np; g.S = b.S; g.n_slots = b.n_slots; g.P = w.P; g.ntiles = ntiles; g.pass0 = 0;
     g.slots = b.slots; g.hyp = b.hyp; g.theta = b.theta; g.prog = b.prog; g.alpha = w.alpha; g.NK = w.NK;
     g.extra = b.extra; g.info = b.info; g.likelihood = b.likelihood; g.partial = w.partial;
+    exp_constants(g.expc);
     const int passes = ceil_div(w.P, kGradChunk);
     const size_t lds = 2 * (size_t)b.n_slots * TB * sizeof(double);
     if (TB == 64)
