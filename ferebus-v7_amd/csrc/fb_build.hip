@@ -10,9 +10,9 @@
 // One work-group owns a TB x TB tile of the lower triangle of the padded
 // matrix and SP particles.  The tile's per-point features (raw x, or
 // sin(x/2) / cos(x/2) for periodic dimensions) are staged once in LDS and
-// shared by the SP particles; the per-particle hyperparameters are read with
-// uniform (scalar) loads.  Lanes of a wave own consecutive rows, so every
-// store of a column segment is one contiguous 8*TB-byte write.
+// shared by the SP particles, and so are the particles' hyperparameters
+// (broadcast reads).  Lanes of a wave own consecutive rows, so every store
+// of a column segment is one contiguous 8*TB-byte write.
 #include "fb_common.h"
 
 namespace fb {
@@ -100,17 +100,35 @@ struct BuildArgs {
     double expc[kExpConsts];   // constants of exp_nonpos
 };
 
+// e[s] += h[s] f for the SP particles of a work-group (h: SP consecutive
+// doubles in LDS, one parameter of every particle).
+template <int SP>
+__device__ inline void accum_dim(double (&e)[SP], const double *__restrict__ h, double f)
+{
+#pragma unroll
+    for (int s = 0; s < SP; s += 2) {
+        const double2 hv = *reinterpret_cast<const double2 *>(h + s);
+        e[s] = fma(hv.x, f, e[s]);
+        e[s + 1] = fma(hv.y, f, e[s + 1]);
+    }
+}
+
 // TB: tile order; MAXL: leaves held in registers (>= n_leaves); SP: particles
 // per work-group.
 template <int TB, int MAXL, int SP>
 __global__ void __launch_bounds__(kThreads) build_kernel(BuildArgs a)
 {
+    static_assert(SP % 2 == 0, "particles are read in pairs");
     constexpr int CG = kThreads / TB;   // column groups
     constexpr int CPT = TB / CG;        // columns per thread
     extern __shared__ double lds[];
     double *row_f = lds;                        // [n_slots][TB]
     double *col_f = lds + (size_t)a.n_slots * TB;
-    double *hyp_s = lds + 2 * (size_t)a.n_slots * TB;   // [SP][hyper record]: broadcast reads
+    // hyperparameter k of particle s0 + s at hyp_t[k * SP + s] (broadcast
+    // reads, two particles per read); particles past the swarm repeat the
+    // last one, so every particle of the work-group is computed without a
+    // branch and only the stores are masked
+    double *hyp_t = lds + 2 * (size_t)a.n_slots * TB;
 
     // lower-triangular tile index: b = ti*(ti+1)/2 + tj, tj <= ti
     const int b = blockIdx.x;
@@ -121,6 +139,7 @@ __global__ void __launch_bounds__(kThreads) build_kernel(BuildArgs a)
     const int s0 = blockIdx.y * SP;
     const int nsp = min(SP, a.S - s0);
     const KernelProgram &p = a.prog;
+    const int hs = hyper_stride(p);
 
     for (int e = threadIdx.x; e < a.n_slots * TB; e += kThreads) {
         const int slot = e / TB, r = e % TB;
@@ -128,9 +147,9 @@ __global__ void __launch_bounds__(kThreads) build_kernel(BuildArgs a)
         row_f[e] = gi < a.n ? a.slots[(size_t)slot * a.n + gi] : 0.0;
         col_f[e] = gj < a.n ? a.slots[(size_t)slot * a.n + gj] : 0.0;
     }
-    {
-        const int hs0 = hyper_stride(p);
-        for (int e = threadIdx.x; e < nsp * hs0; e += kThreads) hyp_s[e] = a.hyp[(size_t)s0 * hs0 + e];
+    for (int e = threadIdx.x; e < SP * hs; e += kThreads) {
+        const int k = e / SP, sp = min(s0 + e % SP, a.S - 1);
+        hyp_t[e] = a.hyp[(size_t)sp * hs + k];
     }
     if (b == 0 && threadIdx.x < nsp) a.info[s0 + threadIdx.x] = 0;
     __syncthreads();
@@ -138,66 +157,81 @@ __global__ void __launch_bounds__(kThreads) build_kernel(BuildArgs a)
     const int r = threadIdx.x % TB;
     const int cg = threadIdx.x / TB;
     const int i = ti * TB + r;
-    const int hs = hyper_stride(p);
     const size_t mstride = (size_t)a.np * a.np;
+    const double *hterm = hyp_t + (size_t)p.n_par_kernel * SP;
 
     for (int m = 0; m < CPT; ++m) {
         const int jc = cg + CG * m;
         const int j = tj * TB + jc;
         if (i < j) continue;  // strictly upper part of a diagonal tile
-        double *kcol = a.K + (size_t)j * a.np + i;
+        double *kcol = a.K + (size_t)j * a.np + i + (size_t)s0 * mstride;
         if (i < a.n && j < a.n) {
-            double e[SP][MAXL];
+            double e[MAXL][SP];
 #pragma unroll
-            for (int q = 0; q < SP; ++q)
+            for (int l = 0; l < MAXL; ++l)
 #pragma unroll
-                for (int l = 0; l < MAXL; ++l) e[q][l] = 0.0;
+                for (int s = 0; s < SP; ++s) e[l][s] = 0.0;
+            // exponents of the stationary leaves, dimension by dimension
+            // (the loops are split by leaf kind so that none branches inside)
 #pragma unroll
             for (int l = 0; l < MAXL; ++l) {
                 if (l >= p.n_leaves) break;
                 const int kind = p.leaf_kind[l];
-                if (kind == kLinear || kind == kConstant) continue;
                 const int nd = p.leaf_ndim[l];
-                const int doff = p.leaf_dim_off[l];
-                const int poff = p.leaf_par_off[l];
-                for (int q = 0; q < nd; ++q) {
-                    const int code = p.dims[doff + q];
-                    const int slot = code & 0xffff;
-                    double f;
-                    if (kind == kPeriodic) {
-                        // sin((xi-xj)/2) = sin(xi/2)cos(xj/2) - cos(xi/2)sin(xj/2)
+                const int *dims = p.dims + p.leaf_dim_off[l];
+                const double *hl = hyp_t + (size_t)p.leaf_par_off[l] * SP;
+                if (kind == kPeriodic) {
+                    // sin((xi-xj)/2) = sin(xi/2)cos(xj/2) - cos(xi/2)sin(xj/2)
+#pragma unroll 2
+                    for (int q = 0; q < nd; ++q) {
+                        const int slot = dims[q] & 0xffff;
                         const double sh_i = row_f[slot * TB + r], ch_i = row_f[(slot + 1) * TB + r];
                         const double sh_j = col_f[slot * TB + jc], ch_j = col_f[(slot + 1) * TB + jc];
                         const double sn = sh_i * ch_j - ch_i * sh_j;
-                        f = sn * sn;
-                    } else {
+                        accum_dim<SP>(e[l], hl + q * SP, sn * sn);
+                    }
+                } else if (kind == kRbf) {
+#pragma unroll 4
+                    for (int q = 0; q < nd; ++q) {
+                        const int slot = dims[q] & 0xffff;
+                        const double diff = row_f[slot * TB + r] - col_f[slot * TB + jc];
+                        accum_dim<SP>(e[l], hl + q * SP, diff * diff);
+                    }
+                } else if (kind == kRbfCyclic) {
+                    for (int q = 0; q < nd; ++q) {
+                        const int code = dims[q];
+                        const int slot = code & 0xffff;
                         double diff = row_f[slot * TB + r] - col_f[slot * TB + jc];
                         if (code >> 16) diff = cyclic_wrap_abs(diff);
-                        f = diff * diff;
+                        accum_dim<SP>(e[l], hl + q * SP, diff * diff);
                     }
-#pragma unroll
-                    for (int s = 0; s < SP; ++s)
-                        if (s < nsp) e[s][l] = fma(hyp_s[s * hs + poff + q], f, e[s][l]);
                 }
             }
+            // sum over terms of coef * exp(sum of the term's exponents); a
+            // term of linear / constant leaves only has exponent 0
+            double v[SP];
 #pragma unroll
-            for (int s = 0; s < SP; ++s) {
-                if (s >= nsp) break;
-                const double *h = hyp_s + s * hs;
-                double v = 0.0;
-                for (int t = 0; t < p.n_terms; ++t) {
-                    const uint32_t mask = p.term_mask[t];
+            for (int s = 0; s < SP; ++s) v[s] = 0.0;
+            for (int t = 0; t < p.n_terms; ++t) {
+                const uint32_t mask = p.term_mask[t];
+                const double *ct = hterm + t * SP;
+#pragma unroll
+                for (int s = 0; s < SP; ++s) {
                     double ex = 0.0;
-                    bool any = false;
 #pragma unroll
                     for (int l = 0; l < MAXL; ++l)
-                        if ((mask >> l) & 1u) { ex += e[s][l]; any = true; }
-                    const double c = h[p.n_par_kernel + t];
-                    v += any ? c * exp_nonpos(ex, a.expc) : c;
+                        if ((mask >> l) & 1u) ex += e[l][s];
+                    v[s] += ct[s] * exp_nonpos(ex, a.expc);
                 }
-                if (i == j) v += h[p.n_par_kernel + kMaxTerms];
-                kcol[(size_t)(s0 + s) * mstride] = v;
             }
+            if (i == j) {
+                const double *nug = hterm + kMaxTerms * SP;
+#pragma unroll
+                for (int s = 0; s < SP; ++s) v[s] += nug[s];
+            }
+#pragma unroll
+            for (int s = 0; s < SP; ++s)
+                if (s < nsp) kcol[s * mstride] = v[s];
         } else {
             // bordered rows (right-hand sides) and the identity padding block
             double v = 0.0;
@@ -208,7 +242,7 @@ __global__ void __launch_bounds__(kThreads) build_kernel(BuildArgs a)
             }
 #pragma unroll
             for (int s = 0; s < SP; ++s)
-                if (s < nsp) kcol[(size_t)(s0 + s) * mstride] = v;
+                if (s < nsp) kcol[s * mstride] = v;
         }
     }
 }
