@@ -254,10 +254,9 @@ static size_t build_lds_bytes(int TB, int n_slots, int SP, const KernelProgram &
     return ((size_t)2 * n_slots * TB + (size_t)SP * hyper_stride(p)) * sizeof(double);
 }
 
-template <int TB, int MAXL>
+template <int TB, int MAXL, int SP>
 static void launch_tb(const BuildArgs &a, int n_slots, hipStream_t st)
 {
-    constexpr int SP = 8;
     const int nt = a.np / TB;
     dim3 grid(nt * (nt + 1) / 2, ceil_div(a.S, SP));
     const size_t shm = build_lds_bytes(TB, n_slots, SP, a.prog);
@@ -267,13 +266,21 @@ static void launch_tb(const BuildArgs &a, int n_slots, hipStream_t st)
     hipLaunchKernelGGL((build_kernel<TB, MAXL, SP>), grid, dim3(kThreads), shm, st, a);
 }
 
-template <int TB>
+// Particles per work-group: 8, or 10 when that pads the swarm less (the
+// padding particles are computed and not stored): 50 -> 10, 20 -> 10,
+// 40 -> 8, 64 -> 8.
+static int build_sp(int S)
+{
+    return round_up(S, 10) - S < round_up(S, 8) - S ? 10 : 8;
+}
+
+template <int TB, int SP>
 static void launch_maxl(const BuildArgs &a, int n_leaves, hipStream_t st)
 {
-    if (n_leaves <= 1) launch_tb<TB, 1>(a, a.n_slots, st);
-    else if (n_leaves <= 2) launch_tb<TB, 2>(a, a.n_slots, st);
-    else if (n_leaves <= 4) launch_tb<TB, 4>(a, a.n_slots, st);
-    else launch_tb<TB, 8>(a, a.n_slots, st);
+    if (n_leaves <= 1) launch_tb<TB, 1, SP>(a, a.n_slots, st);
+    else if (n_leaves <= 2) launch_tb<TB, 2, SP>(a, a.n_slots, st);
+    else if (n_leaves <= 4) launch_tb<TB, 4, SP>(a, a.n_slots, st);
+    else launch_tb<TB, 8, SP>(a, a.n_slots, st);
 }
 
 void launch_build(const SwarmBuffers &b, hipStream_t st)
@@ -284,8 +291,14 @@ void launch_build(const SwarmBuffers &b, hipStream_t st)
     a.slots = b.slots; a.rhs = b.RHS; a.hyp = b.hyp; a.prog = *b.prog_host; a.K = b.K; a.info = b.info;
     exp_constants(a.expc);
     // 64 x 64 tiles while the staged features fit 64 KB of LDS, else 32 x 32
-    if (build_lds_bytes(64, b.n_slots, 8, a.prog) <= 65536) launch_maxl<64>(a, b.n_leaves, st);
-    else launch_maxl<32>(a, b.n_leaves, st);
+    const bool wide = build_lds_bytes(64, b.n_slots, 10, a.prog) <= 65536;
+    if (build_sp(b.S) == 10) {
+        if (wide) launch_maxl<64, 10>(a, b.n_leaves, st);
+        else launch_maxl<32, 10>(a, b.n_leaves, st);
+    } else {
+        if (wide) launch_maxl<64, 8>(a, b.n_leaves, st);
+        else launch_maxl<32, 8>(a, b.n_leaves, st);
+    }
 }
 
 }  // namespace fb
