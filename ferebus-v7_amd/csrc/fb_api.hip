@@ -171,6 +171,8 @@ struct fb_handle {
     bool use_graph = true;
     bool use_dag = true;             // persistent factorisation for the batched path
     int gemm_kc = 32;                // lock-step trailing-update chunk depth
+    bool chain = true;               // lock-step with 16-deep chunks: two launches per panel (FB_CHAIN=0: three)
+    int *d_chain = nullptr;          // per-particle words of the two-launch schedule
     int tile = 0;                    // lock-step panel-update tile shape (16-deep chunks)
     const signed char *panel_tile = nullptr;   // per-panel shapes of mode kModePerPanel
     std::map<int, std::vector<signed char>> panel_tiles;   // per swarm size
@@ -237,6 +239,8 @@ fb::SwarmBuffers buffers(fb_handle *h, int g0, int S, const double *theta, doubl
     b.logdet_part = h->d_logdet + (size_t)g0 * h->nsteps;
     b.dinv = h->d_dinv + (size_t)g0 * h->nsteps * fb::kDinv;
     b.extra = h->d_extra + (size_t)g0 * 4; b.zbuf = h->d_z + (size_t)g0 * h->n;
+    b.chain = (h->chain && h->gemm_kc == 16 && S <= fb::kChainMaxSwarm)
+                  ? h->d_chain + (size_t)g0 * fb::kChainStride : nullptr;
     return b;
 }
 
@@ -270,8 +274,23 @@ void enqueue_group(fb_handle *h, const fb::SwarmBuffers &b, hipStream_t st)
     timed(h, 0, st, [&] { fb::launch_build(b, st); });
     // per 128-column panel: update + first block's factorisation (kind 1),
     // panel solve + second block's factorisation (kind 2), second block's
-    // update + panel solve (kind 3)
+    // update + panel solve (kind 3); chained: the update launch also solves
+    // and factors the second block (kind 1) and the rows below are solved in
+    // one launch (kind 2)
     for (int c0 = 0; c0 < h->np; c0 += fb::kNBO) {
+        if (b.chain) {
+            if (h->k1_events) {
+                std::pair<hipEvent_t, hipEvent_t> ev{take_event(h), take_event(h)};
+                (void)hipEventRecord(ev.first, st);
+                fb::launch_panel_chain(b, c0, st);
+                (void)hipEventRecord(ev.second, st);
+                h->k1_events->push_back(ev);
+            } else {
+                timed(h, 1, st, [&] { fb::launch_panel_chain(b, c0, st); });
+            }
+            if (c0 + fb::kNBO < h->np) timed(h, 2, st, [&] { fb::launch_solve_rows(b, c0, st); });
+            continue;
+        }
         const int c1 = c0 + fb::kNBI;
         if (h->k1_events) {
             std::pair<hipEvent_t, hipEvent_t> ev{take_event(h), take_event(h)};
@@ -421,7 +440,7 @@ void free_all(fb_handle *h)
     if (h->h_err) (void)hipHostFree(h->h_err);
     for (const EventPair &p : h->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
     for (hipEvent_t e : h->event_pool) (void)hipEventDestroy(e);
-    void *dev[] = {h->d_prog, h->d_slots, h->d_rhs, h->d_theta, h->d_hyp, h->d_K, h->d_logdet, h->d_dinv,
+    void *dev[] = {h->d_prog, h->d_slots, h->d_rhs, h->d_theta, h->d_hyp, h->d_chain, h->d_K, h->d_logdet, h->d_dinv,
                    h->d_ll, h->d_extra, h->d_z, h->d_alpha, h->d_info};
     for (void *p : dev)
         if (p) (void)hipFree(p);
@@ -576,6 +595,7 @@ int fb_create(fb_handle **out, int n, int d, const double *x, const double *y, c
     FB_ALLOC(h->d_rhs, sizeof(double) * rhs.size());
     FB_ALLOC(h->d_theta, sizeof(double) * std::max(h->prog.n_par, 1) * S);
     FB_ALLOC(h->d_hyp, sizeof(double) * hs * S);
+    FB_ALLOC(h->d_chain, sizeof(int) * fb::kChainStride * S);
     FB_ALLOC(h->d_K, sizeof(double) * np2 * S);
     FB_ALLOC(h->d_logdet, sizeof(double) * h->nsteps * S);
     FB_ALLOC(h->d_dinv, sizeof(double) * h->nsteps * S * fb::kDinv);
@@ -614,6 +634,7 @@ int fb_create(fb_handle **out, int n, int d, const double *x, const double *y, c
         h->forced_mode = h->use_dag ? 0 : 1;
     }
     if (const char *env = std::getenv("FB_AUTOTUNE")) h->autotune = std::atoi(env) != 0;
+    if (const char *env = std::getenv("FB_CHAIN")) h->chain = std::atoi(env) != 0;
     if (const char *env = std::getenv("FB_FACTOR_PATH")) {
         const int m = std::atoi(env);
         if (m >= 0 && m < kNumModes) h->forced_mode = m;
@@ -846,6 +867,16 @@ static int choose_path(fb_handle *h, int s)
     return FB_OK;
 }
 
+// INFO kInfoWaitTimeout: a hand-over wait of the two-launch factorisation
+// schedule reached its bound (1 s) instead of hanging.
+static int check_waits(const int *info, int s, const char *where)
+{
+    for (int q = 0; q < s; ++q)
+        if (info[q] == fb::kInfoWaitTimeout)
+            return fail(FB_E_HIP, std::string(where) + ": a wait of the factorisation reached its 1 s bound");
+    return FB_OK;
+}
+
 // One evaluation of the swarm already staged in d_theta; leaves ll / INFO in
 // the pinned host buffers.  *dag_error: the persistent factorisation's error
 // word (0 ok, 1 no ready task for 2 s, 2 a queue was not drained).
@@ -881,7 +912,7 @@ static int run_swarm(fb_handle *h, int s, int *dag_error)
     FB_HIP(hipStreamSynchronize(st));
     FB_HIP(hipGetLastError());
     *dag_error = *h->h_err;
-    return FB_OK;
+    return check_waits(h->h_info, s, "swarm evaluation");
 }
 
 int fb_swarm_loglik(fb_handle *h, int s, const double *theta, double *ll, int *info)
@@ -929,6 +960,7 @@ int fb_particle_loglik(fb_handle *h, int s, const double *theta, double *ll, int
         FB_HIP(hipMemcpyAsync(h->h_info, h->d_info, sizeof(int), hipMemcpyDeviceToHost, st));
         FB_HIP(hipStreamSynchronize(st));
         FB_HIP(hipGetLastError());
+        if (int rc = check_waits(h->h_info, 1, "fb_particle_loglik")) return rc;
         ll[q] = h->h_ll[0];
         info[q] = h->h_info[0];
     }
@@ -946,6 +978,7 @@ int fb_get_alpha(fb_handle *h, const double *theta, double *alpha, double *out4,
     enqueue_group(h, b, st);
     FB_HIP(hipMemcpyAsync(h->h_info, h->d_info, sizeof(int), hipMemcpyDeviceToHost, st));
     FB_HIP(hipStreamSynchronize(st));
+    if (int rc = check_waits(h->h_info, 1, "fb_get_alpha")) return rc;
     *info = h->h_info[0];
     if (*info != 0) return h->profile ? drain_profile(h) : FB_OK;
     fb::launch_backsolve(b, 0, h->d_alpha, st);
@@ -1247,6 +1280,7 @@ int pso_finish(fb_pso *p, double *cost, int *info)
         FB_HIP(hipMemcpyAsync(h->h_err, h->d_ctl + fb::kDagErrorWord, sizeof(int), hipMemcpyDeviceToHost, st));
     FB_HIP(hipStreamSynchronize(st));
     FB_HIP(hipGetLastError());
+    if (int rc = check_waits(p->h_info, S, "PSO")) return rc;
     if (*h->h_err == 1) return fail(FB_E_HIP, "PSO: the persistent factorisation found no ready task for 2 s");
     if (*h->h_err == 2) {
         h->forced_mode = 1;

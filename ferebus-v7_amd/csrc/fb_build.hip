@@ -93,6 +93,7 @@ struct BuildArgs {
     const double *hyp;
     double *K;
     int *info;
+    int *chain;            // words of the chained factorisation schedule, reset here (or none)
     // by value: in the kernel-argument segment every field is read with
     // scalar loads, so the loop bounds, dimension codes and hyperparameter
     // addresses derived from it stay uniform (scalar) in the inner loops
@@ -151,7 +152,13 @@ __global__ void __launch_bounds__(kThreads) build_kernel(BuildArgs a)
         const int k = e / SP, sp = min(s0 + e % SP, a.S - 1);
         hyp_t[e] = a.hyp[(size_t)sp * hs + k];
     }
-    if (b == 0 && threadIdx.x < nsp) a.info[s0 + threadIdx.x] = 0;
+    if (b == 0 && threadIdx.x < nsp) {
+        a.info[s0 + threadIdx.x] = 0;
+        if (a.chain) {
+            int *w = a.chain + (size_t)(s0 + threadIdx.x) * kChainStride;
+            w[kChainF0] = 0; w[kChainF1] = 0; w[kChainXccB] = 0; w[kChainXccC] = 0;
+        }
+    }
     __syncthreads();
 
     const int r = threadIdx.x % TB;
@@ -288,7 +295,7 @@ void launch_build(const SwarmBuffers &b, hipStream_t st)
     hipLaunchKernelGGL(hyper_kernel, dim3(b.S), dim3(64), 0, st, b.prog, b.theta, b.S, b.hyp);
     BuildArgs a;
     a.n = b.n; a.np = b.np; a.nrhs = b.nrhs; a.S = b.S; a.n_slots = b.n_slots;
-    a.slots = b.slots; a.rhs = b.RHS; a.hyp = b.hyp; a.prog = *b.prog_host; a.K = b.K; a.info = b.info;
+    a.slots = b.slots; a.rhs = b.RHS; a.hyp = b.hyp; a.prog = *b.prog_host; a.K = b.K; a.info = b.info; a.chain = b.chain;
     exp_constants(a.expc);
     // 64 x 64 tiles while the staged features fit 64 KB of LDS, else 32 x 32
     const bool wide = build_lds_bytes(64, b.n_slots, 10, a.prog) <= 65536;

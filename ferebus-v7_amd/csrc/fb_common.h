@@ -103,6 +103,19 @@ constexpr int kKC = 32;       // depth of one LDS stage of the update GEMM
 constexpr int kThreads = 256; // work-group size of every kernel (4 wave64)
 constexpr int kDinv = 4 * 16 * 16;   // doubles per 64x64 diagonal block: four 16x16 inverses
 
+// Per-particle words of the chained lock-step schedule (two launches per
+// panel; fb_potrf.hip): flags and XCD announcements, each in its own 32-byte
+// sector of a 128-byte line.
+constexpr int kChainStride = 32;
+constexpr int kChainF0 = 0, kChainF1 = 8, kChainXccB = 16, kChainXccC = 24;
+// Swarms up to this size: the waiting work-groups of the update launch (two
+// per particle) hold fewer than its 512 resident slots (two per CU), so every
+// producer they wait for finds a slot.
+constexpr int kChainMaxSwarm = 160;
+// INFO of a particle whose factorisation stopped at a bounded wait (1 s)
+// instead of hanging; the host reports an error for it.
+constexpr int kInfoWaitTimeout = -9;
+
 inline __host__ __device__ int ceil_div(int a, int b) { return (a + b - 1) / b; }
 inline __host__ __device__ int round_up(int a, int b) { return ceil_div(a, b) * b; }
 
@@ -127,12 +140,15 @@ struct SwarmBuffers {
     double *ll;            // S
     double *extra;         // S x 4: ll, logdet, mu_hat, sigma2_hat
     double *zbuf;          // S x n: z = L^{-1} rhs_0 (or the concentrated residual)
+    int *chain = nullptr;  // S x kChainStride: chained schedule (two launches per panel), or none
 };
 
 void launch_build(const SwarmBuffers &b, hipStream_t st);
 void launch_panel_update_factor(const SwarmBuffers &b, int c0, hipStream_t st);
 void launch_trsm_factor(const SwarmBuffers &b, int c0, hipStream_t st);
 void launch_update_trsm(const SwarmBuffers &b, int c1, hipStream_t st);
+void launch_panel_chain(const SwarmBuffers &b, int c0, hipStream_t st);
+void launch_solve_rows(const SwarmBuffers &b, int c0, hipStream_t st);
 void launch_likelihood(const SwarmBuffers &b, hipStream_t st);
 
 // Adjoint gradient (fb_grad.hip).

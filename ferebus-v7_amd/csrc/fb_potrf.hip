@@ -136,6 +136,222 @@ __global__ void __launch_bounds__(kThreads) update_trsm_kernel(double *__restric
                         dinv + ((size_t)s * (np / kNBI) + c1 / kNBI) * kDinv);
 }
 
+// ---- chained lock-step schedule (two launches per 128-column panel) ------
+//
+// The diagonal chain of a panel -- factor (b0, b0), solve (b1, b0) with it,
+// apply that to (b1, b1) and factor it -- runs inside the update launch, on
+// three role work-groups per particle whose updates proceed in parallel:
+//   D updates (b0, b0) on chip and factors it;
+//   B updates (b1, b0) on chip, waits for D, solves the tile with L(b0, b0);
+//   C updates (b1, b1) on chip, waits for B, applies L(b1, b0) L(b1, b0)^T
+//     and factors the block.
+// The remaining row tiles of the panel are then solved in one row-local
+// launch (solve_rows_kernel: solve with L(b0, b0), the 64-deep update of
+// their b1 part, solve with L(b1, b1)).  Same tile routines, same order: the
+// bits of the three-launch schedule.
+//
+// Hand-over.  A producer publishes a flag word (the panel's tag, increasing
+// within an evaluation) after its stores have reached its XCD's L2; the
+// consumer announces at its start which XCD it runs on (tagged), and the
+// producer writes its L2 back to memory before publishing only when the
+// consumer is elsewhere or not yet known.  The role work-groups of particle
+// s get block ids congruent to s modulo 8, which the dispatcher places on one
+// XCD, so the write-back is normally skipped.  The consumer polls the flag
+// with device-scope atomics (they go to memory), then drops its CU's L1 and
+// the clean non-coherent L2 lines before reading.  Role work-groups come
+// first in the grid; at most two per particle wait, fewer than the launch's
+// resident slots (kChainMaxSwarm), and D never waits, so every wait ends; it
+// is still bounded at 1 s of the 100 MHz clock (INFO kInfoWaitTimeout).
+
+__device__ inline int chain_xcc()
+{
+    // hwreg(HW_REG_XCC_ID, 0, 4): id 20, offset 0, 4 bits
+    return (int)(__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 0xf);
+}
+
+// Consumer, at its start: tag and XCD.
+__device__ inline void chain_announce(int *w, int tag)
+{
+    if (threadIdx.x == 0) __hip_atomic_store(w, (tag << 4) | chain_xcc(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Producer: every thread's stores are done; publish `tag` at *flag, writing
+// the L2 back first unless the consumer announced this XCD for this tag.
+__device__ inline void chain_publish(int *flag, const int *consumer, int tag)
+{
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (__hip_atomic_load(consumer, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != ((tag << 4) | chain_xcc()))
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_store(flag, tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// Consumer: wait until *flag >= tag (thread 0 polls, bounded at 1 s), then
+// drop this CU's L1 and the clean non-coherent L2 lines.  False on timeout.
+__device__ inline bool chain_wait(const int *flag, int tag, int *ok)
+{
+    if (threadIdx.x == 0) {
+        const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+        int good = 1;
+        while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < tag) {
+            __builtin_amdgcn_s_sleep(2);
+            if (__builtin_amdgcn_s_memrealtime() - t0 > 100000000ull) {
+                good = 0;
+                break;
+            }
+        }
+        asm volatile("buffer_inv sc1\n\ts_waitcnt vmcnt(0)" ::: "memory");
+        *ok = good;
+    }
+    __syncthreads();
+    return *ok != 0;
+}
+
+template <int KC, int MB, int NB>
+constexpr int chain_smem()
+{
+    constexpr int a = panel_smem<KC, MB, NB>(), c = kPotrfSmem + kNBI * kPLD;
+    return a > c ? a : c;
+}
+
+constexpr int kChainRoles = 24;   // role work-groups per 8 particles (D, B, C)
+
+// Launch 1 of panel c0, chained: role work-groups first (block 24 g + 8 r + l
+// is role r of particle 8 g + l), then the MB x NB update tiles of the rows
+// below the panel.
+template <int KC, int MB, int NB>
+__global__ void __launch_bounds__(kThreads)
+__attribute__((amdgpu_waves_per_eu(KC == 16 ? 2 : 1, KC == 16 ? 2 : 1))) panel_chain_kernel(double *__restrict__ K, int np, int n,
+                                                                       int c0, int S, int *__restrict__ info,
+                                                                       double *__restrict__ logdet_part,
+                                                                       double *__restrict__ dinv,
+                                                                       int *__restrict__ chain)
+{
+    __shared__ __attribute__((aligned(16))) double smem[chain_smem<KC, MB, NB>()];
+    __shared__ int ok;
+    const int b = blockIdx.x;
+    const int nrole = kChainRoles * ceil_div(S, 8);
+    if (b < nrole) {
+        const int role = (b % kChainRoles) / 8;
+        const int s = 8 * (b / kChainRoles) + b % 8;
+        if (s >= S) return;
+        const int tag = c0 / kNBO + 1, c1 = c0 + kNBI;
+        const size_t nbk = (size_t)np / kNBI;
+        double *A = K + (size_t)s * np * np;
+        int *cw = chain + (size_t)s * kChainStride;
+        double *dinv0 = dinv + ((size_t)s * nbk + c0 / kNBI) * kDinv;
+        if (role == 0) {
+            if (info[s] == 0) {
+                gemm_tile_update<kMB, kNBI, KC, 2, true>(A, np, c0, c0, 0, c0, smem);
+                __syncthreads();
+                double lg = 0.0;
+                const int fail = potrf64_block<true>(A, np, n, c0, &lg, smem, dinv0);
+                if (threadIdx.x == 0) {
+                    if (fail) info[s] = fail;
+                    else logdet_part[(size_t)s * nbk + c0 / kNBI] = lg;
+                }
+            }
+            chain_publish(cw + kChainF0, cw + kChainXccB, tag);
+        } else if (role == 1) {
+            chain_announce(cw + kChainXccB, tag);
+            gemm_tile_update<kMB, kNBI, KC, 2, true>(A, np, c1, c0, 0, c0, smem);
+            if (!chain_wait(cw + kChainF0, tag, &ok)) {
+                if (threadIdx.x == 0) info[s] = kInfoWaitTimeout;
+            } else if (info[s] == 0) {
+                trsm64_solve<false>(smem, kPLD, A + (size_t)c0 * np + c1, np, A + (size_t)c0 * np + c0, np, dinv0);
+            }
+            chain_publish(cw + kChainF1, cw + kChainXccC, tag);
+        } else {
+            chain_announce(cw + kChainXccC, tag);
+            gemm_tile_update<kMB, kNBI, KC, 2, true>(A, np, c1, c1, 0, c0, smem);
+            if (!chain_wait(cw + kChainF1, tag, &ok)) {
+                if (threadIdx.x == 0) info[s] = kInfoWaitTimeout;
+                return;
+            }
+            if (info[s] != 0) return;
+            double *Xs = smem + kPotrfSmem;          // L(b1, b0)
+            const double *Xg = A + (size_t)c0 * np + c1;
+            for (int e = threadIdx.x; e < kNBI * kNBI / 2; e += kThreads) {
+                const int col = e / (kNBI / 2), r2 = (e % (kNBI / 2)) * 2;
+                *reinterpret_cast<double2 *>(Xs + col * kPLD + r2) = *reinterpret_cast<const double2 *>(Xg + (size_t)col * np + r2);
+            }
+            __syncthreads();
+            syrk64_lds(smem, Xs);
+            __syncthreads();
+            double lg = 0.0;
+            const int fail = potrf64_block<true>(A, np, n, c1, &lg, smem,
+                                                 dinv + ((size_t)s * nbk + c1 / kNBI) * kDinv);
+            if (threadIdx.x == 0) {
+                if (fail) info[s] = fail;
+                else logdet_part[(size_t)s * nbk + c1 / kNBI] = lg;
+            }
+        }
+        return;
+    }
+    constexpr int CT = kNBO / NB;            // column tiles of the panel
+    const int rt = (np - c0 - kNBO) / MB;    // row tiles below the panel
+    const int t = b - nrole;
+    const int s = t / (rt * CT), u = t % (rt * CT);
+    if (info[s] != 0) return;
+    gemm_tile_update<MB, NB, KC>(K + (size_t)s * np * np, np, c0 + kNBO + (u / CT) * MB, c0 + (u % CT) * NB, 0, c0,
+                                 smem);
+}
+
+// Launch 2 of panel c0, chained: every row tile below the panel is solved
+// with L(b0, b0), takes the 64-deep update of its b1 part with L(b1, b0) and
+// is solved with L(b1, b1) (launches 2 and 3 of the three-launch schedule
+// for one tile).  The tile's own b0 result is re-read after the barrier:
+// global stores of a work-group are visible to it.
+template <int KC>
+__global__ void __launch_bounds__(kThreads) solve_rows_kernel(double *__restrict__ K, int np, int c0,
+                                                              const int *__restrict__ info,
+                                                              const double *__restrict__ dinv)
+{
+    __shared__ __attribute__((aligned(16))) double smem[TileShape<kMB, kNBI, KC>::SMEM];
+    const int rt = (np - c0 - kNBO) / kNBI;
+    const int s = blockIdx.x / rt;
+    const int row0 = c0 + kNBO + (blockIdx.x % rt) * kNBI;
+    if (info[s] != 0) return;
+    double *A = K + (size_t)s * np * np;
+    const int c1 = c0 + kNBI;
+    const size_t nbk = (size_t)np / kNBI;
+    double *tile = A + (size_t)c0 * np + row0;
+    trsm64_solve<false>(tile, np, tile, np, A + (size_t)c0 * np + c0, np, dinv + ((size_t)s * nbk + c0 / kNBI) * kDinv);
+    __syncthreads();
+    gemm_tile_update<kMB, kNBI, KC, 2, true>(A, np, row0, c1, c0, c1, smem);
+    __syncthreads();
+    trsm64_solve<false>(smem, kPLD, A + (size_t)c1 * np + row0, np, A + (size_t)c1 * np + c1, np,
+                        dinv + ((size_t)s * nbk + c1 / kNBI) * kDinv);
+}
+
+template <int KC, int MB, int NB>
+static void chain_launch(const SwarmBuffers &b, int c0, hipStream_t st)
+{
+    const int tiles = ((b.np - c0 - kNBO) / MB) * (kNBO / NB);
+    const int nrole = kChainRoles * ceil_div(b.S, 8);
+    hipLaunchKernelGGL((panel_chain_kernel<KC, MB, NB>), dim3(nrole + b.S * tiles), dim3(kThreads), 0, st, b.K,
+                       b.np, b.n, c0, b.S, b.info, b.logdet_part, b.dinv, b.chain);
+}
+
+// 16-deep chunks only (the 32-deep update tile leaves no room for the roles).
+void launch_panel_chain(const SwarmBuffers &b, int c0, hipStream_t st)
+{
+    const int tile = b.panel_tile ? b.panel_tile[c0 / kNBO] : b.tile;
+    if (tile == 1) chain_launch<16, 32, 128>(b, c0, st);
+    else if (tile == 2) chain_launch<16, 32, 64>(b, c0, st);
+    else chain_launch<16, 64, 128>(b, c0, st);
+}
+
+void launch_solve_rows(const SwarmBuffers &b, int c0, hipStream_t st)
+{
+    const int rt = (b.np - c0 - kNBO) / kNBI;
+    if (rt <= 0) return;
+    hipLaunchKernelGGL((solve_rows_kernel<16>), dim3(b.S * rt), dim3(kThreads), 0, st, b.K, b.np, c0, b.info,
+                       b.dinv);
+}
+
 template <int KC, int MB, int NB>
 static void panel_launch(const SwarmBuffers &b, int c0, hipStream_t st)
 {

@@ -101,3 +101,66 @@ def test_tuning_cache_file(gpu, monkeypatch, tmp_path):
     assert gp2.factor_path(6) == chosen          # known before any evaluation
     ll2, info2 = gp2.swarm_loglik(th)
     assert np.array_equal(ll, ll2) and np.array_equal(info, info2)
+
+
+def _dup_case(n, dup, src, seed):
+    """rbf on 6 features with the nugget in theta; point `dup` repeats point
+    `src`, so a negative nugget makes column dup + 1 fail (INFO = dup + 1)
+    and a positive one leaves K positive definite."""
+    from ferebus_amd import KernelConfig
+    rng = np.random.default_rng(seed)
+    x = synthetic.alf_features(rng, n, 6)
+    x[dup] = x[src]
+    y = synthetic.bump_target(rng, x, nbumps=4)
+    prog = compile_kernel("k1", {"k1": KernelConfig("k1", "rbf")}, 6)
+    return x, y, prog, fit_mean("constant", x, y)
+
+
+@pytest.mark.parametrize("n,dup,src,S", [
+    (700, 200, 5, 9),     # fails in the second 64-block of panel 1 (columns 192..255)
+    (700, 300, 17, 9),    # fails in the first 64-block of panel 2 (columns 256..319)
+    (383, 130, 2, 1),     # single particle, bordered rows in the last panel
+])
+def test_chained_schedule_bit_identical(gpu, monkeypatch, n, dup, src, S):
+    """Two launches per panel (the diagonal chain handed over inside the
+    update launch) against three: same bits, same INFO, the failing
+    particles included."""
+    x, y, prog, mfit = _dup_case(n, dup, src, seed=n + dup)
+    out = {}
+    for chain in (0, 1):
+        monkeypatch.setenv("FB_CHAIN", str(chain))
+        monkeypatch.setenv("FB_FACTOR_PATH", "2")
+        gp = SwarmLikelihood(x, y, prog, mean=mfit, noise=1e-6, optimise_noise=True, max_swarm=S)
+        th = np.random.default_rng(3).uniform(0.0, 1.0, (S, gp.nparams))
+        th[:, -1] = np.where(np.arange(S) % 3 == 1, -1e-8, 1e-4)
+        ll, info = gp.swarm_loglik(th)
+        ll2, info2 = gp.swarm_loglik(th)          # graph replay: the flags start over
+        assert np.array_equal(ll, ll2, equal_nan=True) and np.array_equal(info, info2)
+        out[chain] = (ll, info, th)
+    print("INFO three launches", out[0][1].tolist(), "two launches", out[1][1].tolist())
+    assert np.array_equal(out[0][1], out[1][1])
+    ok = out[0][1] == 0
+    assert np.array_equal(out[0][0][ok], out[1][0][ok])
+    expect = np.where(np.arange(S) % 3 == 1, dup + 1, 0)
+    assert np.array_equal(out[1][1], expect)
+    th = out[1][2]
+    oll, oinfo = _oracle(x, y, mfit.value, prog, th, 1e-6, True, "marginal")
+    assert np.array_equal(oinfo, expect)
+    _check(out[1][0], out[1][1], oll, oinfo, 1e-8, f"chained n={n}")
+
+
+@pytest.mark.parametrize("path", [3, 4, 5])
+def test_chained_schedule_tile_shapes(gpu, monkeypatch, path):
+    """The chained update launch with the 32x128 and 32x64 row tiles and the
+    per-panel mix: the bits of the three-launch schedule."""
+    _, x, y, prog, mfit = _problem(900, 6, seed=path)
+    out = {}
+    for chain in (0, 1):
+        monkeypatch.setenv("FB_CHAIN", str(chain))
+        monkeypatch.setenv("FB_FACTOR_PATH", str(path))
+        if path == 5:
+            monkeypatch.setenv("FB_PANEL_TILES", "2,0,1,2,0,1,2,0")
+        gp = _gp(x, y, prog, mfit, 7)
+        th = np.random.default_rng(4).uniform(0.0, 1.0, (7, gp.nparams))
+        out[chain] = gp.swarm_loglik(th)
+    assert np.array_equal(out[0][1], out[1][1]) and np.array_equal(out[0][0], out[1][0])
