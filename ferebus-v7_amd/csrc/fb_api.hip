@@ -639,7 +639,8 @@ int fb_create(fb_handle **out, int n, int d, const double *x, const double *y, c
         const int m = std::atoi(env);
         if (m >= 0 && m < kNumModes) h->forced_mode = m;
         // the per-panel path with its shapes given: FB_PANEL_TILES=t0,t1,...
-        // (one of 0 64x128, 1 32x128, 2 32x64 per 128-column panel)
+        // (one of 0 64x128, 1 32x128, 2 32x64 per 128-column panel, + 4 for
+        // two role work-groups in the chained update launch)
         const char *tl = std::getenv("FB_PANEL_TILES");
         if (m == kModePerPanel && tl) {
             std::vector<signed char> v;
@@ -649,7 +650,7 @@ int fb_create(fb_handle **out, int n, int d, const double *x, const double *y, c
                 if (*c == ',') ++c;
             }
             bool ok = (int)v.size() == h->np / fb::kNBO;
-            for (signed char t : v) ok = ok && t >= 0 && t <= 2;
+            for (signed char t : v) ok = ok && t >= 0 && t <= 6 && t != 3;
             if (ok) {
                 h->pinned_tiles = v;
                 h->forced_mode = kModePerPanel;
@@ -673,7 +674,7 @@ int fb_create(fb_handle **out, int n, int d, const double *x, const double *y, c
                         if (*c == ',') ++c;
                     }
                     bool ok = (int)v.size() == h->np / fb::kNBO;
-                    for (signed char t : v) ok = ok && t >= 0 && t <= 2;
+                    for (signed char t : v) ok = ok && t >= 0 && t <= 6 && t != 3;
                     if (!ok) continue;
                     h->panel_tiles[r.swarm] = v;
                 }
@@ -815,13 +816,19 @@ static int choose_path(fb_handle *h, int s)
         }
         if (ms_best < best) { best = ms_best; best_mode = mode; }
     }
-    // the panel updates of the three 16-deep shapes, timed panel by panel: a
-    // mix of the fastest shape of each panel is one more candidate
+    // the panel updates of the three 16-deep shapes (chained: each with three
+    // or two role work-groups), timed panel by panel: a mix of the fastest
+    // variant of each panel is one more candidate
     const int npan = h->np / fb::kNBO;
-    std::vector<std::vector<float>> tpan(3, std::vector<float>(npan, 1e30f));
+    std::vector<int> codes = {0, 1, 2};
+    if (h->chain && s <= fb::kChainMaxSwarm)
+        for (int t = 0; t < 3; ++t) codes.push_back(t | fb::kTileTwoRoles);
+    const int nc = (int)codes.size();
+    std::vector<std::vector<float>> tpan(nc, std::vector<float>(npan, 1e30f));
     std::vector<std::pair<hipEvent_t, hipEvent_t>> evs;
-    for (int t = 0; t < 3; ++t) {
-        set_mode(h, 2 + t, s);
+    for (int t = 0; t < nc; ++t) {
+        set_mode(h, 2 + (codes[t] & 3), s);
+        h->tile = codes[t];
         evs.clear();
         h->k1_events = &evs;
         enqueue_pipeline(h, s, h->d_theta, h->d_ll, h->d_info, st);
@@ -838,11 +845,11 @@ static int choose_path(fb_handle *h, int s)
     bool mixed = false;
     for (int k = 0; k < npan; ++k) {
         int bt = 0;
-        for (int t = 1; t < 3; ++t) if (tpan[t][k] < tpan[bt][k]) bt = t;
-        mix[k] = (signed char)bt;
-        mixed = mixed || bt != mix[0];
+        for (int t = 1; t < nc; ++t) if (tpan[t][k] < tpan[bt][k]) bt = t;
+        mix[k] = (signed char)codes[bt];
+        mixed = mixed || mix[k] != mix[0];
     }
-    if (mixed) {
+    if (mixed || (mix[0] & fb::kTileTwoRoles)) {
         h->panel_tiles[s] = mix;
         set_mode(h, kModePerPanel, s);
         float ms_best = 1e30f;

@@ -115,6 +115,9 @@ constexpr int kChainMaxSwarm = 160;
 // INFO of a particle whose factorisation stopped at a bounded wait (1 s)
 // instead of hanging; the host reports an error for it.
 constexpr int kInfoWaitTimeout = -9;
+// Panel tile code bit: the chained update launch with two role work-groups
+// per particle instead of three.
+constexpr int kTileTwoRoles = 4;
 
 inline __host__ __device__ int ceil_div(int a, int b) { return (a + b - 1) / b; }
 inline __host__ __device__ int round_up(int a, int b) { return ceil_div(a, b) * b; }
@@ -131,7 +134,8 @@ struct SwarmBuffers {
     const KernelProgram *prog;   // device copy
     const KernelProgram *prog_host;   // host copy (passed by value to kernels whose loops read it)
     int gemm_kc = 32;      // chunk depth of the lock-step trailing update (16 or 32)
-    int tile = 0;          // lock-step panel-update tiles: 0 64x128, 1 32x128, 2 32x64 (16-deep chunks)
+    int tile = 0;          // lock-step panel-update tiles: 0 64x128, 1 32x128, 2 32x64 (16-deep chunks),
+                           // | kTileTwoRoles: two role work-groups in the chained update launch
     const signed char *panel_tile = nullptr;   // host: per-panel tile shape (overrides `tile`), or none
     double *K;             // S x np x np
     int *info;             // S

@@ -216,12 +216,17 @@ constexpr int chain_smem()
     return a > c ? a : c;
 }
 
-constexpr int kChainRoles = 24;   // role work-groups per 8 particles (D, B, C)
+// Role work-groups per 8 particles: D, B, C (TWO = false), or D and BC
+// (TWO = true: one work-group updates the whole 64 x 128 row tile of b1,
+// then waits for D, solves (b1, b0), applies it to (b1, b1) and factors it
+// -- fewer work-groups holding slots, a longer chain after the update).
+template <bool TWO>
+constexpr int chain_roles() { return TWO ? 16 : 24; }
 
-// Launch 1 of panel c0, chained: role work-groups first (block 24 g + 8 r + l
-// is role r of particle 8 g + l), then the MB x NB update tiles of the rows
-// below the panel.
-template <int KC, int MB, int NB>
+// Launch 1 of panel c0, chained: role work-groups first (block R g + 8 r + l
+// is role r of particle 8 g + l, R = chain_roles), then the MB x NB update
+// tiles of the rows below the panel.
+template <int KC, int MB, int NB, bool TWO>
 __global__ void __launch_bounds__(kThreads)
 __attribute__((amdgpu_waves_per_eu(KC == 16 ? 2 : 1, KC == 16 ? 2 : 1))) panel_chain_kernel(double *__restrict__ K, int np, int n,
                                                                        int c0, int S, int *__restrict__ info,
@@ -232,10 +237,11 @@ __attribute__((amdgpu_waves_per_eu(KC == 16 ? 2 : 1, KC == 16 ? 2 : 1))) panel_c
     __shared__ __attribute__((aligned(16))) double smem[chain_smem<KC, MB, NB>()];
     __shared__ int ok;
     const int b = blockIdx.x;
-    const int nrole = kChainRoles * ceil_div(S, 8);
+    constexpr int R = chain_roles<TWO>();
+    const int nrole = R * ceil_div(S, 8);
     if (b < nrole) {
-        const int role = (b % kChainRoles) / 8;
-        const int s = 8 * (b / kChainRoles) + b % 8;
+        const int role = (b % R) / 8;
+        const int s = 8 * (b / R) + b % 8;
         if (s >= S) return;
         const int tag = c0 / kNBO + 1, c1 = c0 + kNBI;
         const size_t nbk = (size_t)np / kNBI;
@@ -254,6 +260,34 @@ __attribute__((amdgpu_waves_per_eu(KC == 16 ? 2 : 1, KC == 16 ? 2 : 1))) panel_c
                 }
             }
             chain_publish(cw + kChainF0, cw + kChainXccB, tag);
+        } else if (TWO) {
+            // BC: the b1 row tile (b1, b0:b1), then the rest of the chain
+            chain_announce(cw + kChainXccB, tag);
+            gemm_tile_update<kMB, kNBO, KC>(A, np, c1, c0, 0, c0, smem);
+            if (!chain_wait(cw + kChainF0, tag, &ok)) {
+                if (threadIdx.x == 0) info[s] = kInfoWaitTimeout;
+                return;
+            }
+            if (info[s] != 0) return;
+            double *Xs = smem;                       // L(b1, b0) on chip
+            double *Ls = smem + kNBI * kPLD;         // block (b1, b1), then its factorisation
+            double *tile = A + (size_t)c0 * np + c1;
+            trsm64_solve<false, true>(tile, np, tile, np, A + (size_t)c0 * np + c0, np, dinv0, Xs);
+            const double *Cg = A + (size_t)c1 * np + c1;
+            for (int e = threadIdx.x; e < kNBI * kNBI / 2; e += kThreads) {
+                const int col = e / (kNBI / 2), r2 = (e % (kNBI / 2)) * 2;
+                *reinterpret_cast<double2 *>(Ls + col * kPLD + r2) = *reinterpret_cast<const double2 *>(Cg + (size_t)col * np + r2);
+            }
+            __syncthreads();
+            syrk64_lds(Ls, Xs);
+            __syncthreads();
+            double lg = 0.0;
+            const int fail = potrf64_block<true>(A, np, n, c1, &lg, Ls,
+                                                 dinv + ((size_t)s * nbk + c1 / kNBI) * kDinv);
+            if (threadIdx.x == 0) {
+                if (fail) info[s] = fail;
+                else logdet_part[(size_t)s * nbk + c1 / kNBI] = lg;
+            }
         } else if (role == 1) {
             chain_announce(cw + kChainXccB, tag);
             gemm_tile_update<kMB, kNBI, KC, 2, true>(A, np, c1, c0, 0, c0, smem);
@@ -326,22 +360,31 @@ __global__ void __launch_bounds__(kThreads) solve_rows_kernel(double *__restrict
                         dinv + ((size_t)s * nbk + c1 / kNBI) * kDinv);
 }
 
-template <int KC, int MB, int NB>
+template <int KC, int MB, int NB, bool TWO>
 static void chain_launch(const SwarmBuffers &b, int c0, hipStream_t st)
 {
     const int tiles = ((b.np - c0 - kNBO) / MB) * (kNBO / NB);
-    const int nrole = kChainRoles * ceil_div(b.S, 8);
-    hipLaunchKernelGGL((panel_chain_kernel<KC, MB, NB>), dim3(nrole + b.S * tiles), dim3(kThreads), 0, st, b.K,
-                       b.np, b.n, c0, b.S, b.info, b.logdet_part, b.dinv, b.chain);
+    const int nrole = chain_roles<TWO>() * ceil_div(b.S, 8);
+    hipLaunchKernelGGL((panel_chain_kernel<KC, MB, NB, TWO>), dim3(nrole + b.S * tiles), dim3(kThreads), 0, st,
+                       b.K, b.np, b.n, c0, b.S, b.info, b.logdet_part, b.dinv, b.chain);
+}
+
+template <bool TWO>
+static void chain_tiles(const SwarmBuffers &b, int tile, int c0, hipStream_t st)
+{
+    if (tile == 1) chain_launch<16, 32, 128, TWO>(b, c0, st);
+    else if (tile == 2) chain_launch<16, 32, 64, TWO>(b, c0, st);
+    else chain_launch<16, 64, 128, TWO>(b, c0, st);
 }
 
 // 16-deep chunks only (the 32-deep update tile leaves no room for the roles).
+// Tile code: 0 64x128, 1 32x128, 2 32x64 row tiles, plus kTileTwoRoles for
+// the two-role variant.
 void launch_panel_chain(const SwarmBuffers &b, int c0, hipStream_t st)
 {
-    const int tile = b.panel_tile ? b.panel_tile[c0 / kNBO] : b.tile;
-    if (tile == 1) chain_launch<16, 32, 128>(b, c0, st);
-    else if (tile == 2) chain_launch<16, 32, 64>(b, c0, st);
-    else chain_launch<16, 64, 128>(b, c0, st);
+    const int code = b.panel_tile ? b.panel_tile[c0 / kNBO] : b.tile;
+    if (code & kTileTwoRoles) chain_tiles<true>(b, code & 3, c0, st);
+    else chain_tiles<false>(b, code & 3, c0, st);
 }
 
 void launch_solve_rows(const SwarmBuffers &b, int c0, hipStream_t st)

@@ -3,10 +3,13 @@
 The batched evaluation can factorise the swarm with one persistent launch
 (path 0) or with lock-step launches whose trailing-update GEMM stages 32-deep
 (path 1) or 16-deep chunks (paths 2-4, with 64x128, 32x128 or 32x64 update
-tiles).  All of them run the same tile routines in the same per-element
-accumulation order, so their results must be bit-identical; the autotuner
-(first call per swarm size) may then pick any of them.  Each path is also
-held to the oracle.
+tiles; path 5 mixes them panel by panel).  With 16-deep chunks the lock-step
+schedule is chained by default (two launches per panel, the diagonal chain
+handed over between work-groups inside the update launch; FB_CHAIN=0 gives
+three launches).  All of them run the same tile routines in the same
+per-element accumulation order, so their results must be bit-identical; the
+autotuner (first call per swarm size) may then pick any of them.  Each path
+is also held to the oracle.
 """
 import numpy as np
 import pytest
@@ -149,17 +152,19 @@ def test_chained_schedule_bit_identical(gpu, monkeypatch, n, dup, src, S):
     _check(out[1][0], out[1][1], oll, oinfo, 1e-8, f"chained n={n}")
 
 
-@pytest.mark.parametrize("path", [3, 4, 5])
-def test_chained_schedule_tile_shapes(gpu, monkeypatch, path):
-    """The chained update launch with the 32x128 and 32x64 row tiles and the
-    per-panel mix: the bits of the three-launch schedule."""
+@pytest.mark.parametrize("path,tiles", [(3, None), (4, None), (5, "2,0,1,2,0,1,2,0"), (5, "6,4,5,2,4,1,6,5")])
+def test_chained_schedule_tile_shapes(gpu, monkeypatch, path, tiles):
+    """The chained update launch with the 32x128 and 32x64 row tiles, the
+    per-panel mix and the two-role variant (tile code + 4): the bits of the
+    three-launch schedule."""
     _, x, y, prog, mfit = _problem(900, 6, seed=path)
     out = {}
     for chain in (0, 1):
         monkeypatch.setenv("FB_CHAIN", str(chain))
         monkeypatch.setenv("FB_FACTOR_PATH", str(path))
-        if path == 5:
-            monkeypatch.setenv("FB_PANEL_TILES", "2,0,1,2,0,1,2,0")
+        if tiles:
+            # the three-launch schedule has no role work-groups: same shapes
+            monkeypatch.setenv("FB_PANEL_TILES", tiles if chain else ",".join(str(int(t) & 3) for t in tiles.split(",")))
         gp = _gp(x, y, prog, mfit, 7)
         th = np.random.default_rng(4).uniform(0.0, 1.0, (7, gp.nparams))
         out[chain] = gp.swarm_loglik(th)
