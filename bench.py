@@ -30,6 +30,12 @@ sys.path.insert(0, str(REPO))
 
 
 # committed rocprofv3 summaries of this workload, per factorisation path
+# (lock-step paths: the three-launch schedule, FB_CHAIN=0; CHAINED_PROFILES:
+# the default two-launch schedule)
+CHAINED_PROFILES = {
+    "lockstep-kc16-per-panel": (REPO / "profiles" / "chained_factor_per_panel_methanol_n2000_s50.md",
+                                "lockstep_factor"),
+}
 PROFILES = {
     "persistent": (REPO / "profiles" / "dag_factor_kernel_methanol_n2000_s50.md", "dag_factor_kernel"),
     "lockstep-kc16": (REPO / "profiles" / "lockstep_factor_methanol_n2000_s50.md", "lockstep_factor"),
@@ -40,13 +46,14 @@ PROFILES = {
 }
 
 
-def profiled_traffic(path: str):
+def profiled_traffic(path: str, chained: bool = False):
     """(HBM bytes per factorisation, source) from the committed rocprofv3
     counter summary of that path (FETCH_SIZE + WRITE_SIZE, copy-kernel
     calibrated), or (None, None)."""
-    if path not in PROFILES:
+    table = CHAINED_PROFILES if chained else PROFILES
+    if path not in table:
         return None, None
-    f, key = PROFILES[path]
+    f, key = table[path]
     try:
         for line in f.read_text().splitlines():
             if line.startswith(f"traffic_bytes_per_launch {key}:"):
@@ -230,11 +237,19 @@ def main():
         fact_kernel = "dag_factor_kernel (whole batched Cholesky of the swarm, one persistent launch)"
     else:
         fact_ms = (prof["update"][0] + prof["solve_factor"][0] + prof["update_solve"][0]) / prof_steps
+    # chained lock-step schedule: no third launch per panel
+    chained = not persistent and prof["update_solve"][1] == 0
+    if chained:
+        fact_kernel = ("chained lock-step batched Cholesky: per 128-column panel panel_chain_kernel (update, "
+                       "diagonal chain handed over between work-groups) and solve_rows_kernel "
+                       "(31 launches per evaluation, durations summed)")
+    elif not persistent:
         fact_kernel = ("lock-step batched Cholesky: per 128-column panel panel_update_factor_kernel, "
                        "trsm_factor_kernel, update_trsm_kernel (47 launches per evaluation, durations summed)")
     fact_flops = S * potrf_flops(n)
     factor_path = gp.factor_path(S)
-    traffic, traffic_src = profiled_traffic(factor_path) if (n == 2000 and S == 50) else (None, None)
+    traffic, traffic_src = (profiled_traffic(factor_path, chained) if (n == 2000 and S == 50)
+                            else (None, None))
     build_ms = prof["build"][0] / prof_steps
     build_bytes = S * 8.0 * n * (n + 1) / 2.0 + 8.0 * n * d
     achieved = fact_flops / (fact_ms * 1e-3) / 1e12
@@ -257,6 +272,8 @@ def main():
                    "path": "batched (hipGraph replay per swarm evaluation, "
                            + ("persistent factorisation)" if persistent else "lock-step factorisation)"),
                    "factor_path": factor_path,
+                   "schedule": "persistent" if persistent else ("chained (2 launches per panel)" if chained
+                                                               else "3 launches per panel"),
                    "failed_particles_last_step": fails},
         "roofline": {
             "kernel": fact_kernel,

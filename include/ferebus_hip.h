@@ -222,10 +222,16 @@ int fb_profile_read(fb_handle *h, int kind, double *total_ms, long long *launche
  * launch; lock-step launches with 1: 32-deep GEMM chunks and 64x128 update
  * tiles, 2: 16-deep chunks, 64x128 tiles, 3: 16-deep, 32x128, 4: 16-deep,
  * 32x64, 5: 16-deep with each panel's fastest of those three tile shapes
- * (timed panel by panel).  Environment, read at fb_create:
+ * (timed panel by panel).  With 16-deep chunks and swarms up to 160 the
+ * lock-step schedule is chained: two launches per 128-column panel, the
+ * panel's diagonal factorisation handed over between work-groups inside the
+ * update launch (path 5 then also times each shape with two or three such
+ * work-groups per particle).  Environment, read at fb_create:
  * FB_FACTOR_PATH=0..4 pins a path; FB_FACTOR_PATH=5 with
- * FB_PANEL_TILES=t0,t1,... (one shape per 128-column panel, 0..2 as in
- * paths 2..4) pins a per-panel mix; FB_PERSISTENT=0/1 pins path 1/0;
+ * FB_PANEL_TILES=t0,t1,... (one code per 128-column panel: 0..2 the shapes
+ * of paths 2..4, 4..6 the same with two work-groups per particle in the
+ * chained update launch) pins a per-panel mix; FB_CHAIN=0 gives three
+ * launches per panel instead; FB_PERSISTENT=0/1 pins path 1/0;
  * FB_AUTOTUNE=0 keeps path 0 without timing; FB_TUNE_CACHE=<file> keeps the
  * choices across processes (JSON records keyed on device, padded order and
  * swarm size, with the per-panel shapes). */
