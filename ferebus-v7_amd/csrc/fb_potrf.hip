@@ -223,9 +223,21 @@ constexpr int chain_smem()
 template <bool TWO>
 constexpr int chain_roles() { return TWO ? 16 : 24; }
 
+// XCD-aware order of T tiles numbered particle-major: the dispatcher places
+// block b on XCD b mod 8, so block base + t (base a multiple of 8) takes
+// tile (t mod 8) * ceil(T / 8) + t / 8 -- each XCD gets one contiguous
+// eighth of the tiles, and a particle's tiles share one or two L2s (its B
+// panel and diagonal blocks are fetched into those only).  -1: padding.
+__device__ inline int xcd_tile(int t, int T)
+{
+    const int c = ceil_div(T, 8);
+    const int tp = (t % 8) * c + t / 8;
+    return tp < T ? tp : -1;
+}
+
 // Launch 1 of panel c0, chained: role work-groups first (block R g + 8 r + l
 // is role r of particle 8 g + l, R = chain_roles), then the MB x NB update
-// tiles of the rows below the panel.
+// tiles of the rows below the panel (XCD-aware order).
 template <int KC, int MB, int NB, bool TWO>
 __global__ void __launch_bounds__(kThreads)
 __attribute__((amdgpu_waves_per_eu(KC == 16 ? 2 : 1, KC == 16 ? 2 : 1))) panel_chain_kernel(double *__restrict__ K, int np, int n,
@@ -326,7 +338,8 @@ __attribute__((amdgpu_waves_per_eu(KC == 16 ? 2 : 1, KC == 16 ? 2 : 1))) panel_c
     }
     constexpr int CT = kNBO / NB;            // column tiles of the panel
     const int rt = (np - c0 - kNBO) / MB;    // row tiles below the panel
-    const int t = b - nrole;
+    const int t = xcd_tile(b - nrole, S * rt * CT);
+    if (t < 0) return;
     const int s = t / (rt * CT), u = t % (rt * CT);
     if (info[s] != 0) return;
     gemm_tile_update<MB, NB, KC>(K + (size_t)s * np * np, np, c0 + kNBO + (u / CT) * MB, c0 + (u % CT) * NB, 0, c0,
@@ -339,14 +352,16 @@ __attribute__((amdgpu_waves_per_eu(KC == 16 ? 2 : 1, KC == 16 ? 2 : 1))) panel_c
 // for one tile).  The tile's own b0 result is re-read after the barrier:
 // global stores of a work-group are visible to it.
 template <int KC>
-__global__ void __launch_bounds__(kThreads) solve_rows_kernel(double *__restrict__ K, int np, int c0,
+__global__ void __launch_bounds__(kThreads) solve_rows_kernel(double *__restrict__ K, int np, int c0, int S,
                                                               const int *__restrict__ info,
                                                               const double *__restrict__ dinv)
 {
     __shared__ __attribute__((aligned(16))) double smem[TileShape<kMB, kNBI, KC>::SMEM];
     const int rt = (np - c0 - kNBO) / kNBI;
-    const int s = blockIdx.x / rt;
-    const int row0 = c0 + kNBO + (blockIdx.x % rt) * kNBI;
+    const int t = xcd_tile(blockIdx.x, S * rt);
+    if (t < 0) return;
+    const int s = t / rt;
+    const int row0 = c0 + kNBO + (t % rt) * kNBI;
     if (info[s] != 0) return;
     double *A = K + (size_t)s * np * np;
     const int c1 = c0 + kNBI;
@@ -365,7 +380,7 @@ static void chain_launch(const SwarmBuffers &b, int c0, hipStream_t st)
 {
     const int tiles = ((b.np - c0 - kNBO) / MB) * (kNBO / NB);
     const int nrole = chain_roles<TWO>() * ceil_div(b.S, 8);
-    hipLaunchKernelGGL((panel_chain_kernel<KC, MB, NB, TWO>), dim3(nrole + b.S * tiles), dim3(kThreads), 0, st,
+    hipLaunchKernelGGL((panel_chain_kernel<KC, MB, NB, TWO>), dim3(nrole + 8 * ceil_div(b.S * tiles, 8)), dim3(kThreads), 0, st,
                        b.K, b.np, b.n, c0, b.S, b.info, b.logdet_part, b.dinv, b.chain);
 }
 
@@ -391,8 +406,8 @@ void launch_solve_rows(const SwarmBuffers &b, int c0, hipStream_t st)
 {
     const int rt = (b.np - c0 - kNBO) / kNBI;
     if (rt <= 0) return;
-    hipLaunchKernelGGL((solve_rows_kernel<16>), dim3(b.S * rt), dim3(kThreads), 0, st, b.K, b.np, c0, b.info,
-                       b.dinv);
+    hipLaunchKernelGGL((solve_rows_kernel<16>), dim3(8 * ceil_div(b.S * rt, 8)), dim3(kThreads), 0, st, b.K, b.np,
+                       c0, b.S, b.info, b.dinv);
 }
 
 template <int KC, int MB, int NB>
