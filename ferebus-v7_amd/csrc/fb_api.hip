@@ -172,6 +172,7 @@ struct fb_handle {
     bool use_dag = true;             // persistent factorisation for the batched path
     int gemm_kc = 32;                // lock-step trailing-update chunk depth
     bool chain = true;               // lock-step with 16-deep chunks: two launches per panel (FB_CHAIN=0: three)
+    int tail = 3;                    // chained: trailing panels pre-updated by the launch before them (FB_TAIL)
     int *d_chain = nullptr;          // per-particle words of the two-launch schedule
     int tile = 0;                    // lock-step panel-update tile shape (16-deep chunks)
     const signed char *panel_tile = nullptr;   // per-panel shapes of mode kModePerPanel
@@ -241,6 +242,7 @@ fb::SwarmBuffers buffers(fb_handle *h, int g0, int S, const double *theta, doubl
     b.extra = h->d_extra + (size_t)g0 * 4; b.zbuf = h->d_z + (size_t)g0 * h->n;
     b.chain = (h->chain && h->gemm_kc == 16 && S <= fb::kChainMaxSwarm)
                   ? h->d_chain + (size_t)g0 * fb::kChainStride : nullptr;
+    b.tail = b.chain ? h->tail : 0;
     return b;
 }
 
@@ -635,6 +637,7 @@ int fb_create(fb_handle **out, int n, int d, const double *x, const double *y, c
     }
     if (const char *env = std::getenv("FB_AUTOTUNE")) h->autotune = std::atoi(env) != 0;
     if (const char *env = std::getenv("FB_CHAIN")) h->chain = std::atoi(env) != 0;
+    if (const char *env = std::getenv("FB_TAIL")) h->tail = std::max(0, std::atoi(env));
     if (const char *env = std::getenv("FB_FACTOR_PATH")) {
         const int m = std::atoi(env);
         if (m >= 0 && m < kNumModes) h->forced_mode = m;

@@ -145,6 +145,7 @@ struct SwarmBuffers {
     double *extra;         // S x 4: ll, logdet, mu_hat, sigma2_hat
     double *zbuf;          // S x n: z = L^{-1} rhs_0 (or the concentrated residual)
     int *chain = nullptr;  // S x kChainStride: chained schedule (two launches per panel), or none
+    int tail = 0;          // chained: the last `tail` panels are pre-updated in the launch of the one before
 };
 
 void launch_build(const SwarmBuffers &b, hipStream_t st);

@@ -235,13 +235,29 @@ __device__ inline int xcd_tile(int t, int T)
     return tp < T ? tp : -1;
 }
 
+// Tail pre-update: the update launch of panel c0 may also apply columns
+// k0 .. c0 to the next `tail` panels (their diagonal 64 x 64 block and their
+// 64 x 128 row tiles below it), so that those panels' own launches, with few
+// rows left and deep updates, only apply columns from c0 on.  Each element
+// still takes its products in increasing k (the accumulator goes through
+// memory exactly), so the bits do not change.
+__host__ __device__ inline int tail_tiles(int np, int c0, int tail)
+{
+    int t = 0;
+    for (int j = 1; j <= tail; ++j) t += 1 + (np - c0 - kNBO * j - kNBI) / kNBI;
+    return t;
+}
+
 // Launch 1 of panel c0, chained: role work-groups first (block R g + 8 r + l
 // is role r of particle 8 g + l, R = chain_roles), then the MB x NB update
-// tiles of the rows below the panel (XCD-aware order).
+// tiles of the rows below the panel, then the tail pre-update tiles (both in
+// XCD-aware order).  The panel's own updates apply columns k0 .. c0 (k0 > 0:
+// an earlier launch applied the first k0).
 template <int KC, int MB, int NB, bool TWO>
 __global__ void __launch_bounds__(kThreads)
 __attribute__((amdgpu_waves_per_eu(KC == 16 ? 2 : 1, KC == 16 ? 2 : 1))) panel_chain_kernel(double *__restrict__ K, int np, int n,
-                                                                       int c0, int S, int *__restrict__ info,
+                                                                       int c0, int k0, int tail, int S,
+                                                                       int *__restrict__ info,
                                                                        double *__restrict__ logdet_part,
                                                                        double *__restrict__ dinv,
                                                                        int *__restrict__ chain)
@@ -262,7 +278,7 @@ __attribute__((amdgpu_waves_per_eu(KC == 16 ? 2 : 1, KC == 16 ? 2 : 1))) panel_c
         double *dinv0 = dinv + ((size_t)s * nbk + c0 / kNBI) * kDinv;
         if (role == 0) {
             if (info[s] == 0) {
-                gemm_tile_update<kMB, kNBI, KC, 2, true>(A, np, c0, c0, 0, c0, smem);
+                gemm_tile_update<kMB, kNBI, KC, 2, true>(A, np, c0, c0, k0, c0, smem);
                 __syncthreads();
                 double lg = 0.0;
                 const int fail = potrf64_block<true>(A, np, n, c0, &lg, smem, dinv0);
@@ -275,7 +291,7 @@ __attribute__((amdgpu_waves_per_eu(KC == 16 ? 2 : 1, KC == 16 ? 2 : 1))) panel_c
         } else if (TWO) {
             // BC: the b1 row tile (b1, b0:b1), then the rest of the chain
             chain_announce(cw + kChainXccB, tag);
-            gemm_tile_update<kMB, kNBO, KC>(A, np, c1, c0, 0, c0, smem);
+            gemm_tile_update<kMB, kNBO, KC>(A, np, c1, c0, k0, c0, smem);
             if (!chain_wait(cw + kChainF0, tag, &ok)) {
                 if (threadIdx.x == 0) info[s] = kInfoWaitTimeout;
                 return;
@@ -302,7 +318,7 @@ __attribute__((amdgpu_waves_per_eu(KC == 16 ? 2 : 1, KC == 16 ? 2 : 1))) panel_c
             }
         } else if (role == 1) {
             chain_announce(cw + kChainXccB, tag);
-            gemm_tile_update<kMB, kNBI, KC, 2, true>(A, np, c1, c0, 0, c0, smem);
+            gemm_tile_update<kMB, kNBI, KC, 2, true>(A, np, c1, c0, k0, c0, smem);
             if (!chain_wait(cw + kChainF0, tag, &ok)) {
                 if (threadIdx.x == 0) info[s] = kInfoWaitTimeout;
             } else if (info[s] == 0) {
@@ -311,7 +327,7 @@ __attribute__((amdgpu_waves_per_eu(KC == 16 ? 2 : 1, KC == 16 ? 2 : 1))) panel_c
             chain_publish(cw + kChainF1, cw + kChainXccC, tag);
         } else {
             chain_announce(cw + kChainXccC, tag);
-            gemm_tile_update<kMB, kNBI, KC, 2, true>(A, np, c1, c1, 0, c0, smem);
+            gemm_tile_update<kMB, kNBI, KC, 2, true>(A, np, c1, c1, k0, c0, smem);
             if (!chain_wait(cw + kChainF1, tag, &ok)) {
                 if (threadIdx.x == 0) info[s] = kInfoWaitTimeout;
                 return;
@@ -338,12 +354,32 @@ __attribute__((amdgpu_waves_per_eu(KC == 16 ? 2 : 1, KC == 16 ? 2 : 1))) panel_c
     }
     constexpr int CT = kNBO / NB;            // column tiles of the panel
     const int rt = (np - c0 - kNBO) / MB;    // row tiles below the panel
-    const int t = xcd_tile(b - nrole, S * rt * CT);
+    const int nreg = 8 * ceil_div(S * rt * CT, 8);
+    if (b - nrole < nreg) {
+        const int t = xcd_tile(b - nrole, S * rt * CT);
+        if (t < 0) return;
+        const int s = t / (rt * CT), u = t % (rt * CT);
+        if (info[s] != 0) return;
+        gemm_tile_update<MB, NB, KC>(K + (size_t)s * np * np, np, c0 + kNBO + (u / CT) * MB, c0 + (u % CT) * NB, k0,
+                                     c0, smem);
+        return;
+    }
+    // tail pre-update tiles
+    const int ct = tail_tiles(np, c0, tail);
+    const int t = xcd_tile(b - nrole - nreg, S * ct);
     if (t < 0) return;
-    const int s = t / (rt * CT), u = t % (rt * CT);
+    const int s = t / ct;
+    int v = t % ct, cp = c0;
+    for (int j = 1; j <= tail; ++j) {
+        cp = c0 + kNBO * j;
+        const int cnt = 1 + (np - cp - kNBI) / kNBI;
+        if (v < cnt) break;
+        v -= cnt;
+    }
     if (info[s] != 0) return;
-    gemm_tile_update<MB, NB, KC>(K + (size_t)s * np * np, np, c0 + kNBO + (u / CT) * MB, c0 + (u % CT) * NB, 0, c0,
-                                 smem);
+    double *A = K + (size_t)s * np * np;
+    if (v == 0) gemm_tile_update<kMB, kNBI, KC>(A, np, cp, cp, k0, c0, smem);
+    else gemm_tile_update<kMB, kNBO, KC>(A, np, cp + kNBI * v, cp, k0, c0, smem);
 }
 
 // Launch 2 of panel c0, chained: every row tile below the panel is solved
@@ -376,20 +412,21 @@ __global__ void __launch_bounds__(kThreads) solve_rows_kernel(double *__restrict
 }
 
 template <int KC, int MB, int NB, bool TWO>
-static void chain_launch(const SwarmBuffers &b, int c0, hipStream_t st)
+static void chain_launch(const SwarmBuffers &b, int c0, int k0, int tail, hipStream_t st)
 {
     const int tiles = ((b.np - c0 - kNBO) / MB) * (kNBO / NB);
     const int nrole = chain_roles<TWO>() * ceil_div(b.S, 8);
-    hipLaunchKernelGGL((panel_chain_kernel<KC, MB, NB, TWO>), dim3(nrole + 8 * ceil_div(b.S * tiles, 8)), dim3(kThreads), 0, st,
-                       b.K, b.np, b.n, c0, b.S, b.info, b.logdet_part, b.dinv, b.chain);
+    const int grid = nrole + 8 * ceil_div(b.S * tiles, 8) + 8 * ceil_div(b.S * tail_tiles(b.np, c0, tail), 8);
+    hipLaunchKernelGGL((panel_chain_kernel<KC, MB, NB, TWO>), dim3(grid), dim3(kThreads), 0, st, b.K, b.np, b.n, c0,
+                       k0, tail, b.S, b.info, b.logdet_part, b.dinv, b.chain);
 }
 
 template <bool TWO>
-static void chain_tiles(const SwarmBuffers &b, int tile, int c0, hipStream_t st)
+static void chain_tiles(const SwarmBuffers &b, int tile, int c0, int k0, int tail, hipStream_t st)
 {
-    if (tile == 1) chain_launch<16, 32, 128, TWO>(b, c0, st);
-    else if (tile == 2) chain_launch<16, 32, 64, TWO>(b, c0, st);
-    else chain_launch<16, 64, 128, TWO>(b, c0, st);
+    if (tile == 1) chain_launch<16, 32, 128, TWO>(b, c0, k0, tail, st);
+    else if (tile == 2) chain_launch<16, 32, 64, TWO>(b, c0, k0, tail, st);
+    else chain_launch<16, 64, 128, TWO>(b, c0, k0, tail, st);
 }
 
 // 16-deep chunks only (the 32-deep update tile leaves no room for the roles).
@@ -398,8 +435,13 @@ static void chain_tiles(const SwarmBuffers &b, int tile, int c0, hipStream_t st)
 void launch_panel_chain(const SwarmBuffers &b, int c0, hipStream_t st)
 {
     const int code = b.panel_tile ? b.panel_tile[c0 / kNBO] : b.tile;
-    if (code & kTileTwoRoles) chain_tiles<true>(b, code & 3, c0, st);
-    else chain_tiles<false>(b, code & 3, c0, st);
+    // tail pre-update: panel q = npan - 1 - tail applies its columns 0 .. c0q
+    // to the last `tail` panels as well, which then start from c0q
+    const int npan = b.np / kNBO, q = npan - 1 - b.tail;
+    const int k0 = (b.tail > 0 && q >= 1 && c0 > q * kNBO) ? q * kNBO : 0;
+    const int tail = (b.tail > 0 && q >= 1 && c0 == q * kNBO) ? b.tail : 0;
+    if (code & kTileTwoRoles) chain_tiles<true>(b, code & 3, c0, k0, tail, st);
+    else chain_tiles<false>(b, code & 3, c0, k0, tail, st);
 }
 
 void launch_solve_rows(const SwarmBuffers &b, int c0, hipStream_t st)
