@@ -829,6 +829,8 @@ static int choose_path(fb_handle *h, int s)
     const int nc = (int)codes.size();
     std::vector<std::vector<float>> tpan(nc, std::vector<float>(npan, 1e30f));
     std::vector<std::pair<hipEvent_t, hipEvent_t>> evs;
+    // two passes over the candidates, the faster time of each panel kept
+    for (int pass = 0; pass < 2; ++pass)
     for (int t = 0; t < nc; ++t) {
         set_mode(h, 2 + (codes[t] & 3), s);
         h->tile = codes[t];
@@ -837,8 +839,11 @@ static int choose_path(fb_handle *h, int s)
         enqueue_pipeline(h, s, h->d_theta, h->d_ll, h->d_info, st);
         h->k1_events = nullptr;
         if (hipStreamSynchronize(st) != hipSuccess) break;
-        for (size_t k = 0; k < evs.size() && (int)k < npan; ++k)
-            (void)hipEventElapsedTime(&tpan[t][k], evs[k].first, evs[k].second);
+        for (size_t k = 0; k < evs.size() && (int)k < npan; ++k) {
+            float ms = 1e30f;
+            (void)hipEventElapsedTime(&ms, evs[k].first, evs[k].second);
+            tpan[t][k] = std::min(tpan[t][k], ms);
+        }
         for (auto &e : evs) {
             h->event_pool.push_back(e.first);
             h->event_pool.push_back(e.second);
@@ -1063,6 +1068,11 @@ struct fb_multi {
     fb::DagWork w;
     int *h_err = nullptr;
     std::map<std::vector<int>, std::pair<fb::DagPlan, fb::DagMatrix *>> plans;   // per particle counts
+    // per particle counts: all factorisations in one persistent launch (true)
+    // or each handle's own tuned batched path one after another (timed on the
+    // first call; FB_MULTI_PERSISTENT=0/1 pins it)
+    std::map<std::vector<int>, bool> persistent;
+    int forced = -1;
 };
 
 namespace {
@@ -1136,6 +1146,7 @@ int fb_multi_create(fb_multi **out, int n_gps, fb_handle *const *gps)
         (void)hipGetLastError();
         return fail(FB_E_NOMEM, "fb_multi_create: scheduler workspace");
     }
+    if (const char *env = std::getenv("FB_MULTI_PERSISTENT")) m->forced = std::atoi(env) != 0 ? 1 : 0;
     *out = m;
     return FB_OK;
 }
@@ -1175,6 +1186,11 @@ int fb_multi_swarm_loglik(fb_multi *m, const int *s, const double *const *theta,
         for (int g = 0; g < G; ++g) {
             if (sv[g] == 0) continue;
             fb_handle *h = m->gps[g];
+            if (!persistent) {
+                // the handle's own tuned lock-step variant for this swarm size
+                auto it = h->tuned.find(sv[g]);
+                set_mode(h, it != h->tuned.end() && it->second != 0 ? it->second : 2, sv[g]);
+            }
             const fb::SwarmBuffers b = buffers(h, 0, sv[g], h->d_theta, h->d_ll, h->d_info);
             if (persistent) fb::launch_build(b, st);
             else enqueue_group(h, b, st);
@@ -1194,13 +1210,53 @@ int fb_multi_swarm_loglik(fb_multi *m, const int *s, const double *const *theta,
         FB_HIP(hipGetLastError());
         return FB_OK;
     };
-    rc = run(true);
+    if (!m->persistent.count(sv)) {
+        if (m->forced >= 0) {
+            m->persistent[sv] = m->forced != 0;
+        } else {
+            // tune every handle for its swarm size, then time both schedules
+            FB_HIP(hipStreamSynchronize(st));
+            for (int g = 0; g < G; ++g)
+                if (sv[g] > 0) {
+                    rc = choose_path(m->gps[g], sv[g]);
+                    if (rc) return rc;
+                }
+            hipEvent_t e0, e1;
+            FB_HIP(hipEventCreate(&e0));
+            FB_HIP(hipEventCreate(&e1));
+            float best[2] = {1e30f, 1e30f};
+            for (int rep = 0; rep < 2; ++rep)
+                for (int p = 0; p < 2; ++p) {
+                    (void)hipEventRecord(e0, st);
+                    rc = run(p == 1);
+                    (void)hipEventRecord(e1, st);
+                    if (rc || hipEventSynchronize(e1) != hipSuccess) break;
+                    float ms = 0.f;
+                    (void)hipEventElapsedTime(&ms, e0, e1);
+                    if (p == 1 && *m->h_err != 0) ms = 1e30f;   // a persistent run that could not be served
+                    best[p] = std::min(best[p], ms);
+                }
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            if (rc) return rc;
+            m->persistent[sv] = best[1] <= best[0];
+        }
+    }
+    const bool persistent = m->persistent.at(sv);
+    rc = run(persistent);
     if (rc) return rc;
-    if (*m->h_err == 1) return fail(FB_E_HIP, "fb_multi_swarm_loglik: the persistent factorisation found no ready task for 2 s");
-    if (*m->h_err == 2) {
+    if (persistent && *m->h_err == 1)
+        return fail(FB_E_HIP, "fb_multi_swarm_loglik: the persistent factorisation found no ready task for 2 s");
+    if (persistent && *m->h_err == 2) {
+        m->persistent[sv] = false;
         rc = run(false);   // some XCD queue was not served: lock-step path
         if (rc) return rc;
     }
+    for (int g = 0; g < G; ++g)
+        if (sv[g] > 0) {
+            rc = check_waits(m->gps[g]->h_info, sv[g], "fb_multi_swarm_loglik");
+            if (rc) return rc;
+        }
     for (int g = 0; g < G; ++g) {
         if (sv[g] == 0) continue;
         std::memcpy(ll[g], m->gps[g]->h_ll, sizeof(double) * sv[g]);

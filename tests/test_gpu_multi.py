@@ -1,8 +1,11 @@
-"""Several training sets of different N factorised in one persistent launch
-(fb_multi_swarm_loglik, the batched uneven case): every particle's ll and
-INFO must be bit-identical to its own handle's fb_swarm_loglik (same tile
-arithmetic per matrix), with handles of different kernels, likelihoods and
-swarm sizes mixed, and a skipped (s = 0) handle left untouched."""
+"""Several training sets of different N evaluated by one call
+(fb_multi_swarm_loglik, the batched uneven case), all factorised in one
+persistent launch (FB_MULTI_PERSISTENT=1), by each handle's own batched path
+one after another (=0), or by whichever of the two the call's first
+evaluation timed faster (unset): every particle's ll and INFO must be
+bit-identical to its own handle's fb_swarm_loglik (same tile arithmetic per
+matrix), with handles of different kernels, likelihoods and swarm sizes
+mixed, and a skipped (s = 0) handle left untouched."""
 import time
 
 import numpy as np
@@ -23,7 +26,12 @@ def _gp(n, d, seed, kernel="k1+k2", kernels=None, likelihood="marginal", mean="c
     return gp, x, y, prog, rng
 
 
-def test_multi_matches_each_handle(gpu):
+@pytest.mark.parametrize("schedule", ["1", "0", None])
+def test_multi_matches_each_handle(gpu, monkeypatch, schedule):
+    if schedule is None:
+        monkeypatch.delenv("FB_MULTI_PERSISTENT", raising=False)
+    else:
+        monkeypatch.setenv("FB_MULTI_PERSISTENT", schedule)
     specs = [dict(n=150, d=6, seed=81), dict(n=333, d=6, seed=82, likelihood="concentrated"),
              dict(n=520, d=9, seed=83, kernel="k1*k2"), dict(n=64, d=6, seed=84, mean="linear")]
     gps, thetas = [], []
@@ -53,9 +61,10 @@ def test_multi_matches_each_handle(gpu):
     multi.close()
 
 
-def test_multi_glycine_uneven_full_size(gpu):
+def test_multi_glycine_uneven_full_size(gpu, monkeypatch):
     """Glycine-like: 10 GPs, d=24, N ~ U[1000, 4000], 40 particles each --
     400 Cholesky factorisations of mixed size per PSO iteration in one launch."""
+    monkeypatch.setenv("FB_MULTI_PERSISTENT", "1")
     ws = synthetic.glycine_gps(seed=2)
     gps, thetas = [], []
     rng = np.random.default_rng(9)
