@@ -147,9 +147,13 @@ int fb_swarm_loglik_grad(fb_handle *h, int s, const double *theta, double *ll,
 
 /* ---- several training sets at once (uneven N) ------------------------------
  * The swarms of several handles -- e.g. one GP per atom or property, each
- * with its own N, kernel and likelihood -- are factorised together in ONE
- * persistent launch; covariance builds and likelihood reductions stay per
- * handle.  Results are bit-identical to fb_swarm_loglik on each handle.
+ * with its own N, kernel and likelihood -- are evaluated by one call: their
+ * factorisations together in ONE persistent launch, or each handle's own
+ * batched path one after another, whichever the first call for a set of
+ * swarm sizes timed faster (FB_MULTI_PERSISTENT=1/0, read at
+ * fb_multi_create, pins it); covariance builds and likelihood reductions
+ * stay per handle.  Results are bit-identical to fb_swarm_loglik on each
+ * handle.
  * No reference counterpart: the reference trains its GPs one after another
  * (main.f90 loops over atoms and properties; its kernel for one GP is the
  * path of fb_swarm_loglik).  The handles are borrowed (not owned, not to be
