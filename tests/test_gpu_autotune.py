@@ -169,3 +169,33 @@ def test_chained_schedule_tile_shapes(gpu, monkeypatch, path, tiles):
         th = np.random.default_rng(4).uniform(0.0, 1.0, (7, gp.nparams))
         out[chain] = gp.swarm_loglik(th)
     assert np.array_equal(out[0][1], out[1][1]) and np.array_equal(out[0][0], out[1][0])
+
+
+@pytest.mark.parametrize("tail", [1, 5, 7])
+def test_tail_pre_update_bits(gpu, monkeypatch, tail):
+    """The chained schedule with 1, 5 and 7 trailing panels pre-updated by the
+    launch before them (9 panels here; 7 leaves panel 1 carrying all of
+    them): the bits of the three-launch schedule."""
+    _, x, y, prog, mfit = _problem(1100, 6, seed=tail)
+    out = {}
+    for chain in (0, 1):
+        monkeypatch.setenv("FB_CHAIN", str(chain))
+        monkeypatch.setenv("FB_TAIL", str(tail))
+        monkeypatch.setenv("FB_FACTOR_PATH", "2")
+        gp = _gp(x, y, prog, mfit, 5)
+        th = np.random.default_rng(6).uniform(0.0, 1.0, (5, gp.nparams))
+        out[chain] = gp.swarm_loglik(th)
+    assert np.array_equal(out[0][1], out[1][1]) and np.array_equal(out[0][0], out[1][0])
+
+
+def test_swarm_beyond_chain_limit(gpu, monkeypatch):
+    """Swarms above 160 particles take the three-launch schedule (the
+    chained one needs every waiting work-group resident); 170 particles give
+    the bits of 170 single-particle evaluations."""
+    monkeypatch.setenv("FB_FACTOR_PATH", "2")
+    _, x, y, prog, mfit = _problem(200, 6, seed=11)
+    gp = _gp(x, y, prog, mfit, 170)
+    th = np.random.default_rng(7).uniform(0.0, 1.0, (170, gp.nparams))
+    ll, info = gp.swarm_loglik(th)
+    pl, pinfo = gp.particle_loglik(th[::17])
+    assert np.array_equal(pinfo, info[::17]) and np.array_equal(pl[pinfo == 0], ll[::17][pinfo == 0])
