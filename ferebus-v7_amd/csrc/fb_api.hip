@@ -281,16 +281,19 @@ void enqueue_group(fb_handle *h, const fb::SwarmBuffers &b, hipStream_t st)
     // one launch (kind 2)
     for (int c0 = 0; c0 < h->np; c0 += fb::kNBO) {
         if (b.chain) {
+            const bool rows = c0 + fb::kNBO < h->np && !fb::panel_fused(b, c0);
             if (h->k1_events) {
+                // tuning: the panel's whole time, its row-solve launch included
                 std::pair<hipEvent_t, hipEvent_t> ev{take_event(h), take_event(h)};
                 (void)hipEventRecord(ev.first, st);
                 fb::launch_panel_chain(b, c0, st);
+                if (rows) fb::launch_solve_rows(b, c0, st);
                 (void)hipEventRecord(ev.second, st);
                 h->k1_events->push_back(ev);
             } else {
                 timed(h, 1, st, [&] { fb::launch_panel_chain(b, c0, st); });
+                if (rows) timed(h, 2, st, [&] { fb::launch_solve_rows(b, c0, st); });
             }
-            if (c0 + fb::kNBO < h->np) timed(h, 2, st, [&] { fb::launch_solve_rows(b, c0, st); });
             continue;
         }
         const int c1 = c0 + fb::kNBI;
@@ -653,7 +656,7 @@ int fb_create(fb_handle **out, int n, int d, const double *x, const double *y, c
                 if (*c == ',') ++c;
             }
             bool ok = (int)v.size() == h->np / fb::kNBO;
-            for (signed char t : v) ok = ok && t >= 0 && t <= 6 && t != 3;
+            for (signed char t : v) ok = ok && (t == 0 || t == 1 || t == 2 || t == 4 || t == 5 || t == 6 || t == 8 || t == 12);
             if (ok) {
                 h->pinned_tiles = v;
                 h->forced_mode = kModePerPanel;
@@ -677,7 +680,7 @@ int fb_create(fb_handle **out, int n, int d, const double *x, const double *y, c
                         if (*c == ',') ++c;
                     }
                     bool ok = (int)v.size() == h->np / fb::kNBO;
-                    for (signed char t : v) ok = ok && t >= 0 && t <= 6 && t != 3;
+                    for (signed char t : v) ok = ok && (t == 0 || t == 1 || t == 2 || t == 4 || t == 5 || t == 6 || t == 8 || t == 12);
                     if (!ok) continue;
                     h->panel_tiles[r.swarm] = v;
                 }
@@ -824,15 +827,18 @@ static int choose_path(fb_handle *h, int s)
     // variant of each panel is one more candidate
     const int npan = h->np / fb::kNBO;
     std::vector<int> codes = {0, 1, 2};
-    if (h->chain && s <= fb::kChainMaxSwarm)
+    if (h->chain && s <= fb::kChainMaxSwarm) {
         for (int t = 0; t < 3; ++t) codes.push_back(t | fb::kTileTwoRoles);
+        codes.push_back(fb::kTileFused);
+        codes.push_back(fb::kTileFused | fb::kTileTwoRoles);
+    }
     const int nc = (int)codes.size();
     std::vector<std::vector<float>> tpan(nc, std::vector<float>(npan, 1e30f));
     std::vector<std::pair<hipEvent_t, hipEvent_t>> evs;
     // two passes over the candidates, the faster time of each panel kept
     for (int pass = 0; pass < 2; ++pass)
     for (int t = 0; t < nc; ++t) {
-        set_mode(h, 2 + (codes[t] & 3), s);
+        set_mode(h, 2 + ((codes[t] & fb::kTileFused) ? 0 : (codes[t] & 3)), s);
         h->tile = codes[t];
         evs.clear();
         h->k1_events = &evs;
@@ -857,7 +863,7 @@ static int choose_path(fb_handle *h, int s)
         mix[k] = (signed char)codes[bt];
         mixed = mixed || mix[k] != mix[0];
     }
-    if (mixed || (mix[0] & fb::kTileTwoRoles)) {
+    if (mixed || (mix[0] & (fb::kTileTwoRoles | fb::kTileFused))) {
         h->panel_tiles[s] = mix;
         set_mode(h, kModePerPanel, s);
         float ms_best = 1e30f;

@@ -156,7 +156,7 @@ __global__ void __launch_bounds__(kThreads) build_kernel(BuildArgs a)
         a.info[s0 + threadIdx.x] = 0;
         if (a.chain) {
             int *w = a.chain + (size_t)(s0 + threadIdx.x) * kChainStride;
-            w[kChainF0] = 0; w[kChainF1] = 0; w[kChainXccB] = 0; w[kChainXccC] = 0;
+            w[kChainF0] = 0; w[kChainF1] = 0; w[kChainXccB] = 0; w[kChainXccC] = 0; w[kChainF2] = 0;
         }
     }
     __syncthreads();

@@ -104,10 +104,10 @@ constexpr int kThreads = 256; // work-group size of every kernel (4 wave64)
 constexpr int kDinv = 4 * 16 * 16;   // doubles per 64x64 diagonal block: four 16x16 inverses
 
 // Per-particle words of the chained lock-step schedule (two launches per
-// panel; fb_potrf.hip): flags and XCD announcements, each in its own 32-byte
-// sector of a 128-byte line.
-constexpr int kChainStride = 32;
-constexpr int kChainF0 = 0, kChainF1 = 8, kChainXccB = 16, kChainXccC = 24;
+// panel, or one with the row solves fused; fb_potrf.hip): flags and XCD
+// announcements, each in its own 32-byte sector.
+constexpr int kChainStride = 64;
+constexpr int kChainF0 = 0, kChainF1 = 8, kChainXccB = 16, kChainXccC = 24, kChainF2 = 32;
 // Swarms up to this size: the waiting work-groups of the update launch (two
 // per particle) hold fewer than its 512 resident slots (two per CU), so every
 // producer they wait for finds a slot.
@@ -118,6 +118,10 @@ constexpr int kInfoWaitTimeout = -9;
 // Panel tile code bit: the chained update launch with two role work-groups
 // per particle instead of three.
 constexpr int kTileTwoRoles = 4;
+// Panel tile code bit (64 x 128 row tiles only): each row tile also solves
+// its rows once the panel's diagonal chain is published -- one launch for
+// the panel instead of two.
+constexpr int kTileFused = 8;
 
 inline __host__ __device__ int ceil_div(int a, int b) { return (a + b - 1) / b; }
 inline __host__ __device__ int round_up(int a, int b) { return ceil_div(a, b) * b; }
@@ -154,6 +158,7 @@ void launch_trsm_factor(const SwarmBuffers &b, int c0, hipStream_t st);
 void launch_update_trsm(const SwarmBuffers &b, int c1, hipStream_t st);
 void launch_panel_chain(const SwarmBuffers &b, int c0, hipStream_t st);
 void launch_solve_rows(const SwarmBuffers &b, int c0, hipStream_t st);
+bool panel_fused(const SwarmBuffers &b, int c0);
 void launch_likelihood(const SwarmBuffers &b, hipStream_t st);
 
 // Adjoint gradient (fb_grad.hip).

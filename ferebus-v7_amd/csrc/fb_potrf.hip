@@ -209,6 +209,33 @@ __device__ inline bool chain_wait(const int *flag, int tag, int *ok)
     return *ok != 0;
 }
 
+// Producer whose results were written to memory (st_f64<true>): publish
+// `tag` once every thread's stores are done.
+__device__ inline void chain_publish_mem(int *flag, int tag)
+{
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(flag, tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The row solve of one 64-row tile at row0 below panel c0 of the matrix at
+// A: solve with L(b0, b0), the 64-deep update of its b1 part with
+// L(b1, b0), solve with L(b1, b1) (dinv_s: the particle's 16x16 inverses).
+// The tile's own b0 result is re-read after the barrier: global stores of a
+// work-group are visible to it.
+template <int KC>
+__device__ inline void solve_rows_tile(double *A, int np, int c0, int row0, const double *dinv_s, double *smem)
+{
+    const int c1 = c0 + kNBI;
+    double *tile = A + (size_t)c0 * np + row0;
+    trsm64_solve<false>(tile, np, tile, np, A + (size_t)c0 * np + c0, np, dinv_s + (size_t)(c0 / kNBI) * kDinv);
+    __syncthreads();
+    gemm_tile_update<kMB, kNBI, KC, 2, true>(A, np, row0, c1, c0, c1, smem);
+    __syncthreads();
+    trsm64_solve<false>(smem, kPLD, A + (size_t)c1 * np + row0, np, A + (size_t)c1 * np + c1, np,
+                        dinv_s + (size_t)(c1 / kNBI) * kDinv);
+}
+
 template <int KC, int MB, int NB>
 constexpr int chain_smem()
 {
@@ -252,8 +279,10 @@ __host__ __device__ inline int tail_tiles(int np, int c0, int tail)
 // is role r of particle 8 g + l, R = chain_roles), then the MB x NB update
 // tiles of the rows below the panel, then the tail pre-update tiles (both in
 // XCD-aware order).  The panel's own updates apply columns k0 .. c0 (k0 > 0:
-// an earlier launch applied the first k0).
-template <int KC, int MB, int NB, bool TWO>
+// an earlier launch applied the first k0).  FUSED (64 x 128 row tiles): the
+// roles write their results to memory and the last one publishes F2; each
+// row tile then solves its rows (solve_rows_tile) -- the panel in one launch.
+template <int KC, int MB, int NB, bool TWO, bool FUSED>
 __global__ void __launch_bounds__(kThreads)
 __attribute__((amdgpu_waves_per_eu(KC == 16 ? 2 : 1, KC == 16 ? 2 : 1))) panel_chain_kernel(double *__restrict__ K, int np, int n,
                                                                        int c0, int k0, int tail, int S,
@@ -281,7 +310,7 @@ __attribute__((amdgpu_waves_per_eu(KC == 16 ? 2 : 1, KC == 16 ? 2 : 1))) panel_c
                 gemm_tile_update<kMB, kNBI, KC, 2, true>(A, np, c0, c0, k0, c0, smem);
                 __syncthreads();
                 double lg = 0.0;
-                const int fail = potrf64_block<true>(A, np, n, c0, &lg, smem, dinv0);
+                const int fail = potrf64_block<true, FUSED>(A, np, n, c0, &lg, smem, dinv0);
                 if (threadIdx.x == 0) {
                     if (fail) info[s] = fail;
                     else logdet_part[(size_t)s * nbk + c0 / kNBI] = lg;
@@ -294,13 +323,17 @@ __attribute__((amdgpu_waves_per_eu(KC == 16 ? 2 : 1, KC == 16 ? 2 : 1))) panel_c
             gemm_tile_update<kMB, kNBO, KC>(A, np, c1, c0, k0, c0, smem);
             if (!chain_wait(cw + kChainF0, tag, &ok)) {
                 if (threadIdx.x == 0) info[s] = kInfoWaitTimeout;
+                if constexpr (FUSED) chain_publish_mem(cw + kChainF2, tag);
                 return;
             }
-            if (info[s] != 0) return;
+            if (info[s] != 0) {
+                if constexpr (FUSED) chain_publish_mem(cw + kChainF2, tag);
+                return;
+            }
             double *Xs = smem;                       // L(b1, b0) on chip
             double *Ls = smem + kNBI * kPLD;         // block (b1, b1), then its factorisation
             double *tile = A + (size_t)c0 * np + c1;
-            trsm64_solve<false, true>(tile, np, tile, np, A + (size_t)c0 * np + c0, np, dinv0, Xs);
+            trsm64_solve<false, true, FUSED>(tile, np, tile, np, A + (size_t)c0 * np + c0, np, dinv0, Xs);
             const double *Cg = A + (size_t)c1 * np + c1;
             for (int e = threadIdx.x; e < kNBI * kNBI / 2; e += kThreads) {
                 const int col = e / (kNBI / 2), r2 = (e % (kNBI / 2)) * 2;
@@ -310,19 +343,21 @@ __attribute__((amdgpu_waves_per_eu(KC == 16 ? 2 : 1, KC == 16 ? 2 : 1))) panel_c
             syrk64_lds(Ls, Xs);
             __syncthreads();
             double lg = 0.0;
-            const int fail = potrf64_block<true>(A, np, n, c1, &lg, Ls,
-                                                 dinv + ((size_t)s * nbk + c1 / kNBI) * kDinv);
+            const int fail = potrf64_block<true, FUSED>(A, np, n, c1, &lg, Ls,
+                                                        dinv + ((size_t)s * nbk + c1 / kNBI) * kDinv);
             if (threadIdx.x == 0) {
                 if (fail) info[s] = fail;
                 else logdet_part[(size_t)s * nbk + c1 / kNBI] = lg;
             }
+            if constexpr (FUSED) chain_publish_mem(cw + kChainF2, tag);
         } else if (role == 1) {
             chain_announce(cw + kChainXccB, tag);
             gemm_tile_update<kMB, kNBI, KC, 2, true>(A, np, c1, c0, k0, c0, smem);
             if (!chain_wait(cw + kChainF0, tag, &ok)) {
                 if (threadIdx.x == 0) info[s] = kInfoWaitTimeout;
             } else if (info[s] == 0) {
-                trsm64_solve<false>(smem, kPLD, A + (size_t)c0 * np + c1, np, A + (size_t)c0 * np + c0, np, dinv0);
+                trsm64_solve<false, false, FUSED>(smem, kPLD, A + (size_t)c0 * np + c1, np, A + (size_t)c0 * np + c0,
+                                                  np, dinv0);
             }
             chain_publish(cw + kChainF1, cw + kChainXccC, tag);
         } else {
@@ -330,9 +365,13 @@ __attribute__((amdgpu_waves_per_eu(KC == 16 ? 2 : 1, KC == 16 ? 2 : 1))) panel_c
             gemm_tile_update<kMB, kNBI, KC, 2, true>(A, np, c1, c1, k0, c0, smem);
             if (!chain_wait(cw + kChainF1, tag, &ok)) {
                 if (threadIdx.x == 0) info[s] = kInfoWaitTimeout;
+                if constexpr (FUSED) chain_publish_mem(cw + kChainF2, tag);
                 return;
             }
-            if (info[s] != 0) return;
+            if (info[s] != 0) {
+                if constexpr (FUSED) chain_publish_mem(cw + kChainF2, tag);
+                return;
+            }
             double *Xs = smem + kPotrfSmem;          // L(b1, b0)
             const double *Xg = A + (size_t)c0 * np + c1;
             for (int e = threadIdx.x; e < kNBI * kNBI / 2; e += kThreads) {
@@ -343,12 +382,13 @@ __attribute__((amdgpu_waves_per_eu(KC == 16 ? 2 : 1, KC == 16 ? 2 : 1))) panel_c
             syrk64_lds(smem, Xs);
             __syncthreads();
             double lg = 0.0;
-            const int fail = potrf64_block<true>(A, np, n, c1, &lg, smem,
-                                                 dinv + ((size_t)s * nbk + c1 / kNBI) * kDinv);
+            const int fail = potrf64_block<true, FUSED>(A, np, n, c1, &lg, smem,
+                                                        dinv + ((size_t)s * nbk + c1 / kNBI) * kDinv);
             if (threadIdx.x == 0) {
                 if (fail) info[s] = fail;
                 else logdet_part[(size_t)s * nbk + c1 / kNBI] = lg;
             }
+            if constexpr (FUSED) chain_publish_mem(cw + kChainF2, tag);
         }
         return;
     }
@@ -360,8 +400,20 @@ __attribute__((amdgpu_waves_per_eu(KC == 16 ? 2 : 1, KC == 16 ? 2 : 1))) panel_c
         if (t < 0) return;
         const int s = t / (rt * CT), u = t % (rt * CT);
         if (info[s] != 0) return;
-        gemm_tile_update<MB, NB, KC>(K + (size_t)s * np * np, np, c0 + kNBO + (u / CT) * MB, c0 + (u % CT) * NB, k0,
-                                     c0, smem);
+        double *A = K + (size_t)s * np * np;
+        const int row0 = c0 + kNBO + (u / CT) * MB;
+        gemm_tile_update<MB, NB, KC>(A, np, row0, c0 + (u % CT) * NB, k0, c0, smem);
+        if constexpr (FUSED) {
+            static_assert(MB == kNBI && NB == kNBO, "the fused row solve needs 64 x 128 row tiles");
+            const int tag = c0 / kNBO + 1;
+            if (!chain_wait(chain + (size_t)s * kChainStride + kChainF2, tag, &ok)) {
+                if (threadIdx.x == 0) info[s] = kInfoWaitTimeout;
+                return;
+            }
+            if (info[s] != 0) return;
+            __syncthreads();
+            solve_rows_tile<KC>(A, np, c0, row0, dinv + (size_t)s * (np / kNBI) * kDinv, smem);
+        }
         return;
     }
     // tail pre-update tiles
@@ -383,10 +435,8 @@ __attribute__((amdgpu_waves_per_eu(KC == 16 ? 2 : 1, KC == 16 ? 2 : 1))) panel_c
 }
 
 // Launch 2 of panel c0, chained: every row tile below the panel is solved
-// with L(b0, b0), takes the 64-deep update of its b1 part with L(b1, b0) and
-// is solved with L(b1, b1) (launches 2 and 3 of the three-launch schedule
-// for one tile).  The tile's own b0 result is re-read after the barrier:
-// global stores of a work-group are visible to it.
+// (solve_rows_tile: launches 2 and 3 of the three-launch schedule for one
+// tile).
 template <int KC>
 __global__ void __launch_bounds__(kThreads) solve_rows_kernel(double *__restrict__ K, int np, int c0, int S,
                                                               const int *__restrict__ info,
@@ -399,32 +449,24 @@ __global__ void __launch_bounds__(kThreads) solve_rows_kernel(double *__restrict
     const int s = t / rt;
     const int row0 = c0 + kNBO + (t % rt) * kNBI;
     if (info[s] != 0) return;
-    double *A = K + (size_t)s * np * np;
-    const int c1 = c0 + kNBI;
-    const size_t nbk = (size_t)np / kNBI;
-    double *tile = A + (size_t)c0 * np + row0;
-    trsm64_solve<false>(tile, np, tile, np, A + (size_t)c0 * np + c0, np, dinv + ((size_t)s * nbk + c0 / kNBI) * kDinv);
-    __syncthreads();
-    gemm_tile_update<kMB, kNBI, KC, 2, true>(A, np, row0, c1, c0, c1, smem);
-    __syncthreads();
-    trsm64_solve<false>(smem, kPLD, A + (size_t)c1 * np + row0, np, A + (size_t)c1 * np + c1, np,
-                        dinv + ((size_t)s * nbk + c1 / kNBI) * kDinv);
+    solve_rows_tile<KC>(K + (size_t)s * np * np, np, c0, row0, dinv + (size_t)s * (np / kNBI) * kDinv, smem);
 }
 
-template <int KC, int MB, int NB, bool TWO>
+template <int KC, int MB, int NB, bool TWO, bool FUSED = false>
 static void chain_launch(const SwarmBuffers &b, int c0, int k0, int tail, hipStream_t st)
 {
     const int tiles = ((b.np - c0 - kNBO) / MB) * (kNBO / NB);
     const int nrole = chain_roles<TWO>() * ceil_div(b.S, 8);
     const int grid = nrole + 8 * ceil_div(b.S * tiles, 8) + 8 * ceil_div(b.S * tail_tiles(b.np, c0, tail), 8);
-    hipLaunchKernelGGL((panel_chain_kernel<KC, MB, NB, TWO>), dim3(grid), dim3(kThreads), 0, st, b.K, b.np, b.n, c0,
+    hipLaunchKernelGGL((panel_chain_kernel<KC, MB, NB, TWO, FUSED>), dim3(grid), dim3(kThreads), 0, st, b.K, b.np, b.n, c0,
                        k0, tail, b.S, b.info, b.logdet_part, b.dinv, b.chain);
 }
 
 template <bool TWO>
 static void chain_tiles(const SwarmBuffers &b, int tile, int c0, int k0, int tail, hipStream_t st)
 {
-    if (tile == 1) chain_launch<16, 32, 128, TWO>(b, c0, k0, tail, st);
+    if (tile & kTileFused) chain_launch<16, 64, 128, TWO, true>(b, c0, k0, tail, st);
+    else if (tile == 1) chain_launch<16, 32, 128, TWO>(b, c0, k0, tail, st);
     else if (tile == 2) chain_launch<16, 32, 64, TWO>(b, c0, k0, tail, st);
     else chain_launch<16, 64, 128, TWO>(b, c0, k0, tail, st);
 }
@@ -440,8 +482,17 @@ void launch_panel_chain(const SwarmBuffers &b, int c0, hipStream_t st)
     const int npan = b.np / kNBO, q = npan - 1 - b.tail;
     const int k0 = (b.tail > 0 && q >= 1 && c0 > q * kNBO) ? q * kNBO : 0;
     const int tail = (b.tail > 0 && q >= 1 && c0 == q * kNBO) ? b.tail : 0;
-    if (code & kTileTwoRoles) chain_tiles<true>(b, code & 3, c0, k0, tail, st);
-    else chain_tiles<false>(b, code & 3, c0, k0, tail, st);
+    const int tile = (code & kTileFused) ? kTileFused : (code & 3);
+    if (code & kTileTwoRoles) chain_tiles<true>(b, tile, c0, k0, tail, st);
+    else chain_tiles<false>(b, tile, c0, k0, tail, st);
+}
+
+// The panel's update launch also solved its rows (kTileFused): no row-solve
+// launch for it.
+bool panel_fused(const SwarmBuffers &b, int c0)
+{
+    const int code = b.panel_tile ? b.panel_tile[c0 / kNBO] : b.tile;
+    return b.chain && (code & kTileFused);
 }
 
 void launch_solve_rows(const SwarmBuffers &b, int c0, hipStream_t st)

@@ -213,6 +213,19 @@ __device__ inline void gemm_tile_update(double *__restrict__ A, int np, int row0
 
 constexpr int kPotrfSmem = kNBI * kPLD + kNBI + 8;   // doubles: block, 1 / L_ii, flag, log sum
 
+// A store of v at p; WT: one that reaches memory (a device-scope atomic store
+// goes past the XCD's L2), for results read by work-groups of other XCDs in
+// the same launch.
+template <bool WT>
+__device__ inline void st_f64(double *p, double v)
+{
+    if constexpr (WT)
+        __hip_atomic_store(reinterpret_cast<unsigned long long *>(p), (unsigned long long)__double_as_longlong(v),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+        *p = v;
+}
+
 // sqrt(x) and 1 / sqrt(x) for x > 0: the hardware reciprocal square root
 // estimate refined by two steps of Goldschmidt's coupled iteration
 // (g -> sqrt x, h -> 1 / (2 sqrt x); r = 1/2 - g h, g += g r, h += h r),
@@ -265,7 +278,7 @@ __device__ inline double readlane_f64(double x, int l)
 // (thread 0) and the inverses of its four 16x16 diagonal blocks to dinv
 // (block q column major at dinv[256 q]), for trsm64_mfma.  STAGED: the
 // block is already in smem (Ls layout), left there by the caller's update.
-template <bool STAGED = false>
+template <bool STAGED = false, bool WT = false>
 __device__ inline int potrf64_block(double *__restrict__ A, int np, int n, int c0, double *logdet,
                                     double *__restrict__ smem, double *__restrict__ dinv)
 {
@@ -310,12 +323,12 @@ __device__ inline int potrf64_block(double *__restrict__ A, int np, int n, int c
                 }
                 double *W = dinv + q * 256 + lane * 16;
 #pragma unroll
-                for (int i = 0; i < 16; ++i) W[i] = x[i];
+                for (int i = 0; i < 16; ++i) st_f64<WT>(W + i, x[i]);
             }
         } else if (wave == 3) {
             for (int e = lane; e < 16 * B; e += 64) {
                 const int col = 16 * q + e / B, r = e % B;
-                if (r >= col) Ag[(size_t)col * np + r] = Ls[col * kPLD + r];
+                if (r >= col) st_f64<WT>(Ag + (size_t)col * np + r, Ls[col * kPLD + r]);
             }
         }
     };
@@ -436,12 +449,12 @@ __device__ inline int potrf64_block(double *__restrict__ A, int np, int n, int c
                         }
                         double *W = dinv + q * 256 + lane * 16;
 #pragma unroll
-                        for (int i = 0; i < 16; ++i) W[i] = x[i];
+                        for (int i = 0; i < 16; ++i) st_f64<WT>(W + i, x[i]);
                     }
                 } else {
                     for (int e = lane + (w == 3 ? 64 : 0); e < 16 * B; e += 128) {
                         const int col = 16 * q + e / B, r = e % B;
-                        if (r >= col) Ag[(size_t)col * np + r] = Ls[col * kPLD + r];
+                        if (r >= col) st_f64<WT>(Ag + (size_t)col * np + r, Ls[col * kPLD + r]);
                     }
                 }
             }
@@ -468,7 +481,7 @@ __device__ inline int potrf64_block(double *__restrict__ A, int np, int n, int c
 // in lane l, component v: the B-operand layout of four consecutive k-steps,
 // so each Y_k feeds the later products straight from registers.  No LDS, no
 // barrier; the tile is read completely before it is overwritten.
-template <bool IDENT, bool COPY = false>
+template <bool IDENT, bool COPY = false, bool WT = false>
 __device__ inline void trsm64_solve(const double *Cin, int ldi, double *Ct, int ldc,
                                     const double *__restrict__ Lb, int ldl, const double *__restrict__ dinv,
                                     double *__restrict__ copy = nullptr)
@@ -513,7 +526,7 @@ __device__ inline void trsm64_solve(const double *Cin, int ldi, double *Ct, int 
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) Cw[(size_t)(16 * j + 4 * v + lq) * ldc] = y[j][v];
+        for (int v = 0; v < 4; ++v) st_f64<WT>(Cw + (size_t)(16 * j + 4 * v + lq) * ldc, y[j][v]);
     if constexpr (COPY) {
         // the solved tile also on chip: copy[c * kPLD + r]
 #pragma unroll

@@ -152,11 +152,13 @@ def test_chained_schedule_bit_identical(gpu, monkeypatch, n, dup, src, S):
     _check(out[1][0], out[1][1], oll, oinfo, 1e-8, f"chained n={n}")
 
 
-@pytest.mark.parametrize("path,tiles", [(3, None), (4, None), (5, "2,0,1,2,0,1,2,0"), (5, "6,4,5,2,4,1,6,5")])
+@pytest.mark.parametrize("path,tiles", [(3, None), (4, None), (5, "2,0,1,2,0,1,2,0"), (5, "6,4,5,2,4,1,6,5"),
+                                        (5, "8,12,8,12,0,8,4,12")])
 def test_chained_schedule_tile_shapes(gpu, monkeypatch, path, tiles):
     """The chained update launch with the 32x128 and 32x64 row tiles, the
-    per-panel mix and the two-role variant (tile code + 4): the bits of the
-    three-launch schedule."""
+    per-panel mix, the two-role variant (tile code + 4) and the fused row
+    solve (8, 12: one launch for the panel): the bits of the three-launch
+    schedule."""
     _, x, y, prog, mfit = _problem(900, 6, seed=path)
     out = {}
     for chain in (0, 1):
