@@ -234,8 +234,12 @@ int fb_profile_read(fb_handle *h, int kind, double *total_ms, long long *launche
  * FB_FACTOR_PATH=0..4 pins a path; FB_FACTOR_PATH=5 with
  * FB_PANEL_TILES=t0,t1,... (one code per 128-column panel: 0..2 the shapes
  * of paths 2..4, 4..6 the same with two work-groups per particle in the
- * chained update launch) pins a per-panel mix; FB_CHAIN=0 gives three
- * launches per panel instead; FB_PERSISTENT=0/1 pins path 1/0;
+ * chained update launch, 8 and 12 the 64x128 shape with three or two such
+ * work-groups and the row solve fused into the update launch) pins a
+ * per-panel mix; FB_CHAIN=0 gives three launches per panel instead;
+ * FB_TAIL=n sets how many trailing panels the chained schedule pre-updates
+ * in the launch before them (default 3, 0 off); FB_PERSISTENT=0/1 pins path
+ * 1/0;
  * FB_AUTOTUNE=0 keeps path 0 without timing; FB_TUNE_CACHE=<file> keeps the
  * choices across processes (JSON records keyed on device, padded order and
  * swarm size, with the per-panel shapes). */
