@@ -112,8 +112,9 @@ constexpr int kChainF0 = 0, kChainF1 = 8, kChainXccB = 16, kChainXccC = 24, kCha
 // per particle) hold fewer than its 512 resident slots (two per CU), so every
 // producer they wait for finds a slot.
 constexpr int kChainMaxSwarm = 160;
-// INFO of a particle whose factorisation stopped at a bounded wait (1 s)
-// instead of hanging; the host reports an error for it.
+// INFO of a particle whose factorisation stopped at a bounded wait (1 s, or
+// FB_CHAIN_WAIT_TICKS of the 100 MHz clock) instead of hanging; the host
+// reports an error for it.
 constexpr int kInfoWaitTimeout = -9;
 // Panel tile code bit: the chained update launch with two role work-groups
 // per particle instead of three.
@@ -150,6 +151,7 @@ struct SwarmBuffers {
     double *zbuf;          // S x n: z = L^{-1} rhs_0 (or the concentrated residual)
     int *chain = nullptr;  // S x kChainStride: chained schedule (two launches per panel), or none
     int tail = 0;          // chained: the last `tail` panels are pre-updated in the launch of the one before
+    unsigned long long wait_limit = 100000000ull;   // chained: bound of a hand-over wait (100 MHz ticks)
 };
 
 void launch_build(const SwarmBuffers &b, hipStream_t st);

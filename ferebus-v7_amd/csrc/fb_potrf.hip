@@ -188,16 +188,17 @@ __device__ inline void chain_publish(int *flag, const int *consumer, int tag)
     }
 }
 
-// Consumer: wait until *flag >= tag (thread 0 polls, bounded at 1 s), then
-// drop this CU's L1 and the clean non-coherent L2 lines.  False on timeout.
-__device__ inline bool chain_wait(const int *flag, int tag, int *ok)
+// Consumer: wait until *flag >= tag (thread 0 polls, bounded at `limit`
+// ticks of the 100 MHz clock: 1 s normally), then drop this CU's L1 and the
+// clean non-coherent L2 lines.  False on timeout.
+__device__ inline bool chain_wait(const int *flag, int tag, int *ok, unsigned long long limit)
 {
     if (threadIdx.x == 0) {
         const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
         int good = 1;
         while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < tag) {
             __builtin_amdgcn_s_sleep(2);
-            if (__builtin_amdgcn_s_memrealtime() - t0 > 100000000ull) {
+            if (__builtin_amdgcn_s_memrealtime() - t0 >= limit) {
                 good = 0;
                 break;
             }
@@ -289,7 +290,8 @@ __attribute__((amdgpu_waves_per_eu(KC == 16 ? 2 : 1, KC == 16 ? 2 : 1))) panel_c
                                                                        int *__restrict__ info,
                                                                        double *__restrict__ logdet_part,
                                                                        double *__restrict__ dinv,
-                                                                       int *__restrict__ chain)
+                                                                       int *__restrict__ chain,
+                                                                       unsigned long long wait_limit)
 {
     __shared__ __attribute__((aligned(16))) double smem[chain_smem<KC, MB, NB>()];
     __shared__ int ok;
@@ -321,7 +323,7 @@ __attribute__((amdgpu_waves_per_eu(KC == 16 ? 2 : 1, KC == 16 ? 2 : 1))) panel_c
             // BC: the b1 row tile (b1, b0:b1), then the rest of the chain
             chain_announce(cw + kChainXccB, tag);
             gemm_tile_update<kMB, kNBO, KC>(A, np, c1, c0, k0, c0, smem);
-            if (!chain_wait(cw + kChainF0, tag, &ok)) {
+            if (!chain_wait(cw + kChainF0, tag, &ok, wait_limit)) {
                 if (threadIdx.x == 0) info[s] = kInfoWaitTimeout;
                 if constexpr (FUSED) chain_publish_mem(cw + kChainF2, tag);
                 return;
@@ -353,7 +355,7 @@ __attribute__((amdgpu_waves_per_eu(KC == 16 ? 2 : 1, KC == 16 ? 2 : 1))) panel_c
         } else if (role == 1) {
             chain_announce(cw + kChainXccB, tag);
             gemm_tile_update<kMB, kNBI, KC, 2, true>(A, np, c1, c0, k0, c0, smem);
-            if (!chain_wait(cw + kChainF0, tag, &ok)) {
+            if (!chain_wait(cw + kChainF0, tag, &ok, wait_limit)) {
                 if (threadIdx.x == 0) info[s] = kInfoWaitTimeout;
             } else if (info[s] == 0) {
                 trsm64_solve<false, false, FUSED>(smem, kPLD, A + (size_t)c0 * np + c1, np, A + (size_t)c0 * np + c0,
@@ -363,7 +365,7 @@ __attribute__((amdgpu_waves_per_eu(KC == 16 ? 2 : 1, KC == 16 ? 2 : 1))) panel_c
         } else {
             chain_announce(cw + kChainXccC, tag);
             gemm_tile_update<kMB, kNBI, KC, 2, true>(A, np, c1, c1, k0, c0, smem);
-            if (!chain_wait(cw + kChainF1, tag, &ok)) {
+            if (!chain_wait(cw + kChainF1, tag, &ok, wait_limit)) {
                 if (threadIdx.x == 0) info[s] = kInfoWaitTimeout;
                 if constexpr (FUSED) chain_publish_mem(cw + kChainF2, tag);
                 return;
@@ -406,7 +408,7 @@ __attribute__((amdgpu_waves_per_eu(KC == 16 ? 2 : 1, KC == 16 ? 2 : 1))) panel_c
         if constexpr (FUSED) {
             static_assert(MB == kNBI && NB == kNBO, "the fused row solve needs 64 x 128 row tiles");
             const int tag = c0 / kNBO + 1;
-            if (!chain_wait(chain + (size_t)s * kChainStride + kChainF2, tag, &ok)) {
+            if (!chain_wait(chain + (size_t)s * kChainStride + kChainF2, tag, &ok, wait_limit)) {
                 if (threadIdx.x == 0) info[s] = kInfoWaitTimeout;
                 return;
             }
@@ -459,7 +461,7 @@ static void chain_launch(const SwarmBuffers &b, int c0, int k0, int tail, hipStr
     const int nrole = chain_roles<TWO>() * ceil_div(b.S, 8);
     const int grid = nrole + 8 * ceil_div(b.S * tiles, 8) + 8 * ceil_div(b.S * tail_tiles(b.np, c0, tail), 8);
     hipLaunchKernelGGL((panel_chain_kernel<KC, MB, NB, TWO, FUSED>), dim3(grid), dim3(kThreads), 0, st, b.K, b.np, b.n, c0,
-                       k0, tail, b.S, b.info, b.logdet_part, b.dinv, b.chain);
+                       k0, tail, b.S, b.info, b.logdet_part, b.dinv, b.chain, b.wait_limit);
 }
 
 template <bool TWO>

@@ -201,3 +201,23 @@ def test_swarm_beyond_chain_limit(gpu, monkeypatch):
     ll, info = gp.swarm_loglik(th)
     pl, pinfo = gp.particle_loglik(th[::17])
     assert np.array_equal(pinfo, info[::17]) and np.array_equal(pl[pinfo == 0], ll[::17][pinfo == 0])
+
+
+def test_chain_wait_bound_reported(gpu, monkeypatch):
+    """A hand-over wait of the chained schedule that reaches its bound ends
+    the kernel normally and the evaluation returns an error instead of a
+    result: forced from the host with a bound of 0 ticks (the consumer gives
+    up at its first poll, before its producer has factored its block).  A
+    handle with the default bound then evaluates normally."""
+    monkeypatch.setenv("FB_FACTOR_PATH", "2")
+    _, x, y, prog, mfit = _problem(400, 6, seed=12)
+    monkeypatch.setenv("FB_CHAIN_WAIT_TICKS", "0")
+    gp = _gp(x, y, prog, mfit, 4)
+    th = np.random.default_rng(8).uniform(0.0, 1.0, (4, gp.nparams))
+    with pytest.raises(RuntimeError, match="bound"):
+        gp.swarm_loglik(th)
+    gp.close()
+    monkeypatch.delenv("FB_CHAIN_WAIT_TICKS")
+    gp = _gp(x, y, prog, mfit, 4)
+    ll, info = gp.swarm_loglik(th)
+    assert (info == 0).all() and np.isfinite(ll).all()
