@@ -120,7 +120,25 @@ def uneven_run(steps: int = 3):
     return out
 
 
-def converge_runs(gp, S, cap):
+def nma_converge(cap: int):
+    """The same comparison on the NMA-like gradient workload: N=3000, d=30,
+    ARD rbf (30) + periodic (9 dihedrals) + optimised nugget searched in
+    [1e-12, 1e-4], swarm 32; its schedule, 5 L-BFGS steps on gbest every 10
+    iterations, first."""
+    from ferebus_amd import SwarmLikelihood, compile_kernel, fit_mean, synthetic
+    w = synthetic.nma(n=3000, seed=3)
+    prog = compile_kernel(w.kernel, w.kernels, w.x.shape[1])
+    gp = SwarmLikelihood(w.x, w.y, prog, mean=fit_mean(w.mean, w.x, w.y), noise=w.noise, optimise_noise=True,
+                         likelihood=w.likelihood, max_swarm=32)
+    try:
+        out = converge_runs(gp, 32, cap, schedules=((10, 5), (10, 20)), noise_max=w.extra["noise_search"][1])
+    finally:
+        gp.close()
+    out["workload"] = "NMA-like N=3000 d=30, ARD rbf + periodic + optimised nugget in [1e-12, 1e-4], swarm 32"
+    return out
+
+
+def converge_runs(gp, S, cap, schedules=((10, 20), (10, 5)), **settings):
     """Wall time to the converged logL of the swarm search on this workload.
 
     Plain PSO (the reference's optimiser, seed 1) runs until the relative
@@ -133,7 +151,8 @@ def converge_runs(gp, S, cap):
     from ferebus_amd.lbfgs import LbfgsSettings
     from ferebus_amd.pso import PsoSettings, run_pso
 
-    base = dict(swarm_size=S, iterations=cap, relative_change=True, tolerance=1e-8, stall_iterations=50, seed=1)
+    base = dict(swarm_size=S, iterations=cap, relative_change=True, tolerance=1e-8, stall_iterations=50, seed=1,
+                **settings)
     plain = run_pso(gp, PsoSettings(**base))
     target = plain.best_cost
     out = {
@@ -146,7 +165,7 @@ def converge_runs(gp, S, cap):
     }
     # polishing schedules: every 10 iterations, 20 or 5 L-BFGS steps (trial
     # steps evaluated 4 at a time)
-    for every, steps in ((10, 20), (10, 5)):
+    for every, steps in schedules:
         pol = run_pso(gp, PsoSettings(**base, lbfgs_every=every, lbfgs=LbfgsSettings(steps=steps)),
                       target_cost=target)
         reached = next((t for it, t, c in pol.history if c <= target), None)
@@ -314,6 +333,7 @@ def main():
 
     if not args.no_converge:
         result["converged"] = converge_runs(gp, S, args.converge_cap)
+        result["converged_nma"] = nma_converge(args.converge_cap)
 
     if not args.no_uneven:
         result["batched_uneven"] = uneven_run()

@@ -97,6 +97,7 @@ program ferebus_amd
       if (rc /= FB_OK) error stop "ferebus_amd: " // fb_error_message()
       gp%search_min = cfg%search_min
       gp%search_max = cfg%search_max
+      gp%noise_max = cfg%noise_max
       print '(A)', "running pso"
       call run_pso(gp, pso, res)
       if (res%status /= 0) error stop "ferebus_amd: " // res%message

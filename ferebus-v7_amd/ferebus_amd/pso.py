@@ -49,6 +49,7 @@ class PsoSettings:
     max_retries: int = 10000
     search_min: float = 0.0                    # optimiser.f90(config):35-38
     search_max: float = 1.0
+    noise_max: float = 1e-6                    # optimised nugget's upper bound (gpr.f90:229, 242 hard-code 1e-6)
     lbfgs_every: int = 0
     lbfgs: LbfgsSettings = field(default_factory=LbfgsSettings)
 
@@ -141,11 +142,12 @@ class DeviceSwarm:
 def search_bounds(gp, cfg: PsoSettings) -> Tuple[np.ndarray, np.ndarray]:
     """search_min_gpr / search_max_gpr (gpr.f90:204-246): the kernel
     parameters in [search_min, search_max], an optimised nugget in
-    [1e-12, 1e-6] (gpr.f90:229, 242)."""
+    [1e-12, noise_max] (the reference fixes noise_max = 1e-6, gpr.f90:229,
+    242)."""
     lo = np.full(gp.nparams, cfg.search_min)
     hi = np.full(gp.nparams, cfg.search_max)
     if gp.optimise_noise:
-        lo[-1], hi[-1] = 1e-12, 1e-6
+        lo[-1], hi[-1] = 1e-12, cfg.noise_max
     return lo, hi
 
 

@@ -6,7 +6,9 @@
 !>     (src/config/system/system.f90:52-122; nfeats = 3*natoms - 6)
 !>   [model] mean="constant", optimiser="pso", kernel, noise=1e-10,
 !>     optimise_noise=false, likelihood="marginal", standardise=false
-!>     (src/config/model_config.f90:47-116)
+!>     (src/config/model_config.f90:47-116); noise_max=1e-6, the upper end of
+!>     the optimised nugget's search box (hard-coded in the reference,
+!>     src/model/gpr/gpr.f90:229,242; a key of this build)
 !>   [optimiser] search_min=0, search_max=1 (src/config/optimiser/optimiser.f90:35-38)
 !>   [optimiser.pso] swarm_size=50, iterations=1000, inertia_weight=0.789,
 !>     cognitive/social_learning_rate=1.494, stopping_criteria
@@ -57,6 +59,7 @@ module fb_config
     character(len=:), allocatable :: mean, kernel, likelihood
     real(wp) :: noise = 1e-10_wp
     logical :: optimise_noise = .false.
+    real(wp) :: noise_max = 1e-6_wp
     real(wp) :: search_min = 0.0_wp, search_max = 1.0_wp
     integer :: swarm_size = 50, iterations = 1000
     real(wp) :: inertia_weight = 0.789_wp, cognitive = 1.494_wp, social = 1.494_wp
@@ -358,6 +361,7 @@ contains
     cfg%likelihood = doc%get_str("model.likelihood", "marginal")
     cfg%noise = doc%get_real("model.noise", 1e-10_wp)
     cfg%optimise_noise = doc%get_bool("model.optimise_noise", .false.)
+    cfg%noise_max = doc%get_real("model.noise_max", 1e-6_wp)
     cfg%search_min = doc%get_real("optimiser.search_min", 0.0_wp)
     cfg%search_max = doc%get_real("optimiser.search_max", 1.0_wp)
     cfg%swarm_size = doc%get_int("optimiser.pso.swarm_size", 50)
