@@ -59,3 +59,35 @@ def test_ibuprofen_full_size(gpu):
         rel = abs(ll[s] - ref) / abs(ref)
         print(f"particle {s}: gpu {ll[s]:.15e} lapack {ref:.15e} rel {rel:.2e} (orderings gap {gap:.2e})")
         assert linfo == 0 and rel <= max(1e-9, 10.0 * gap)
+
+
+def test_ibuprofen_gradient_and_polish(gpu):
+    """The same workload's adjoint gradient (concentrated likelihood, linear
+    mean, 93 parameters) against central finite differences on a subset of
+    the parameters, and one L-BFGS polish of a particle lowering its cost."""
+    from test_gpu_gradient import FD_RTOL
+    from ferebus_amd import lbfgs
+    w = synthetic.ibuprofen(n=6000, seed=4)
+    d = w.x.shape[1]
+    prog = compile_kernel(w.kernel, w.kernels, d)
+    gp = SwarmLikelihood(w.x, w.y, prog, mean=fit_mean(w.mean, w.x, w.y), noise=w.noise,
+                         likelihood=w.likelihood, max_swarm=4)
+    rng = np.random.default_rng(11)
+    th = rng.uniform(0.0, 1.0, (1, gp.nparams)) + 2.0      # short lengthscales: conditioned
+    ll, g, info = gp.swarm_grad(th)
+    print("ibuprofen gradient info", info.tolist(), "ll", ll.tolist())
+    assert info[0] == 0 and np.isfinite(g).all()
+    t = th[0]
+    for q in (0, 31, 62, 63, 92):
+        h = max(1e-6 * abs(t[q]), 1e-7)
+        tp, tm = t.copy(), t.copy()
+        tp[q] += h
+        tm[q] -= h
+        fd = (gp.swarm_loglik(tp)[0][0] - gp.swarm_loglik(tm)[0][0]) / (2 * h)
+        err = abs(fd - g[0, q]) / max(np.abs(g[0]).max(), 1e-300)
+        print(f"ibuprofen param {q}: adjoint {g[0, q]:.6e} fd {fd:.6e} rel {err:.2e}")
+        assert err <= FD_RTOL
+    xp, fp, evals, accepted = lbfgs.polish(gp, t, lbfgs.LbfgsSettings(steps=2))
+    print(f"polish: cost {-ll[0]:.10e} -> {fp:.10e} in {accepted} steps, {evals} evaluations")
+    assert accepted >= 1 and fp < -ll[0]
+    gp.close()
