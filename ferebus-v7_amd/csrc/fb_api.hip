@@ -173,6 +173,7 @@ struct fb_handle {
     int gemm_kc = 32;                // lock-step trailing-update chunk depth
     bool chain = true;               // lock-step with 16-deep chunks: two launches per panel (FB_CHAIN=0: three)
     int tail = 3;                    // chained: trailing panels pre-updated by the launch before them (FB_TAIL)
+    int lookahead = 0;               // chained: each update launch also pre-updates the next panel (FB_LOOKAHEAD)
     unsigned long long wait_limit = 100000000ull;   // chained: hand-over wait bound (FB_CHAIN_WAIT_TICKS)
     int *d_chain = nullptr;          // per-particle words of the two-launch schedule
     int tile = 0;                    // lock-step panel-update tile shape (16-deep chunks)
@@ -244,6 +245,7 @@ fb::SwarmBuffers buffers(fb_handle *h, int g0, int S, const double *theta, doubl
     b.chain = (h->chain && h->gemm_kc == 16 && S <= fb::kChainMaxSwarm)
                   ? h->d_chain + (size_t)g0 * fb::kChainStride : nullptr;
     b.tail = b.chain ? h->tail : 0;
+    b.lookahead = b.chain ? h->lookahead : 0;
     b.wait_limit = h->wait_limit;
     return b;
 }
@@ -643,6 +645,7 @@ int fb_create(fb_handle **out, int n, int d, const double *x, const double *y, c
     if (const char *env = std::getenv("FB_AUTOTUNE")) h->autotune = std::atoi(env) != 0;
     if (const char *env = std::getenv("FB_CHAIN")) h->chain = std::atoi(env) != 0;
     if (const char *env = std::getenv("FB_TAIL")) h->tail = std::max(0, std::atoi(env));
+    if (const char *env = std::getenv("FB_LOOKAHEAD")) h->lookahead = std::atoi(env) != 0;
     // a shorter bound for the hand-over waits (a test of the timeout path:
     // 0 gives up at the first poll)
     if (const char *env = std::getenv("FB_CHAIN_WAIT_TICKS")) h->wait_limit = std::strtoull(env, nullptr, 10);

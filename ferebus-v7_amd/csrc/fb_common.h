@@ -151,6 +151,7 @@ struct SwarmBuffers {
     double *zbuf;          // S x n: z = L^{-1} rhs_0 (or the concentrated residual)
     int *chain = nullptr;  // S x kChainStride: chained schedule (two launches per panel), or none
     int tail = 0;          // chained: the last `tail` panels are pre-updated in the launch of the one before
+    int lookahead = 0;     // chained: every update launch also pre-updates the next panel
     unsigned long long wait_limit = 100000000ull;   // chained: bound of a hand-over wait (100 MHz ticks)
 };
 

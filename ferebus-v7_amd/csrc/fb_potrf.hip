@@ -286,7 +286,7 @@ __host__ __device__ inline int tail_tiles(int np, int c0, int tail)
 template <int KC, int MB, int NB, bool TWO, bool FUSED>
 __global__ void __launch_bounds__(kThreads)
 __attribute__((amdgpu_waves_per_eu(KC == 16 ? 2 : 1, KC == 16 ? 2 : 1))) panel_chain_kernel(double *__restrict__ K, int np, int n,
-                                                                       int c0, int k0, int tail, int S,
+                                                                       int c0, int k0, int kt, int tail, int S,
                                                                        int *__restrict__ info,
                                                                        double *__restrict__ logdet_part,
                                                                        double *__restrict__ dinv,
@@ -432,8 +432,8 @@ __attribute__((amdgpu_waves_per_eu(KC == 16 ? 2 : 1, KC == 16 ? 2 : 1))) panel_c
     }
     if (info[s] != 0) return;
     double *A = K + (size_t)s * np * np;
-    if (v == 0) gemm_tile_update<kMB, kNBI, KC>(A, np, cp, cp, k0, c0, smem);
-    else gemm_tile_update<kMB, kNBO, KC>(A, np, cp + kNBI * v, cp, k0, c0, smem);
+    if (v == 0) gemm_tile_update<kMB, kNBI, KC>(A, np, cp, cp, kt, c0, smem);
+    else gemm_tile_update<kMB, kNBO, KC>(A, np, cp + kNBI * v, cp, kt, c0, smem);
 }
 
 // Launch 2 of panel c0, chained: every row tile below the panel is solved
@@ -455,22 +455,47 @@ __global__ void __launch_bounds__(kThreads) solve_rows_kernel(double *__restrict
 }
 
 template <int KC, int MB, int NB, bool TWO, bool FUSED = false>
-static void chain_launch(const SwarmBuffers &b, int c0, int k0, int tail, hipStream_t st)
+static void chain_launch(const SwarmBuffers &b, int c0, int k0, int kt, int tail, hipStream_t st)
 {
     const int tiles = ((b.np - c0 - kNBO) / MB) * (kNBO / NB);
     const int nrole = chain_roles<TWO>() * ceil_div(b.S, 8);
     const int grid = nrole + 8 * ceil_div(b.S * tiles, 8) + 8 * ceil_div(b.S * tail_tiles(b.np, c0, tail), 8);
     hipLaunchKernelGGL((panel_chain_kernel<KC, MB, NB, TWO, FUSED>), dim3(grid), dim3(kThreads), 0, st, b.K, b.np, b.n, c0,
-                       k0, tail, b.S, b.info, b.logdet_part, b.dinv, b.chain, b.wait_limit);
+                       k0, kt, tail, b.S, b.info, b.logdet_part, b.dinv, b.chain, b.wait_limit);
 }
 
 template <bool TWO>
-static void chain_tiles(const SwarmBuffers &b, int tile, int c0, int k0, int tail, hipStream_t st)
+static void chain_tiles(const SwarmBuffers &b, int tile, int c0, int k0, int kt, int tail, hipStream_t st)
 {
-    if (tile & kTileFused) chain_launch<16, 64, 128, TWO, true>(b, c0, k0, tail, st);
-    else if (tile == 1) chain_launch<16, 32, 128, TWO>(b, c0, k0, tail, st);
-    else if (tile == 2) chain_launch<16, 32, 64, TWO>(b, c0, k0, tail, st);
-    else chain_launch<16, 64, 128, TWO>(b, c0, k0, tail, st);
+    if (tile & kTileFused) chain_launch<16, 64, 128, TWO, true>(b, c0, k0, kt, tail, st);
+    else if (tile == 1) chain_launch<16, 32, 128, TWO>(b, c0, k0, kt, tail, st);
+    else if (tile == 2) chain_launch<16, 32, 64, TWO>(b, c0, k0, kt, tail, st);
+    else chain_launch<16, 64, 128, TWO>(b, c0, k0, kt, tail, st);
+}
+
+// Pre-update schedule of the chained update launches: launch p applies
+// columns k0 .. c0p to its own panel (the first k0 were applied by earlier
+// launches) and columns kt .. c0p to the next `tail` panels.  Launch
+// npan - 1 - b.tail pre-updates the last b.tail panels; with b.lookahead
+// every other launch pre-updates the next panel.  Each panel's columns are
+// applied in increasing k across launches, so the bits do not change.
+static void chain_schedule(const SwarmBuffers &b, int p, int &k0, int &kt, int &tail)
+{
+    const int npan = b.np / kNBO, q = npan - 1 - b.tail;
+    int k[1024];
+    const int np_ = npan < 1024 ? npan : 1024;
+    for (int r = 0; r < np_; ++r) k[r] = 0;
+    for (int r = 0; r <= p && r < np_; ++r) {
+        int t = 0;
+        if (b.tail > 0 && q >= 1 && r == q) t = b.tail;
+        else if (b.lookahead && r >= 1 && r + 1 < npan) t = 1;
+        if (r == p) {
+            k0 = k[r];
+            kt = r + 1 < np_ ? k[r + 1] : 0;
+            tail = t;
+        }
+        for (int j = 1; j <= t && r + j < np_; ++j) k[r + j] = r * kNBO;
+    }
 }
 
 // 16-deep chunks only (the 32-deep update tile leaves no room for the roles).
@@ -479,14 +504,11 @@ static void chain_tiles(const SwarmBuffers &b, int tile, int c0, int k0, int tai
 void launch_panel_chain(const SwarmBuffers &b, int c0, hipStream_t st)
 {
     const int code = b.panel_tile ? b.panel_tile[c0 / kNBO] : b.tile;
-    // tail pre-update: panel q = npan - 1 - tail applies its columns 0 .. c0q
-    // to the last `tail` panels as well, which then start from c0q
-    const int npan = b.np / kNBO, q = npan - 1 - b.tail;
-    const int k0 = (b.tail > 0 && q >= 1 && c0 > q * kNBO) ? q * kNBO : 0;
-    const int tail = (b.tail > 0 && q >= 1 && c0 == q * kNBO) ? b.tail : 0;
+    int k0 = 0, kt = 0, tail = 0;
+    chain_schedule(b, c0 / kNBO, k0, kt, tail);
     const int tile = (code & kTileFused) ? kTileFused : (code & 3);
-    if (code & kTileTwoRoles) chain_tiles<true>(b, tile, c0, k0, tail, st);
-    else chain_tiles<false>(b, tile, c0, k0, tail, st);
+    if (code & kTileTwoRoles) chain_tiles<true>(b, tile, c0, k0, kt, tail, st);
+    else chain_tiles<false>(b, tile, c0, k0, kt, tail, st);
 }
 
 // The panel's update launch also solved its rows (kTileFused): no row-solve

@@ -173,16 +173,18 @@ def test_chained_schedule_tile_shapes(gpu, monkeypatch, path, tiles):
     assert np.array_equal(out[0][1], out[1][1]) and np.array_equal(out[0][0], out[1][0])
 
 
-@pytest.mark.parametrize("tail", [1, 5, 7])
-def test_tail_pre_update_bits(gpu, monkeypatch, tail):
+@pytest.mark.parametrize("tail,lookahead", [(1, 0), (5, 0), (7, 0), (0, 1), (3, 1), (7, 1)])
+def test_tail_pre_update_bits(gpu, monkeypatch, tail, lookahead):
     """The chained schedule with 1, 5 and 7 trailing panels pre-updated by the
     launch before them (9 panels here; 7 leaves panel 1 carrying all of
-    them): the bits of the three-launch schedule."""
-    _, x, y, prog, mfit = _problem(1100, 6, seed=tail)
+    them), alone or with every launch also pre-updating the next panel
+    (look-ahead): the bits of the three-launch schedule."""
+    _, x, y, prog, mfit = _problem(1100, 6, seed=tail + 10 * lookahead)
     out = {}
     for chain in (0, 1):
         monkeypatch.setenv("FB_CHAIN", str(chain))
         monkeypatch.setenv("FB_TAIL", str(tail))
+        monkeypatch.setenv("FB_LOOKAHEAD", str(lookahead))
         monkeypatch.setenv("FB_FACTOR_PATH", "2")
         gp = _gp(x, y, prog, mfit, 5)
         th = np.random.default_rng(6).uniform(0.0, 1.0, (5, gp.nparams))
