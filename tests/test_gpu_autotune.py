@@ -119,20 +119,30 @@ def _dup_case(n, dup, src, seed):
     return x, y, prog, fit_mean("constant", x, y)
 
 
-@pytest.mark.parametrize("n,dup,src,S", [
-    (700, 200, 5, 9),     # fails in the second 64-block of panel 1 (columns 192..255)
-    (700, 300, 17, 9),    # fails in the first 64-block of panel 2 (columns 256..319)
-    (383, 130, 2, 1),     # single particle, bordered rows in the last panel
+@pytest.mark.parametrize("n,dup,src,S,code", [
+    (700, 200, 5, 9, 0),     # fails in the second 64-block of panel 1 (columns 192..255)
+    (700, 300, 17, 9, 0),    # fails in the first 64-block of panel 2 (columns 256..319)
+    (383, 130, 2, 1, 0),     # single particle, bordered rows in the last panel
+    (700, 200, 5, 9, 4),     # the same failures with two role work-groups,
+    (700, 300, 17, 9, 4),
+    (700, 200, 5, 9, 8),     # with the fused row solve (three and two roles)
+    (700, 300, 17, 9, 12),
 ])
-def test_chained_schedule_bit_identical(gpu, monkeypatch, n, dup, src, S):
+def test_chained_schedule_bit_identical(gpu, monkeypatch, n, dup, src, S, code):
     """Two launches per panel (the diagonal chain handed over inside the
-    update launch) against three: same bits, same INFO, the failing
-    particles included."""
+    update launch; code 4: two role work-groups, 8 / 12: the row solve fused
+    into the update launch) against three: same bits, same INFO, the failing
+    particles included (every role must publish on every exit path)."""
     x, y, prog, mfit = _dup_case(n, dup, src, seed=n + dup)
     out = {}
+    npan = (n + 1 + 127) // 128
     for chain in (0, 1):
         monkeypatch.setenv("FB_CHAIN", str(chain))
-        monkeypatch.setenv("FB_FACTOR_PATH", "2")
+        if code:
+            monkeypatch.setenv("FB_FACTOR_PATH", "5")
+            monkeypatch.setenv("FB_PANEL_TILES", ",".join([str(code if chain else 0)] * npan))
+        else:
+            monkeypatch.setenv("FB_FACTOR_PATH", "2")
         gp = SwarmLikelihood(x, y, prog, mean=mfit, noise=1e-6, optimise_noise=True, max_swarm=S)
         th = np.random.default_rng(3).uniform(0.0, 1.0, (S, gp.nparams))
         th[:, -1] = np.where(np.arange(S) % 3 == 1, -1e-8, 1e-4)
