@@ -3,10 +3,12 @@
 // Host side of the drop-in boundary: validates the kernel specification,
 // expands it into the device KernelProgram, owns every device buffer of a
 // training set, and enqueues the swarm pipeline
-//   hyper + build -> per 128-wide panel { gemm_update, potrf64, trsm64,
-//   gemm_update, potrf64, trsm64 } -> likelihood
-// on one HIP stream.  The batched path replays a captured hipGraph of that
-// pipeline; the plain path launches it eagerly, one particle at a time.
+//   hyper + build -> factorisation (per 128-wide panel: the chained update
+//   and row-solve launches, or the three-launch schedule, or one persistent
+//   launch for the whole swarm) -> likelihood
+// on one HIP stream; the autotuner (choose_path) picks the factorisation per
+// swarm size.  The batched path replays a captured hipGraph of that pipeline;
+// the plain path launches it eagerly, one particle at a time.
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>

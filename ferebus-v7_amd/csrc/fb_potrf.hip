@@ -6,7 +6,10 @@
 //
 // Left-looking blocked algorithm with two block levels.  Panels of width
 // kNBO = 128 carry the dominant trailing update; inside a panel two blocks of
-// kNBI = 64 (b0 = c0, b1 = c0 + 64) are factored.  Three launches per panel:
+// kNBI = 64 (b0 = c0, b1 = c0 + 64) are factored.  Two schedules: the chained
+// one (two launches per panel, or one with the row solve fused; below, the
+// default for 16-deep chunks and swarms up to kChainMaxSwarm), and three
+// launches per panel:
 //   1. C(c0:, c0:c0+128) -= L(c0:, 0:c0) L(c0:c0+128, 0:c0)^T (FP64 MFMA
 //      16x16x4, LDS-staged); the first work-groups update the diagonal block
 //      b0 on chip and factor it (INFO, partial log-det, 16x16 inverses)
