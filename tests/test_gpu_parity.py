@@ -138,6 +138,19 @@ def test_kernel_compositions(gpu, name):
     _check(ll, info, oll, oinfo, RTOL, f"kernel {name}", _lapack(x, y, m, prog, th, 1e-6, False, "marginal"))
 
 
+@pytest.mark.parametrize("name", ["product", "nested", "scaled", "linear", "cyclic_sum"])
+def test_kernel_compositions_multi_panel(gpu, name):
+    """The same compositions at N = 700 (six 128-column panels: the chained
+    schedule's roles, row solves and tail pre-update all in play), autotuned."""
+    expr, ks = KERNEL_CASES[name]
+    x, y, prog, m, gp, th = _case(700, 6, seed=17, kernel=expr, kernels=ks, S=4)
+    if name == "scaled":
+        th[:, 0] = np.abs(th[:, 0]) + 0.5
+    ll, info = gp.swarm_loglik(th)
+    oll, oinfo = _oracle(x, y, m, prog, th, 1e-6, False, "marginal")
+    _check(ll, info, oll, oinfo, RTOL, f"kernel {name} N=700", _lapack(x, y, m, prog, th, 1e-6, False, "marginal"))
+
+
 @pytest.mark.parametrize("mean", ["zero", "constant", "linear", "quadratic"])
 def test_means(gpu, mean):
     x, y, prog, m, gp, th = _case(120, 6, seed=11, mean=mean)
