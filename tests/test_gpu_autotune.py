@@ -233,3 +233,22 @@ def test_chain_wait_bound_reported(gpu, monkeypatch):
     gp = _gp(x, y, prog, mfit, 4)
     ll, info = gp.swarm_loglik(th)
     assert (info == 0).all() and np.isfinite(ll).all()
+
+
+def test_chained_schedule_stream_groups(gpu, monkeypatch):
+    """The swarm split into groups on concurrent streams (FB_STREAMS), each
+    group running its own chained schedule and hand-overs: the bits of one
+    group, graph replay included."""
+    _, x, y, prog, mfit = _problem(700, 6, seed=21)
+    monkeypatch.setenv("FB_FACTOR_PATH", "2")
+    out = {}
+    for streams in (1, 3):
+        monkeypatch.setenv("FB_STREAMS", str(streams))
+        gp = _gp(x, y, prog, mfit, 11)
+        th = np.random.default_rng(9).uniform(0.0, 1.0, (11, gp.nparams))
+        ll, info = gp.swarm_loglik(th)
+        ll2, info2 = gp.swarm_loglik(th)
+        assert np.array_equal(ll, ll2) and np.array_equal(info, info2)
+        out[streams] = (ll, info)
+        gp.close()
+    assert np.array_equal(out[1][1], out[3][1]) and np.array_equal(out[1][0], out[3][0])
