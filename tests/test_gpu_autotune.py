@@ -202,17 +202,20 @@ def test_tail_pre_update_bits(gpu, monkeypatch, tail, lookahead):
     assert np.array_equal(out[0][1], out[1][1]) and np.array_equal(out[0][0], out[1][0])
 
 
-def test_swarm_beyond_chain_limit(gpu, monkeypatch):
+@pytest.mark.parametrize("S", [160, 170])
+def test_swarm_beyond_chain_limit(gpu, monkeypatch, S):
     """Swarms above 160 particles take the three-launch schedule (the
-    chained one needs every waiting work-group resident); 170 particles give
-    the bits of 170 single-particle evaluations."""
+    chained one needs its waiting work-groups to leave room for the
+    producers); 160 particles (the most waiting work-groups the chained
+    schedule allows) and 170 give the bits of single-particle evaluations."""
     monkeypatch.setenv("FB_FACTOR_PATH", "2")
-    _, x, y, prog, mfit = _problem(200, 6, seed=11)
-    gp = _gp(x, y, prog, mfit, 170)
-    th = np.random.default_rng(7).uniform(0.0, 1.0, (170, gp.nparams))
+    _, x, y, prog, mfit = _problem(300, 6, seed=11)
+    gp = _gp(x, y, prog, mfit, S)
+    th = np.random.default_rng(7).uniform(0.0, 1.0, (S, gp.nparams))
     ll, info = gp.swarm_loglik(th)
     pl, pinfo = gp.particle_loglik(th[::17])
     assert np.array_equal(pinfo, info[::17]) and np.array_equal(pl[pinfo == 0], ll[::17][pinfo == 0])
+    assert (info == 0).sum() > S // 2
 
 
 def test_chain_wait_bound_reported(gpu, monkeypatch):
