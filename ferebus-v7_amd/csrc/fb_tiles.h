@@ -56,7 +56,9 @@ struct TileShape {
 // the accumulators: no LDS epilogue, no barrier after the loop.  K is
 // streamed in chunks of KC through two LDS buffers; the global loads of a
 // chunk are issued two chunks ahead (two register sets), so each has two
-// chunks of matrix-core work to land.
+// chunks of matrix-core work to land.  For -= the A chunk is negated as it
+// is staged, so the inner loop passes LDS reads straight to the matrix cores
+// (the products see the same operands either way).
 // ZERO_C: C starts at zero (not loaded); PLUS: C += A B^T instead of -=.
 template <int MB, int NB, int KC, int DIST = 2, bool TO_LDS = false, bool ZERO_C = false, bool PLUS = false>
 __device__ inline void tile_gemm_sub(double *__restrict__ Cg, int ldc,
@@ -123,7 +125,8 @@ __device__ inline void tile_gemm_sub(double *__restrict__ Cg, int ldc,
             double *an = smem + (BUF) * T::ASZ;                                                 \
             double *bn = smem + 2 * T::ASZ + (BUF) * T::BSZ;                                    \
             _Pragma("unroll") for (int u = 0; u < NA; ++u)                                      \
-                *reinterpret_cast<double2 *>(an + a_lds[u]) = double2{r##SET##ax[u], r##SET##ay[u]}; \
+                *reinterpret_cast<double2 *>(an + a_lds[u]) =                                   \
+                    PLUS ? double2{r##SET##ax[u], r##SET##ay[u]} : double2{-r##SET##ax[u], -r##SET##ay[u]}; \
             _Pragma("unroll") for (int u = 0; u < NBL; ++u)                                     \
                 *reinterpret_cast<double2 *>(bn + b_lds[u]) = double2{r##SET##bx[u], r##SET##by[u]}; \
         } while (0)
@@ -134,10 +137,7 @@ __device__ inline void tile_gemm_sub(double *__restrict__ Cg, int ldc,
             for (int kk = 0; kk < KC / 4; ++kk) {
                 double af[T::BM], bf[T::BN];
 #pragma unroll
-                for (int i = 0; i < T::BM; ++i) {
-                    const double a = as[(kk * 4 + lq) * T::LDA + wrow + i * 16 + lr];
-                    af[i] = PLUS ? a : -a;
-                }
+                for (int i = 0; i < T::BM; ++i) af[i] = as[(kk * 4 + lq) * T::LDA + wrow + i * 16 + lr];
 #pragma unroll
                 for (int j = 0; j < T::BN; ++j) bf[j] = bs[(kk * 4 + lq) * T::LDB + wcol + j * 16 + lr];
 #pragma unroll
@@ -161,17 +161,24 @@ __device__ inline void tile_gemm_sub(double *__restrict__ Cg, int ldc,
         } else {
         if (nchunks > 1) FB_LOAD(1, 1);
         __syncthreads();
-        for (int c = 0; c < nchunks; c += 2) {
-            // chunk c in LDS buffer 0, chunk c + 1 in flight in set 1
+        // pairs of chunks, one loop exit: an exit between the two halves
+        // makes the compiler keep a second accumulator set and copy it
+        int c = 0;
+        for (; c + 1 < nchunks; c += 2) {
+            // chunk c in LDS buffer 0, chunk c + 1 in register set 1
             if (c + 2 < nchunks) FB_LOAD(0, c + 2);
             compute(0);
-            if (c + 1 < nchunks) FB_STORE(1, 1);
+            FB_STORE(1, 1);
             __syncthreads();
-            if (c + 1 >= nchunks) break;
-            // chunk c + 1 in LDS buffer 1, chunk c + 2 in flight in set 0
+            // chunk c + 1 in LDS buffer 1, chunk c + 2 in register set 0
             if (c + 3 < nchunks) FB_LOAD(1, c + 3);
             compute(1);
             if (c + 2 < nchunks) FB_STORE(0, 0);
+            __syncthreads();
+        }
+        if (c < nchunks) {
+            // an odd count: the last chunk is in LDS buffer 0
+            compute(0);
             __syncthreads();
         }
         }
