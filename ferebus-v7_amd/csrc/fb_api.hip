@@ -494,6 +494,17 @@ struct TuneRecord {
     std::string tiles;     // per-panel shapes of the per-panel path, "0,2,1,..."
 };
 
+// Per-panel tile codes the lock-step launchers accept: a shape (0 64x128,
+// 1 32x128, 2 32x64), + kTileTwoRoles; kTileFused and kTileThreePerCU are
+// 64x128 only.
+static bool valid_tile_code(int t)
+{
+    const int shape = t & 3, var = t & ~(3 | fb::kTileTwoRoles);
+    if (t < 0 || shape == 3) return false;
+    if (var == 0) return true;
+    return shape == 0 && (var == fb::kTileFused || var == fb::kTileThreePerCU);
+}
+
 static std::vector<TuneRecord> read_tune_file(const std::string &path)
 {
     std::vector<TuneRecord> out;
@@ -656,7 +667,8 @@ int fb_create(fb_handle **out, int n, int d, const double *x, const double *y, c
         if (m >= 0 && m < kNumModes) h->forced_mode = m;
         // the per-panel path with its shapes given: FB_PANEL_TILES=t0,t1,...
         // (one of 0 64x128, 1 32x128, 2 32x64 per 128-column panel, + 4 for
-        // two role work-groups in the chained update launch)
+        // two role work-groups in the chained update launch; 8 fused, 16
+        // three work-groups per CU, each 64x128, + 4)
         const char *tl = std::getenv("FB_PANEL_TILES");
         if (m == kModePerPanel && tl) {
             std::vector<signed char> v;
@@ -666,7 +678,7 @@ int fb_create(fb_handle **out, int n, int d, const double *x, const double *y, c
                 if (*c == ',') ++c;
             }
             bool ok = (int)v.size() == h->np / fb::kNBO;
-            for (signed char t : v) ok = ok && (t == 0 || t == 1 || t == 2 || t == 4 || t == 5 || t == 6 || t == 8 || t == 12);
+            for (signed char t : v) ok = ok && valid_tile_code(t);
             if (ok) {
                 h->pinned_tiles = v;
                 h->forced_mode = kModePerPanel;
@@ -690,7 +702,7 @@ int fb_create(fb_handle **out, int n, int d, const double *x, const double *y, c
                         if (*c == ',') ++c;
                     }
                     bool ok = (int)v.size() == h->np / fb::kNBO;
-                    for (signed char t : v) ok = ok && (t == 0 || t == 1 || t == 2 || t == 4 || t == 5 || t == 6 || t == 8 || t == 12);
+                    for (signed char t : v) ok = ok && valid_tile_code(t);
                     if (!ok) continue;
                     h->panel_tiles[r.swarm] = v;
                 }
@@ -841,6 +853,8 @@ static int choose_path(fb_handle *h, int s)
         for (int t = 0; t < 3; ++t) codes.push_back(t | fb::kTileTwoRoles);
         codes.push_back(fb::kTileFused);
         codes.push_back(fb::kTileFused | fb::kTileTwoRoles);
+        codes.push_back(fb::kTileThreePerCU);
+        codes.push_back(fb::kTileThreePerCU | fb::kTileTwoRoles);
     }
     const int nc = (int)codes.size();
     std::vector<std::vector<float>> tpan(nc, std::vector<float>(npan, 1e30f));
@@ -873,7 +887,7 @@ static int choose_path(fb_handle *h, int s)
         mix[k] = (signed char)codes[bt];
         mixed = mixed || mix[k] != mix[0];
     }
-    if (mixed || (mix[0] & (fb::kTileTwoRoles | fb::kTileFused))) {
+    if (mixed || (mix[0] & (fb::kTileTwoRoles | fb::kTileFused | fb::kTileThreePerCU))) {
         h->panel_tiles[s] = mix;
         set_mode(h, kModePerPanel, s);
         float ms_best = 1e30f;

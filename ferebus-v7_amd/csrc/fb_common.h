@@ -123,6 +123,11 @@ constexpr int kTileTwoRoles = 4;
 // its rows once the panel's diagonal chain is published -- one launch for
 // the panel instead of two.
 constexpr int kTileFused = 8;
+// Panel tile code bit (64 x 128 row tiles, not fused): the chained update
+// launch in 8-deep chunks, whose smaller register and LDS footprint holds
+// three work-groups per CU instead of two (more resident row tiles for
+// launches of more than 512 work-groups; the same bits).
+constexpr int kTileThreePerCU = 16;
 
 inline __host__ __device__ int ceil_div(int a, int b) { return (a + b - 1) / b; }
 inline __host__ __device__ int round_up(int a, int b) { return ceil_div(a, b) * b; }
@@ -140,7 +145,8 @@ struct SwarmBuffers {
     const KernelProgram *prog_host;   // host copy (passed by value to kernels whose loops read it)
     int gemm_kc = 32;      // chunk depth of the lock-step trailing update (16 or 32)
     int tile = 0;          // lock-step panel-update tiles: 0 64x128, 1 32x128, 2 32x64 (16-deep chunks),
-                           // | kTileTwoRoles: two role work-groups in the chained update launch
+                           // | kTileTwoRoles: two role work-groups in the chained update launch,
+                           // | kTileFused, | kTileThreePerCU (see their definitions)
     const signed char *panel_tile = nullptr;   // host: per-panel tile shape (overrides `tile`), or none
     double *K;             // S x np x np
     int *info;             // S

@@ -79,7 +79,7 @@ __attribute__((amdgpu_waves_per_eu(KC == 16 ? 2 : 1, KC == 16 ? 2 : 1))) panel_u
 // Launch 2 of panel c0: every row tile below block b0 = c0 is solved with
 // L(b0, b0) (trsm64).  Work-groups 0 .. S-1 take the tile of block
 // b1 = c0 + 64, keep the solved tile on chip, apply it to the diagonal block
-// (b1, b1) (syrk64_lds) and factor that block.
+// (b1, b1) (syrk64) and factor that block.
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2)))
 trsm_factor_kernel(double *__restrict__ K, int np, int n, int c0, int S, int *__restrict__ info,
                    double *__restrict__ logdet_part, double *__restrict__ dinv)
@@ -108,7 +108,7 @@ trsm_factor_kernel(double *__restrict__ K, int np, int n, int c0, int S, int *__
         *reinterpret_cast<double2 *>(Ls + col * kPLD + r2) = *reinterpret_cast<const double2 *>(Cg + (size_t)col * np + r2);
     }
     __syncthreads();
-    syrk64_lds(Ls, Xs);
+    syrk64(Ls, Xs, kPLD);
     __syncthreads();
     double lg = 0.0;
     const int fail = potrf64_block<true>(A, np, n, c1, &lg, Ls,
@@ -240,10 +240,13 @@ __device__ inline void solve_rows_tile(double *A, int np, int c0, int row0, cons
                         dinv_s + (size_t)(c1 / kNBI) * kDinv);
 }
 
+// LDS of the chained update launch: its row tiles, the 64 x 64 and 64 x 128
+// tiles of the roles and of the tail pre-update (whatever the row-tile
+// shape), the diagonal factorisation.
 template <int KC, int MB, int NB>
 constexpr int chain_smem()
 {
-    constexpr int a = panel_smem<KC, MB, NB>(), c = kPotrfSmem + kNBI * kPLD;
+    constexpr int a = panel_smem<KC, MB, NB>(), c = TileShape<kMB, kNBO, KC>::SMEM;
     return a > c ? a : c;
 }
 
@@ -288,7 +291,7 @@ __host__ __device__ inline int tail_tiles(int np, int c0, int tail)
 // row tile then solves its rows (solve_rows_tile) -- the panel in one launch.
 template <int KC, int MB, int NB, bool TWO, bool FUSED>
 __global__ void __launch_bounds__(kThreads)
-__attribute__((amdgpu_waves_per_eu(KC == 16 ? 2 : 1, KC == 16 ? 2 : 1))) panel_chain_kernel(double *__restrict__ K, int np, int n,
+__attribute__((amdgpu_waves_per_eu(KC == 8 ? 3 : 2, KC == 8 ? 3 : 2))) panel_chain_kernel(double *__restrict__ K, int np, int n,
                                                                        int c0, int k0, int kt, int tail, int S,
                                                                        int *__restrict__ info,
                                                                        double *__restrict__ logdet_part,
@@ -335,17 +338,18 @@ __attribute__((amdgpu_waves_per_eu(KC == 16 ? 2 : 1, KC == 16 ? 2 : 1))) panel_c
                 if constexpr (FUSED) chain_publish_mem(cw + kChainF2, tag);
                 return;
             }
-            double *Xs = smem;                       // L(b1, b0) on chip
-            double *Ls = smem + kNBI * kPLD;         // block (b1, b1), then its factorisation
+            // L(b1, b0) solved in place, then re-read by every wave (global
+            // stores of a work-group are visible to it after the barrier)
+            double *Ls = smem;                       // block (b1, b1), then its factorisation
             double *tile = A + (size_t)c0 * np + c1;
-            trsm64_solve<false, true, FUSED>(tile, np, tile, np, A + (size_t)c0 * np + c0, np, dinv0, Xs);
+            trsm64_solve<false, false, FUSED>(tile, np, tile, np, A + (size_t)c0 * np + c0, np, dinv0);
             const double *Cg = A + (size_t)c1 * np + c1;
             for (int e = threadIdx.x; e < kNBI * kNBI / 2; e += kThreads) {
                 const int col = e / (kNBI / 2), r2 = (e % (kNBI / 2)) * 2;
                 *reinterpret_cast<double2 *>(Ls + col * kPLD + r2) = *reinterpret_cast<const double2 *>(Cg + (size_t)col * np + r2);
             }
             __syncthreads();
-            syrk64_lds(Ls, Xs);
+            syrk64(Ls, tile, np);
             __syncthreads();
             double lg = 0.0;
             const int fail = potrf64_block<true, FUSED>(A, np, n, c1, &lg, Ls,
@@ -377,14 +381,8 @@ __attribute__((amdgpu_waves_per_eu(KC == 16 ? 2 : 1, KC == 16 ? 2 : 1))) panel_c
                 if constexpr (FUSED) chain_publish_mem(cw + kChainF2, tag);
                 return;
             }
-            double *Xs = smem + kPotrfSmem;          // L(b1, b0)
-            const double *Xg = A + (size_t)c0 * np + c1;
-            for (int e = threadIdx.x; e < kNBI * kNBI / 2; e += kThreads) {
-                const int col = e / (kNBI / 2), r2 = (e % (kNBI / 2)) * 2;
-                *reinterpret_cast<double2 *>(Xs + col * kPLD + r2) = *reinterpret_cast<const double2 *>(Xg + (size_t)col * np + r2);
-            }
-            __syncthreads();
-            syrk64_lds(smem, Xs);
+            // L(b1, b0), written by role B, read from memory
+            syrk64(smem, A + (size_t)c0 * np + c1, np);
             __syncthreads();
             double lg = 0.0;
             const int fail = potrf64_block<true, FUSED>(A, np, n, c1, &lg, smem,
@@ -471,6 +469,7 @@ template <bool TWO>
 static void chain_tiles(const SwarmBuffers &b, int tile, int c0, int k0, int kt, int tail, hipStream_t st)
 {
     if (tile & kTileFused) chain_launch<16, 64, 128, TWO, true>(b, c0, k0, kt, tail, st);
+    else if (tile & kTileThreePerCU) chain_launch<8, 64, 128, TWO>(b, c0, k0, kt, tail, st);
     else if (tile == 1) chain_launch<16, 32, 128, TWO>(b, c0, k0, kt, tail, st);
     else if (tile == 2) chain_launch<16, 32, 64, TWO>(b, c0, k0, kt, tail, st);
     else chain_launch<16, 64, 128, TWO>(b, c0, k0, kt, tail, st);
@@ -501,15 +500,16 @@ static void chain_schedule(const SwarmBuffers &b, int p, int &k0, int &kt, int &
     }
 }
 
-// 16-deep chunks only (the 32-deep update tile leaves no room for the roles).
-// Tile code: 0 64x128, 1 32x128, 2 32x64 row tiles, plus kTileTwoRoles for
-// the two-role variant.
+// 16- or 8-deep chunks (the 32-deep update tile leaves no room for the
+// roles).  Tile code: 0 64x128, 1 32x128, 2 32x64 row tiles, plus
+// kTileTwoRoles for the two-role variant; kTileFused and kTileThreePerCU
+// select 64x128 variants.
 void launch_panel_chain(const SwarmBuffers &b, int c0, hipStream_t st)
 {
     const int code = b.panel_tile ? b.panel_tile[c0 / kNBO] : b.tile;
     int k0 = 0, kt = 0, tail = 0;
     chain_schedule(b, c0 / kNBO, k0, kt, tail);
-    const int tile = (code & kTileFused) ? kTileFused : (code & 3);
+    const int tile = (code & kTileFused) ? kTileFused : (code & kTileThreePerCU) ? kTileThreePerCU : (code & 3);
     if (code & kTileTwoRoles) chain_tiles<true>(b, tile, c0, k0, kt, tail, st);
     else chain_tiles<false>(b, tile, c0, k0, kt, tail, st);
 }

@@ -504,30 +504,27 @@ __device__ inline void trsm64_solve(const double *Cin, int ldi, double *Ct, int 
         for (int v = 0; v < 4; ++v)
             t[j][v] = IDENT ? ((16 * j + 4 * v + lq == 16 * wave + lr) ? 1.0 : 0.0)
                             : Ciw[(size_t)(16 * j + 4 * v + lq) * ldi];
-    // L_jk (j > k) in A layout: lane l holds L_jk[l%16][4kk + l/16]
-    double lf[6][4];
-#pragma unroll
-    for (int j = 1, p = 0; j < 4; ++j)
-#pragma unroll
-        for (int k = 0; k < j; ++k, ++p)
-#pragma unroll
-            for (int kk = 0; kk < 4; ++kk) lf[p][kk] = Lb[(size_t)(16 * k + 4 * kk + lq) * ldl + 16 * j + lr];
-    double wf[4][4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-#pragma unroll
-        for (int kk = 0; kk < 4; ++kk) wf[j][kk] = dinv[j * 256 + (4 * kk + lq) * 16 + lr];
+    // L_jk (j > k) in A layout: lane l holds L_jk[l%16][4kk + l/16], and the
+    // 16x16 inverses likewise, each read where it is used (held all at once
+    // they take 80 registers)
     d4 y[4];
 #pragma unroll
-    for (int j = 0, p = 0; j < 4; ++j) {
+    for (int j = 0; j < 4; ++j) {
         d4 r = t[j];
 #pragma unroll
-        for (int k = 0; k < j; ++k, ++p)
+        for (int k = 0; k < j; ++k) {
+            double lf[4];
 #pragma unroll
-            for (int kk = 0; kk < 4; ++kk) r = mfma_f64(-lf[p][kk], y[k][kk], r);
+            for (int kk = 0; kk < 4; ++kk) lf[kk] = Lb[(size_t)(16 * k + 4 * kk + lq) * ldl + 16 * j + lr];
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) r = mfma_f64(-lf[kk], y[k][kk], r);
+        }
+        double wf[4];
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) wf[kk] = dinv[j * 256 + (4 * kk + lq) * 16 + lr];
         d4 o = d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int kk = 0; kk < 4; ++kk) o = mfma_f64(wf[j][kk], r[kk], o);
+        for (int kk = 0; kk < 4; ++kk) o = mfma_f64(wf[kk], r[kk], o);
         y[j] = o;
     }
 #pragma unroll
@@ -543,30 +540,33 @@ __device__ inline void trsm64_solve(const double *Cin, int ldi, double *Ct, int 
     }
 }
 
-// Ls -= X X^T on the lower 16x16 blocks of a 64x64 block, both on chip
-// (Ls[c * kPLD + r], X likewise), 64 deep on the matrix cores with the
-// accumulation order of tile_gemm_sub (so the same bits as that routine's
-// depth-64 update of the same block).
-__device__ inline void syrk64_lds(double *__restrict__ Ls, const double *__restrict__ Xs)
+// Ls -= X X^T on the lower 16x16 blocks of a 64x64 block on chip
+// (Ls[c * kPLD + r]), X(r, k) at X[k * ldx + r] on chip or in global memory,
+// 64 deep on the matrix cores with the accumulation order of tile_gemm_sub
+// (so the same bits as that routine's depth-64 update of the same block).
+__device__ inline void syrk64(double *__restrict__ Ls, const double *__restrict__ X, int ldx)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int lr = lane & 15, lq = lane >> 4;
-    int t = 0;
-    for (int bj = 0; bj < 4; ++bj)
-        for (int bi = bj; bi < 4; ++bi, ++t) {
-            if ((t & 3) != wave) continue;
-            d4 acc;
+    // lower blocks t = 0 .. 9 in column order, block t on wave t mod 4; one
+    // block at a time (a wave's blocks are not unrolled together, which would
+    // hold all their operand loads in registers at once)
+#pragma unroll 1
+    for (int t = wave; t < 10; t += 4) {
+        const int bj = t < 4 ? 0 : t < 7 ? 1 : t < 9 ? 2 : 3;
+        const int bi = t - (bj == 0 ? 0 : bj == 1 ? 3 : bj == 2 ? 5 : 6);
+        d4 acc;
 #pragma unroll
-            for (int v = 0; v < 4; ++v) acc[v] = Ls[(16 * bj + 4 * v + lq) * kPLD + 16 * bi + lr];
+        for (int v = 0; v < 4; ++v) acc[v] = Ls[(16 * bj + 4 * v + lq) * kPLD + 16 * bi + lr];
 #pragma unroll
-            for (int kk = 0; kk < 16; ++kk) {
-                const double af = Xs[(4 * kk + lq) * kPLD + 16 * bi + lr];
-                const double bf = Xs[(4 * kk + lq) * kPLD + 16 * bj + lr];
-                acc = mfma_f64(bf, -af, acc);
-            }
-#pragma unroll
-            for (int v = 0; v < 4; ++v) Ls[(16 * bj + 4 * v + lq) * kPLD + 16 * bi + lr] = acc[v];
+        for (int kk = 0; kk < 16; ++kk) {
+            const double af = X[(size_t)(4 * kk + lq) * ldx + 16 * bi + lr];
+            const double bf = X[(size_t)(4 * kk + lq) * ldx + 16 * bj + lr];
+            acc = mfma_f64(bf, -af, acc);
         }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Ls[(16 * bj + 4 * v + lq) * kPLD + 16 * bi + lr] = acc[v];
+    }
 }
 
 // L(row0:row0+64, c0:c0+64) = C L_bb^{-T} in place, C the tile and L_bb the

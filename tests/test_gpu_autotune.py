@@ -127,11 +127,14 @@ def _dup_case(n, dup, src, seed):
     (700, 300, 17, 9, 4),
     (700, 200, 5, 9, 8),     # with the fused row solve (three and two roles)
     (700, 300, 17, 9, 12),
+    (700, 200, 5, 9, 16),    # 8-deep chunks, three work-groups per CU
+    (700, 300, 17, 9, 20),
 ])
 def test_chained_schedule_bit_identical(gpu, monkeypatch, n, dup, src, S, code):
     """Two launches per panel (the diagonal chain handed over inside the
     update launch; code 4: two role work-groups, 8 / 12: the row solve fused
-    into the update launch) against three: same bits, same INFO, the failing
+    into the update launch, 16 / 20: 8-deep chunks at three work-groups per
+    CU) against three: same bits, same INFO, the failing
     particles included (every role must publish on every exit path)."""
     x, y, prog, mfit = _dup_case(n, dup, src, seed=n + dup)
     out = {}
@@ -163,12 +166,12 @@ def test_chained_schedule_bit_identical(gpu, monkeypatch, n, dup, src, S, code):
 
 
 @pytest.mark.parametrize("path,tiles", [(3, None), (4, None), (5, "2,0,1,2,0,1,2,0"), (5, "6,4,5,2,4,1,6,5"),
-                                        (5, "8,12,8,12,0,8,4,12")])
+                                        (5, "8,12,8,12,0,8,4,12"), (5, "16,20,16,20,0,16,4,20")])
 def test_chained_schedule_tile_shapes(gpu, monkeypatch, path, tiles):
     """The chained update launch with the 32x128 and 32x64 row tiles, the
-    per-panel mix, the two-role variant (tile code + 4) and the fused row
-    solve (8, 12: one launch for the panel): the bits of the three-launch
-    schedule."""
+    per-panel mix, the two-role variant (tile code + 4), the fused row
+    solve (8, 12: one launch for the panel) and the 8-deep chunks at three
+    work-groups per CU (16, 20): the bits of the three-launch schedule."""
     _, x, y, prog, mfit = _problem(900, 6, seed=path)
     out = {}
     for chain in (0, 1):
