@@ -252,6 +252,16 @@ __device__ inline void sqrt_rsqrt(double x, double &sq, double &rsq)
     rsq = 2.0 * h;
 }
 
+// v, made where it is used: the compiler cannot hoist the value out of a
+// loop (the identity column of a 16x16 inverse, hoisted out of the panel
+// loop, stays live across the whole block factorisation and spills when the
+// register budget is that of three waves per SIMD).
+__device__ inline double opaque_f64(double v)
+{
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
 // x from lane l (uniform l) of the wave.
 __device__ inline double readlane_f64(double x, int l)
 {
@@ -320,8 +330,9 @@ __device__ inline int potrf64_block(double *__restrict__ A, int np, int n, int c
                 const double *D = Ls + 16 * q * kPLD + 16 * q;   // D(i, k) at D[k * kPLD + i]
                 const double *rd = rdiag + 16 * q;
                 double x[16];
+                const double one = opaque_f64(1.0), zero = opaque_f64(0.0);
 #pragma unroll
-                for (int i = 0; i < 16; ++i) x[i] = i == lane ? 1.0 : 0.0;
+                for (int i = 0; i < 16; ++i) x[i] = i == lane ? one : zero;
 #pragma unroll
                 for (int k = 0; k < 16; ++k) {
                     x[k] = x[k] * rd[k];
@@ -446,8 +457,9 @@ __device__ inline int potrf64_block(double *__restrict__ A, int np, int n, int c
                         const double *D = Ls + 16 * q * kPLD + 16 * q;
                         const double *rd = rdiag + 16 * q;
                         double x[16];
+                        const double one = opaque_f64(1.0), zero = opaque_f64(0.0);
 #pragma unroll
-                        for (int i = 0; i < 16; ++i) x[i] = i == lane ? 1.0 : 0.0;
+                        for (int i = 0; i < 16; ++i) x[i] = i == lane ? one : zero;
 #pragma unroll
                         for (int k = 0; k < 16; ++k) {
                             x[k] = x[k] * rd[k];
