@@ -225,8 +225,8 @@ int fb_profile_read(fb_handle *h, int kind, double *total_ms, long long *launche
  * them compute the same bits.  path: -1 not chosen yet, 0 persistent single
  * launch; lock-step launches with 1: 32-deep GEMM chunks and 64x128 update
  * tiles, 2: 16-deep chunks, 64x128 tiles, 3: 16-deep, 32x128, 4: 16-deep,
- * 32x64, 5: 16-deep with each panel's fastest of those three tile shapes
- * (timed panel by panel).  With 16-deep chunks and swarms up to 160 the
+ * 32x64, 5: each panel's fastest of those three tile shapes and the
+ * variants below (timed panel by panel).  With 16-deep chunks and swarms up to 160 the
  * lock-step schedule is chained: two launches per 128-column panel, the
  * panel's diagonal factorisation handed over between work-groups inside the
  * update launch (path 5 then also times each shape with two or three such
@@ -235,7 +235,9 @@ int fb_profile_read(fb_handle *h, int kind, double *total_ms, long long *launche
  * FB_PANEL_TILES=t0,t1,... (one code per 128-column panel: 0..2 the shapes
  * of paths 2..4, 4..6 the same with two work-groups per particle in the
  * chained update launch, 8 and 12 the 64x128 shape with three or two such
- * work-groups and the row solve fused into the update launch) pins a
+ * work-groups and the row solve fused into the update launch, 16 and 20 the
+ * 64x128 shape with three or two such work-groups in 8-deep chunks, three
+ * work-groups per CU instead of two) pins a
  * per-panel mix; FB_CHAIN=0 gives three launches per panel instead;
  * FB_TAIL=n sets how many trailing panels the chained schedule pre-updates
  * in the launch before them (default 3, 0 off); FB_PERSISTENT=0/1 pins path
