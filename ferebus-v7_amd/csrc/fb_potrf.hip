@@ -223,21 +223,12 @@ __device__ inline void chain_publish_mem(int *flag, int tag)
 }
 
 // The row solve of one 64-row tile at row0 below panel c0 of the matrix at
-// A: solve with L(b0, b0), the 64-deep update of its b1 part with
-// L(b1, b0), solve with L(b1, b1) (dinv_s: the particle's 16x16 inverses).
-// The tile's own b0 result is re-read after the barrier: global stores of a
-// work-group are visible to it.
-template <int KC>
-__device__ inline void solve_rows_tile(double *A, int np, int c0, int row0, const double *dinv_s, double *smem)
+// A: each wave solves its 16 rows alone (solve_rows16: solve with
+// L(b0, b0), the 64-deep update of the b1 part, solve with L(b1, b1);
+// dinv_s: the particle's 16x16 inverses).
+__device__ inline void solve_rows_tile(double *A, int np, int c0, int row0, const double *dinv_s)
 {
-    const int c1 = c0 + kNBI;
-    double *tile = A + (size_t)c0 * np + row0;
-    trsm64_solve<false>(tile, np, tile, np, A + (size_t)c0 * np + c0, np, dinv_s + (size_t)(c0 / kNBI) * kDinv);
-    __syncthreads();
-    gemm_tile_update<kMB, kNBI, KC, 2, true>(A, np, row0, c1, c0, c1, smem);
-    __syncthreads();
-    trsm64_solve<false>(smem, kPLD, A + (size_t)c1 * np + row0, np, A + (size_t)c1 * np + c1, np,
-                        dinv_s + (size_t)(c1 / kNBI) * kDinv);
+    solve_rows16(A, np, c0, row0 + 16 * (threadIdx.x >> 6), dinv_s);
 }
 
 // LDS of the chained update launch: its row tiles, the 64 x 64 and 64 x 128
@@ -414,8 +405,7 @@ __attribute__((amdgpu_waves_per_eu(KC == 8 ? 3 : 2, KC == 8 ? 3 : 2))) panel_cha
                 return;
             }
             if (info[s] != 0) return;
-            __syncthreads();
-            solve_rows_tile<KC>(A, np, c0, row0, dinv + (size_t)s * (np / kNBI) * kDinv, smem);
+            solve_rows_tile(A, np, c0, row0, dinv + (size_t)s * (np / kNBI) * kDinv);
         }
         return;
     }
@@ -439,20 +429,18 @@ __attribute__((amdgpu_waves_per_eu(KC == 8 ? 3 : 2, KC == 8 ? 3 : 2))) panel_cha
 
 // Launch 2 of panel c0, chained: every row tile below the panel is solved
 // (solve_rows_tile: launches 2 and 3 of the three-launch schedule for one
-// tile).
-template <int KC>
+// tile, each wave on its own 16 rows; no LDS).
 __global__ void __launch_bounds__(kThreads) solve_rows_kernel(double *__restrict__ K, int np, int c0, int S,
                                                               const int *__restrict__ info,
                                                               const double *__restrict__ dinv)
 {
-    __shared__ __attribute__((aligned(16))) double smem[TileShape<kMB, kNBI, KC>::SMEM];
     const int rt = (np - c0 - kNBO) / kNBI;
     const int t = xcd_tile(blockIdx.x, S * rt);
     if (t < 0) return;
     const int s = t / rt;
     const int row0 = c0 + kNBO + (t % rt) * kNBI;
     if (info[s] != 0) return;
-    solve_rows_tile<KC>(K + (size_t)s * np * np, np, c0, row0, dinv + (size_t)s * (np / kNBI) * kDinv, smem);
+    solve_rows_tile(K + (size_t)s * np * np, np, c0, row0, dinv + (size_t)s * (np / kNBI) * kDinv);
 }
 
 template <int KC, int MB, int NB, bool TWO, bool FUSED = false>
@@ -526,7 +514,7 @@ void launch_solve_rows(const SwarmBuffers &b, int c0, hipStream_t st)
 {
     const int rt = (b.np - c0 - kNBO) / kNBI;
     if (rt <= 0) return;
-    hipLaunchKernelGGL((solve_rows_kernel<16>), dim3(8 * ceil_div(b.S * rt, 8)), dim3(kThreads), 0, st, b.K, b.np,
+    hipLaunchKernelGGL(solve_rows_kernel, dim3(8 * ceil_div(b.S * rt, 8)), dim3(kThreads), 0, st, b.K, b.np,
                        c0, b.S, b.info, b.dinv);
 }
 
@@ -541,7 +529,9 @@ static void panel_launch(const SwarmBuffers &b, int c0, hipStream_t st)
 
 void launch_panel_update_factor(const SwarmBuffers &b, int c0, hipStream_t st)
 {
-    const int tile = b.panel_tile ? b.panel_tile[c0 / kNBO] : b.tile;
+    // the shape only: the role, fused and three-per-CU variants belong to
+    // the chained schedule
+    const int tile = (b.panel_tile ? b.panel_tile[c0 / kNBO] : b.tile) & 3;
     if (b.gemm_kc == 32) panel_launch<32, 64, 128>(b, c0, st);
     else if (tile == 1) panel_launch<16, 32, 128>(b, c0, st);
     else if (tile == 2) panel_launch<16, 32, 64>(b, c0, st);

@@ -500,26 +500,16 @@ __device__ inline int potrf64_block(double *__restrict__ A, int np, int n, int c
 // in lane l, component v: the B-operand layout of four consecutive k-steps,
 // so each Y_k feeds the later products straight from registers.  No LDS, no
 // barrier; the tile is read completely before it is overwritten.
-template <bool IDENT, bool COPY = false, bool WT = false>
-__device__ inline void trsm64_solve(const double *Cin, int ldi, double *Ct, int ldc,
-                                    const double *__restrict__ Lb, int ldl, const double *__restrict__ dinv,
-                                    double *__restrict__ copy = nullptr)
+// The solve of one wave's 16 rows: t[j][v] = C(lr, 16j + 4v + lq) in lane
+// (lr, lq) = (l % 16, l / 16) gives y = X = C L_bb^{-T} in the same layout.
+// L_jk (j > k) in A layout (lane l holds L_jk[l%16][4kk + l/16]) and the
+// 16x16 inverses likewise, each read where it is used (held all at once
+// they take 80 registers).
+__device__ inline void trsm16_rows(const d4 t[4], const double *__restrict__ Lb, int ldl,
+                                   const double *__restrict__ dinv, d4 y[4])
 {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
     const int lr = lane & 15, lq = lane >> 4;
-    const double *Ciw = Cin + 16 * wave + lr;
-    double *Cw = Ct + 16 * wave + lr;
-    d4 t[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-#pragma unroll
-        for (int v = 0; v < 4; ++v)
-            t[j][v] = IDENT ? ((16 * j + 4 * v + lq == 16 * wave + lr) ? 1.0 : 0.0)
-                            : Ciw[(size_t)(16 * j + 4 * v + lq) * ldi];
-    // L_jk (j > k) in A layout: lane l holds L_jk[l%16][4kk + l/16], and the
-    // 16x16 inverses likewise, each read where it is used (held all at once
-    // they take 80 registers)
-    d4 y[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         d4 r = t[j];
@@ -539,6 +529,26 @@ __device__ inline void trsm64_solve(const double *Cin, int ldi, double *Ct, int 
         for (int kk = 0; kk < 4; ++kk) o = mfma_f64(wf[kk], r[kk], o);
         y[j] = o;
     }
+}
+
+template <bool IDENT, bool COPY = false, bool WT = false>
+__device__ inline void trsm64_solve(const double *Cin, int ldi, double *Ct, int ldc,
+                                    const double *__restrict__ Lb, int ldl, const double *__restrict__ dinv,
+                                    double *__restrict__ copy = nullptr)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lr = lane & 15, lq = lane >> 4;
+    const double *Ciw = Cin + 16 * wave + lr;
+    double *Cw = Ct + 16 * wave + lr;
+    d4 t[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+            t[j][v] = IDENT ? ((16 * j + 4 * v + lq == 16 * wave + lr) ? 1.0 : 0.0)
+                            : Ciw[(size_t)(16 * j + 4 * v + lq) * ldi];
+    d4 y[4];
+    trsm16_rows(t, Lb, ldl, dinv, y);
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
@@ -550,6 +560,54 @@ __device__ inline void trsm64_solve(const double *Cin, int ldi, double *Ct, int 
 #pragma unroll
             for (int v = 0; v < 4; ++v) copy[(16 * j + 4 * v + lq) * kPLD + 16 * wave + lr] = y[j][v];
     }
+}
+
+// The row solve of the 16 rows at `row` (a multiple of 16, below the
+// 128-column panel c0) of the matrix at A by one wave, with no LDS and no
+// barrier (blocks b0 = c0, b1 = c0 + 64; dinv_s: the matrix's 16x16
+// inverses):
+//     X0 = C(row, b0) L(b0, b0)^{-T}
+//     C(row, b1) -= X0 L(b1, b0)^T
+//     X1 = C(row, b1) L(b1, b1)^{-T}.
+// The middle product runs as tile_gemm_sub runs it (C^T accumulators, the
+// L(b1, b0) fragment as the first operand, k in steps of four from c0), its
+// A operand -X0 taken from the solve's registers: an MFMA result holds four
+// consecutive k-steps of an operand.  Each element sees the operations of
+// the 64-row tile routines in the same order, so the bits are theirs.
+__device__ inline void solve_rows16(double *A, int np, int c0, int row, const double *dinv_s)
+{
+    const int lane = threadIdx.x & 63;
+    const int lr = lane & 15, lq = lane >> 4;
+    const int c1 = c0 + kNBI;
+    double *C0 = A + (size_t)c0 * np + row + lr;   // C(row + lr, c0 + c) at C0[c * np]
+    double *C1 = A + (size_t)c1 * np + row + lr;
+    d4 t[4], x0[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) t[j][v] = C0[(size_t)(16 * j + 4 * v + lq) * np];
+    trsm16_rows(t, A + (size_t)c0 * np + c0, np, dinv_s + (size_t)(c0 / kNBI) * kDinv, x0);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) C0[(size_t)(16 * j + 4 * v + lq) * np] = x0[j][v];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) t[j][v] = C1[(size_t)(16 * j + 4 * v + lq) * np];
+    const double *Bg = A + (size_t)c0 * np + c1 + lr;   // L(c1 + r, c0 + k) at Bg[k * np + r]
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk) {
+        const double af = -x0[kk / 4][kk % 4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) t[j] = mfma_f64(Bg[(size_t)(4 * kk + lq) * np + 16 * j], af, t[j]);
+    }
+    d4 x1[4];
+    trsm16_rows(t, A + (size_t)c1 * np + c1, np, dinv_s + (size_t)(c1 / kNBI) * kDinv, x1);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) C1[(size_t)(16 * j + 4 * v + lq) * np] = x1[j][v];
 }
 
 // Ls -= X X^T on the lower 16x16 blocks of a 64x64 block on chip
