@@ -859,8 +859,10 @@ static int choose_path(fb_handle *h, int s)
     const int nc = (int)codes.size();
     std::vector<std::vector<float>> tpan(nc, std::vector<float>(npan, 1e30f));
     std::vector<std::pair<hipEvent_t, hipEvent_t>> evs;
-    // two passes over the candidates, the faster time of each panel kept
-    for (int pass = 0; pass < 2; ++pass)
+    // three passes over the candidates, the fastest time of each panel kept
+    // (with two, the choice between variants a few per cent apart changed
+    // from run to run)
+    for (int pass = 0; pass < 3; ++pass)
     for (int t = 0; t < nc; ++t) {
         set_mode(h, 2 + ((codes[t] & fb::kTileFused) ? 0 : (codes[t] & 3)), s);
         h->tile = codes[t];
