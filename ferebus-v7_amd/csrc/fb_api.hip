@@ -155,8 +155,9 @@ struct EventPair {
 // factorisation paths the autotuner chooses from: 0 persistent single launch;
 // lock-step launches with 1: 32-deep GEMM chunks, 64x128 update tiles,
 // 2: 16-deep, 64x128, 3: 16-deep, 32x128, 4: 16-deep, 32x64 (kNumModes, each
-// can be pinned); 5: 16-deep with each panel's fastest of those three tile
-// shapes (measured per panel, so only reachable by tuning)
+// can be pinned); 5: each panel's fastest of those three tile shapes and,
+// chained, of their role, fused and three-per-CU variants (measured per
+// panel: reachable by tuning or FB_PANEL_TILES)
 constexpr int kNumModes = 5;
 constexpr int kModePerPanel = 5;
 
@@ -845,8 +846,9 @@ static int choose_path(fb_handle *h, int s)
         if (ms_best < best) { best = ms_best; best_mode = mode; }
     }
     // the panel updates of the three 16-deep shapes (chained: each with three
-    // or two role work-groups), timed panel by panel: a mix of the fastest
-    // variant of each panel is one more candidate
+    // or two role work-groups, and the fused and three-per-CU 64x128
+    // variants), timed panel by panel: a mix of the fastest variant of each
+    // panel is one more candidate
     const int npan = h->np / fb::kNBO;
     std::vector<int> codes = {0, 1, 2};
     if (h->chain && s <= fb::kChainMaxSwarm) {

@@ -109,8 +109,9 @@ constexpr int kDinv = 4 * 16 * 16;   // doubles per 64x64 diagonal block: four 1
 constexpr int kChainStride = 64;
 constexpr int kChainF0 = 0, kChainF1 = 8, kChainXccB = 16, kChainXccC = 24, kChainF2 = 32;
 // Swarms up to this size: the waiting work-groups of the update launch (two
-// per particle) hold fewer than its 512 resident slots (two per CU), so every
-// producer they wait for finds a slot.
+// per particle) hold fewer than its 512 resident slots (two per CU; the
+// three-per-CU variant has 768), so every producer they wait for finds a
+// slot.
 constexpr int kChainMaxSwarm = 160;
 // INFO of a particle whose factorisation stopped at a bounded wait (1 s, or
 // FB_CHAIN_WAIT_TICKS of the 100 MHz clock) instead of hanging; the host
@@ -144,7 +145,7 @@ struct SwarmBuffers {
     const KernelProgram *prog;   // device copy
     const KernelProgram *prog_host;   // host copy (passed by value to kernels whose loops read it)
     int gemm_kc = 32;      // chunk depth of the lock-step trailing update (16 or 32)
-    int tile = 0;          // lock-step panel-update tiles: 0 64x128, 1 32x128, 2 32x64 (16-deep chunks),
+    int tile = 0;          // lock-step panel-update tiles: 0 64x128, 1 32x128, 2 32x64,
                            // | kTileTwoRoles: two role work-groups in the chained update launch,
                            // | kTileFused, | kTileThreePerCU (see their definitions)
     const signed char *panel_tile = nullptr;   // host: per-panel tile shape (overrides `tile`), or none
