@@ -1039,6 +1039,29 @@ int fb_get_alpha(fb_handle *h, const double *theta, double *alpha, double *out4,
     return h->profile ? drain_profile(h) : FB_OK;
 }
 
+int fb_debug_matrix(fb_handle *h, const double *theta, int stage, double *out, int *info)
+{
+    if (!h || !theta || !out || !info) return fail(FB_E_ARG, "fb_debug_matrix: null argument");
+    if (stage != 0 && stage != 1) return fail(FB_E_ARG, "fb_debug_matrix: stage must be 0 (K) or 1 (L)");
+    const int P = h->prog.n_par, n = h->n;
+    hipStream_t st = h->stream;
+    std::memcpy(h->h_theta, theta, sizeof(double) * P);
+    FB_HIP(hipMemcpyAsync(h->d_theta, h->h_theta, sizeof(double) * P, hipMemcpyHostToDevice, st));
+    const fb::SwarmBuffers b = buffers(h, 0, 1, h->d_theta, h->d_ll, h->d_info);
+    if (stage == 0) fb::launch_build(b, st);
+    else enqueue_group(h, b, st);   // the plain path's kernels: the bits of every batched path
+    FB_HIP(hipMemcpyAsync(h->h_info, h->d_info, sizeof(int), hipMemcpyDeviceToHost, st));
+    FB_HIP(hipMemcpy2DAsync(out, sizeof(double) * n, h->d_K, sizeof(double) * h->np, sizeof(double) * n, n,
+                            hipMemcpyDeviceToHost, st));
+    FB_HIP(hipStreamSynchronize(st));
+    FB_HIP(hipGetLastError());
+    if (int rc = check_waits(h->h_info, 1, "fb_debug_matrix")) return rc;
+    *info = h->h_info[0];
+    for (int j = 1; j < n; ++j)   // the strictly upper part is never written on the device
+        for (int i = 0; i < j; ++i) out[i + (size_t)n * j] = 0.0;
+    return h->profile ? drain_profile(h) : FB_OK;
+}
+
 int fb_swarm_loglik_grad(fb_handle *h, int s, const double *theta, double *ll, double *grad, int *info)
 {
     if (!h || !theta || !ll || !grad || !info) return fail(FB_E_ARG, "fb_swarm_loglik_grad: null argument");

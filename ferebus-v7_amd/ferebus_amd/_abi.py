@@ -25,7 +25,7 @@ EXPORTS = (
     "fb_profile_enable", "fb_profile_read", "fb_factor_path", "fb_device_name", "fb_last_error",
     "fb_multi_create", "fb_multi_swarm_loglik", "fb_multi_destroy",
     "fb_pso_create", "fb_pso_destroy", "fb_pso_init", "fb_pso_step", "fb_pso_retry", "fb_pso_select",
-    "fb_pso_set_gbest", "fb_pso_positions",
+    "fb_pso_set_gbest", "fb_pso_positions", "fb_debug_matrix",
 )
 
 
@@ -75,6 +75,7 @@ def load() -> ctypes.CDLL:
         "fb_particle_loglik": (ctypes.c_int, [vp, ctypes.c_int, _P_DBL, _P_DBL, _P_INT]),
         "fb_get_alpha": (ctypes.c_int, [vp, _P_DBL, _P_DBL, _P_DBL, _P_INT]),
         "fb_swarm_loglik_grad": (ctypes.c_int, [vp, ctypes.c_int, _P_DBL, _P_DBL, _P_DBL, _P_INT]),
+        "fb_debug_matrix": (ctypes.c_int, [vp, _P_DBL, ctypes.c_int, _P_DBL, _P_INT]),
         "fb_profile_enable": (ctypes.c_int, [vp, ctypes.c_int]),
         "fb_factor_path": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
         "fb_profile_read": (ctypes.c_int, [vp, ctypes.c_int, _P_DBL, ctypes.POINTER(ctypes.c_longlong)]),

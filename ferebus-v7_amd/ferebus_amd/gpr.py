@@ -129,6 +129,17 @@ class SwarmLikelihood:
                                           ctypes.byref(info)), "fb_get_alpha")
         return alpha, out4, info.value
 
+    def matrix(self, theta, stage: str = "K"):
+        """The device's K(theta) (stage "K", nugget included) or its Cholesky
+        factor (stage "L") for one theta: ((N, N) lower triangle, INFO)."""
+        t = self._theta(theta)[0].copy()
+        out = np.zeros((self.n, self.n), order="F")
+        info = ctypes.c_int()
+        _abi.check(self._lib.fb_debug_matrix(self._h, _abi.dptr(t), {"K": 0, "L": 1}[stage],
+                                             out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                             ctypes.byref(info)), "fb_debug_matrix")
+        return out, info.value
+
     def swarm_grad(self, theta):
         """(ll[S], grad[S, P], info[S]) of the adjoint gradient path."""
         t = self._theta(theta)

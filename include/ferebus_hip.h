@@ -129,6 +129,20 @@ int fb_particle_loglik(fb_handle *h, int s, const double *theta, double *ll, int
  *   info   LAPACK INFO of the factorisation. */
 int fb_get_alpha(fb_handle *h, const double *theta, double *alpha, double *out4, int *info);
 
+/* The covariance matrix of one parameter vector as the device builds it
+ * (stage 0: K(theta) with the nugget on the diagonal, compute_R
+ * gpr.f90:185-189 with the kernel builders of kernel_module.f90:208-219,
+ * stationary_kernel.f90:101-122, periodic_kernel.f90:181-206,
+ * stationary_cyclic_kernel.f90:104-127), or its Cholesky factor as the
+ * factorisation leaves it (stage 1: L of dpotrf('L'), linalg.f90:95).
+ * A diagnostic for element-by-element checks of the build and the
+ * factorisation; no counterpart in the reference's call path.
+ *   out   n*n FP64, column major: the lower triangle and diagonal; the
+ *         strictly upper part is set to 0.  With stage 1 and info != 0 the
+ *         columns from the failing one on are unspecified.
+ *   info  0 for stage 0; the factorisation's LAPACK INFO for stage 1. */
+int fb_debug_matrix(fb_handle *h, const double *theta, int stage, double *out, int *info);
+
 /* Gradient of the log-likelihood with respect to theta for s particles:
  * d logL / d theta_k = 1/2 tr((alpha alpha^T / s2 - K^{-1}) dK/dtheta_k)
  * (s2 = 1 marginal, sigma2_hat concentrated), evaluated without forming

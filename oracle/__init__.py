@@ -52,6 +52,10 @@ def load():
                                         _P_DBL, _P_INT, ctypes.c_int]
         lib.or_loglik_grad.argtypes = [ctypes.c_int, ctypes.c_int, _P_DBL, _P_DBL, _P_DBL, sp, ctypes.c_double,
                                        ctypes.c_int, ctypes.c_int, _P_DBL, _P_DBL, _P_DBL]
+        lib.or_kernel_matrix_ext.argtypes = [ctypes.c_int, ctypes.c_int, _P_DBL, sp, _P_DBL, _P_DBL, _P_DBL,
+                                             ctypes.c_int]
+        lib.or_loglik_ext.argtypes = [ctypes.c_int, _P_DBL, ctypes.c_int, _P_DBL, _P_DBL, ctypes.c_int, _P_DBL,
+                                      _P_DBL]
         lib.or_mean.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _P_DBL, _P_DBL, _P_DBL, _P_DBL,
                                 _P_DBL, _P_DBL]
         lib.or_baseline_create.argtypes = [ctypes.c_int, ctypes.c_int, _P_DBL, _P_DBL, _P_DBL, sp,
@@ -173,6 +177,44 @@ def loglik_grad(x, y, mean, spec: Spec, theta, noise=1e-10, optimise_noise=False
     info = load().or_loglik_grad(n, d, _d(x), _d(y), _d(m), spec.ref(), noise, int(optimise_noise),
                                  LIK[likelihood], _d(th), ctypes.byref(ll), _d(grad))
     return ll.value, grad, info
+
+
+def kernel_matrix_ext(x, spec: Spec, theta):
+    """The kernel matrix (lower triangle, no nugget) evaluated in long double:
+    (k_hi, k_lo), the nearest FP64 values and their remainders."""
+    x = np.asfortranarray(x, dtype=np.float64)
+    n, d = x.shape
+    hi = np.zeros((n, n), order="F")
+    lo = np.zeros((n, n), order="F")
+    th = np.ascontiguousarray(theta, dtype=np.float64)
+    if load().or_kernel_matrix_ext(n, d, _d(x), spec.ref(), _d(th), _d(hi), _d(lo), n):
+        raise ValueError("bad kernel spec")
+    return hi, lo
+
+
+def ulp_error(a, k_hi, k_lo):
+    """Signed error of the FP64 matrix a against the extended truth
+    (k_hi + k_lo), in units of the spacing of FP64 numbers at k_hi."""
+    LD = np.longdouble
+    err = (np.asarray(a, dtype=LD) - np.asarray(k_hi, dtype=LD)) - np.asarray(k_lo, dtype=LD)
+    sp = np.spacing(np.abs(np.asarray(k_hi, dtype=np.float64)))
+    return np.asarray(err / np.asarray(sp, dtype=LD), dtype=np.float64)
+
+
+def loglik_ext(k, y, mean=None, likelihood="marginal", factor=False):
+    """Extended-precision (80-bit long double) likelihood of the given FP64
+    matrix k (lower triangle read, nugget included): (ll, info, out4) or,
+    with factor=True, (ll, info, out4, L rounded to FP64)."""
+    k = np.asfortranarray(k, dtype=np.float64)
+    n = k.shape[0]
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    m = None if mean is None else np.ascontiguousarray(np.broadcast_to(mean, (n,)), dtype=np.float64)
+    out4 = np.full(4, np.nan)
+    lf = np.zeros((n, n), order="F") if factor else None
+    info = load().or_loglik_ext(n, _d(k), n, _d(y), _d(m), LIK[likelihood], _d(out4), _d(lf))
+    if info < 0:
+        raise MemoryError("or_loglik_ext: allocation failed")
+    return (out4[0], info, out4, lf) if factor else (out4[0], info, out4)
 
 
 MEAN_KIND = {"zero": 0, "constant": 1, "linear": 2, "quadratic": 3}

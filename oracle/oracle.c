@@ -457,6 +457,166 @@ int or_loglik_grad(int n, int d, const double *x, const double *y, const double 
     return 0;
 }
 
+/* ---- extended precision (x87 80-bit long double) ------------------------- */
+
+/* One leaf's value at (i, j) in long double, the formulas of leaf_value
+ * (rbf_kernel.f90:113-119, periodic_kernel.f90:111-118,
+ * distance_matrix.f90:86-87, 113-125, 140-141, stationary_kernel.f90:101-122,
+ * periodic_kernel.f90:181-206, stationary_cyclic_kernel.f90:104-127,
+ * linear_kernel.f90:70-100, constant.f90:62-96) without FP64 rounding. */
+static long double leaf_value_ext(const or_node *o, const int *dims, int n, int d, const double *x,
+                                  const double *theta, int i, int j)
+{
+    const double *th = theta + o->par_off;
+    const int *ad = dims + o->dim_off;
+    const long double pi = 3.141592653589793238462643383279503L;
+    long double t = 0.0L;
+    switch (o->type) {
+    case FB_LEAF_RBF:
+        for (int q = 0; q < o->ndim; ++q) {
+            int f = ad[q] - 1;
+            long double diff = (long double)x[i + (size_t)n * f] - (long double)x[j + (size_t)n * f];
+            t += expl((long double)th[q]) * diff * diff;
+        }
+        return expl(-0.5L * t);
+    case FB_LEAF_PERIODIC:
+        for (int q = 0; q < o->ndim; ++q) {
+            int f = ad[q] - 1;
+            long double diff = (long double)x[i + (size_t)n * f] - (long double)x[j + (size_t)n * f];
+            long double s = sinl(fabsl(diff) / 2.0L);
+            long double l = 1.0L / expl((long double)th[q]);
+            t += l * l * s * s;
+        }
+        return expl(-2.0L * t);
+    case FB_LEAF_RBF_CYCLIC:
+        for (int f = 0; f < d; ++f) {
+            long double diff = (long double)x[i + (size_t)n * f] - (long double)x[j + (size_t)n * f];
+            if ((f + 1) % 3 == 0) diff = fmodl(fabsl(diff) + pi, 2.0L * pi) - pi;
+            t += expl((long double)th[f]) * diff * diff;
+        }
+        return expl(-0.5L * t);
+    case FB_LEAF_LINEAR:
+        for (int q = 0; q < o->ndim; ++q) t += 1.0L / expl(-(long double)th[q]);
+        return t;
+    case FB_LEAF_CONSTANT:
+        return (long double)th[0];
+    default:
+        return NAN;
+    }
+}
+
+int or_kernel_matrix_ext(int n, int d, const double *x, const fb_kernel_spec *spec,
+                         const double *theta, double *k_hi, double *k_lo, int ldk)
+{
+    or_node nodes[OR_MAX_NODES];
+    int p;
+    if (parse_spec(spec, d, nodes, &p)) return -1;
+    for (int j = 0; j < n; ++j)
+        for (int i = j; i < n; ++i) {
+            long double st[OR_MAX_NODES];
+            int sp = 0;
+            for (int q = 0; q < spec->n_nodes; ++q) {
+                const or_node *o = &nodes[q];
+                if (o->type == FB_OP_SUM) { st[sp - 2] += st[sp - 1]; --sp; }
+                else if (o->type == FB_OP_PROD) { st[sp - 2] *= st[sp - 1]; --sp; }
+                else st[sp++] = leaf_value_ext(o, spec->active_dims, n, d, x, theta, i, j);
+            }
+            const double hi = (double)st[0];
+            k_hi[i + (size_t)ldk * j] = hi;
+            if (k_lo) k_lo[i + (size_t)ldk * j] = (double)(st[0] - (long double)hi);
+        }
+    return 0;
+}
+
+/* Cholesky of the given FP64 matrix in long double (64-bit significand),
+ * left-looking: each entry starts from a(i,j) and the products of the
+ * earlier columns are subtracted one k at a time, the order in which the
+ * remainder shrinks toward the pivot (dpotf2's dot-product form,
+ * linalg.f90:95).  Returns INFO (LAPACK semantics). */
+static int potrf_ext(int n, long double *a)
+{
+    for (int j = 0; j < n; ++j) {
+        long double *cj = a + (size_t)n * j;
+        for (int k = 0; k < j; ++k) {
+            const long double *ck = a + (size_t)n * k;
+            const long double ajk = ck[j];
+            for (int i = j; i < n; ++i) cj[i] -= ck[i] * ajk;
+        }
+        long double ajj = cj[j];
+        if (!(ajj > 0.0L)) return j + 1;
+        ajj = sqrtl(ajj);
+        cj[j] = ajj;
+        for (int i = j + 1; i < n; ++i) cj[i] /= ajj;
+    }
+    return 0;
+}
+
+/* L L^T x = b in place, long double (dpotrs('L'), linalg.f90:135-168). */
+static void potrs_ext(int n, const long double *l, long double *b)
+{
+    for (int j = 0; j < n; ++j) {
+        const long double *c = l + (size_t)n * j;
+        b[j] /= c[j];
+        for (int i = j + 1; i < n; ++i) b[i] -= c[i] * b[j];
+    }
+    for (int j = n - 1; j >= 0; --j) {
+        const long double *c = l + (size_t)n * j;
+        long double t = b[j];
+        for (int i = j + 1; i < n; ++i) t -= c[i] * b[i];
+        b[j] = t / c[j];
+    }
+}
+
+int or_loglik_ext(int n, const double *k, int ldk, const double *y, const double *mean,
+                  int likelihood, double *out4, double *lfac)
+{
+    size_t nn = (size_t)n * n;
+    long double *a = (long double *)malloc(sizeof(long double) * nn);
+    long double *r = (long double *)malloc(sizeof(long double) * n);
+    long double *w = (long double *)malloc(sizeof(long double) * n);
+    long double *v = (long double *)malloc(sizeof(long double) * n);
+    if (!a || !r || !w || !v) { free(a); free(r); free(w); free(v); return -1; }
+    for (int j = 0; j < n; ++j)
+        for (int i = j; i < n; ++i) a[i + (size_t)n * j] = (long double)k[i + (size_t)ldk * j];
+    int info = potrf_ext(n, a);
+    if (lfac) {
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) lfac[i + (size_t)n * j] = i >= j ? (double)a[i + (size_t)n * j] : 0.0;
+    }
+    if (info == 0) {
+        long double ld = 0.0L;
+        for (int i = 0; i < n; ++i) ld += logl(a[i + (size_t)n * i]);   /* arith.f90:72-93 */
+        ld *= 2.0L;
+        long double ll, mu = 0.0L, s2 = 0.0L;
+        if (likelihood == FB_LIKELIHOOD_CONCENTRATED) {
+            /* likelihood.f90:220-243 */
+            long double sy = 0.0L, s1 = 0.0L;
+            for (int i = 0; i < n; ++i) { w[i] = y[i]; v[i] = 1.0L; }
+            potrs_ext(n, a, w);
+            potrs_ext(n, a, v);
+            for (int i = 0; i < n; ++i) { sy += w[i]; s1 += v[i]; }
+            mu = sy / s1;
+            for (int i = 0; i < n; ++i) { r[i] = (long double)y[i] - mu; w[i] = r[i]; }
+            potrs_ext(n, a, w);
+            long double q = 0.0L;
+            for (int i = 0; i < n; ++i) q += r[i] * w[i];
+            s2 = q / (long double)n;
+            ll = -0.5L * (long double)n * logl(s2) - 0.5L * ld;
+        } else {
+            /* likelihood.f90:178-191; r = y - m is formed in FP64 as both
+             * the reference and the device do */
+            for (int i = 0; i < n; ++i) { r[i] = (long double)(y[i] - (mean ? mean[i] : 0.0)); w[i] = r[i]; }
+            potrs_ext(n, a, w);
+            long double q = 0.0L;
+            for (int i = 0; i < n; ++i) q += r[i] * w[i];
+            ll = -0.5L * q - 0.5L * ld - 0.5L * (long double)n * logl(2.0L * 3.141592653589793238462643383279503L);
+        }
+        if (out4) { out4[0] = (double)ll; out4[1] = (double)ld; out4[2] = (double)mu; out4[3] = (double)s2; }
+    }
+    free(a); free(r); free(w); free(v);
+    return info;
+}
+
 /* ---- mean functions (src/mean/) ------------------------------------------ */
 
 int or_mean(int kind, int n, int d, const double *x, const double *y, double *m,

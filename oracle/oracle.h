@@ -58,6 +58,27 @@ int or_loglik_grad(int n, int d, const double *x, const double *y, const double 
                    const fb_kernel_spec *spec, double noise, int optimise_noise,
                    int likelihood, const double *theta, double *ll_out, double *grad);
 
+/* The kernel matrix of or_kernel_matrix evaluated in long double (expl, sinl,
+ * no FP64 rounding of the exponents), returned as the nearest FP64 value
+ * k_hi and the remainder k_lo (nullable): the truth each FP64 element is
+ * measured against, in ulps of k_hi. */
+int or_kernel_matrix_ext(int n, int d, const double *x, const fb_kernel_spec *spec,
+                         const double *theta, double *k_hi, double *k_lo, int ldk);
+
+/* Extended-precision truth for one given FP64 matrix: the same likelihood as
+ * or_loglik (likelihood.f90:161-253 after dpotrf, linalg.f90:95) with the
+ * Cholesky factorisation, log-det, solves and reductions in x87 long double
+ * (64-bit significand, unit roundoff 5.4e-20), so that the likelihood of an
+ * ill-conditioned K (a 1e-10 nugget) is known far below FP64 rounding.
+ *   k      n x n FP64 (leading dimension ldk), lower triangle and diagonal
+ *          read, nugget already on the diagonal
+ *   out4   [ll, logdet, mu_hat, sigma2_hat] rounded to FP64
+ *   lfac   (nullable) n x n: the long double factor rounded to FP64
+ * Returns INFO (0, or the first non-positive pivot), -1 on allocation
+ * failure. */
+int or_loglik_ext(int n, const double *k, int ldk, const double *y, const double *mean,
+                  int likelihood, double *out4, double *lfac);
+
 /* Mean functions evaluated once per training set (src/mean/).
  * kind: 0 zero, 1 constant, 2 linear, 3 quadratic.  For the least-squares
  * means beta, xmin (d entries each) and ymin (1) are returned when non-NULL.

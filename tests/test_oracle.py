@@ -263,3 +263,215 @@ def test_baseline_matches_oracle():
         assert np.array_equal(info, binfo)
         np.testing.assert_allclose(bll, ll, rtol=1e-9)
         bl.close()
+
+
+# ---- extended-precision truth (or_loglik_ext) --------------------------------
+
+def _dyadic_factor(n, seed, tiny_from=None):
+    """Lower-triangular L0 with dyadic entries, diagonally dominant (diagonal
+    in [1/2, 1), off-diagonal within 1/4): K = L0 L0' is exact in FP64 and its
+    Cholesky factor is L0 itself.  From row tiny_from on the trailing block is
+    scaled by 2**-16 (diagonal ~1e-5, pivots ~1e-10: a nugget-sized remainder
+    below a smooth part, as a GP kernel with the 1e-10 nugget has), with O(1)
+    coupling to the leading columns; every entry of K then spans at most 50
+    bits, so K is still exact."""
+    rng = np.random.default_rng(seed)
+    l0 = np.tril(rng.integers(-16, 17, (n, n)).astype(float) / 64, -1)
+    l0[np.diag_indices(n)] = rng.integers(32, 64, n).astype(float) / 64
+    if tiny_from is not None:
+        t = tiny_from
+        l0[t:, :t] = rng.integers(-32, 33, (n - t, t)).astype(float) / 64
+        tiny = np.tril(rng.integers(-8, 9, (n - t, n - t)).astype(float), -1)
+        tiny[np.diag_indices(n - t)] = rng.integers(32, 64, n - t)
+        l0[t:, t:] = tiny / 2 ** 22
+    return l0
+
+
+def _exact_quad(l0, r):
+    """r' (L0 L0')^{-1} r = |L0^{-1} r|^2 in rational arithmetic."""
+    from fractions import Fraction
+    n = len(r)
+    z = [Fraction(0)] * n
+    for i in range(n):
+        acc = Fraction(float(r[i]))
+        for k in range(i):
+            acc -= Fraction(float(l0[i, k])) * z[k]
+        z[i] = acc / Fraction(float(l0[i, i]))
+    return float(sum(v * v for v in z))
+
+
+@pytest.mark.parametrize("n,tiny_from", [(12, None), (40, None), (40, 10), (60, 20)])
+def test_ext_exact_dyadic_factor(n, tiny_from):
+    """K = L0 L0' exact in FP64 with cond(K) up to ~1e12: the extended path
+    recovers L0, log det = 2 sum log L0_ii and the quadratic form computed in
+    rational arithmetic, where FP64 Cholesky loses most digits."""
+    l0 = _dyadic_factor(n, seed=n + (tiny_from or 0), tiny_from=tiny_from)
+    k = l0 @ l0.T
+    assert np.array_equal(k, k.T)
+    rng = np.random.default_rng(1)
+    y = rng.normal(size=n)
+    ll, info, out4, lf = oracle.loglik_ext(k, y, None, factor=True)
+    assert info == 0
+    np.testing.assert_allclose(lf, l0, rtol=0, atol=1e-12 * np.abs(l0).max())
+    logdet = 2.0 * np.sum(np.log(np.diag(l0)))
+    quad = _exact_quad(l0, y)
+    np.testing.assert_allclose(out4[1], logdet, rtol=1e-14)
+    ll_ref = -0.5 * quad - 0.5 * logdet - 0.5 * n * np.log(2 * np.pi)
+    np.testing.assert_allclose(ll, ll_ref, rtol=1e-13)
+
+
+def _exact_ldl_loglik(k, r):
+    """log det K and r' K^{-1} r of the FP64 matrix k, exactly: LDL' in
+    rational arithmetic (no square roots), det = prod D_i and
+    r' K^{-1} r = sum (L^{-1} r)_i^2 / D_i; the logarithm is taken of the
+    exact rationals at the end.  Returns (logdet, quad) as Python floats
+    computed from the exact values."""
+    from fractions import Fraction
+    import math
+    n = len(r)
+    a = [[Fraction(float(k[max(i, j), min(i, j)])) for j in range(n)] for i in range(n)]
+    z = [Fraction(float(v)) for v in r]
+    logdet, quad = 0.0, Fraction(0)
+    for j in range(n):
+        dj = a[j][j]
+        assert dj > 0
+        logdet += math.log(dj.numerator) - math.log(dj.denominator)
+        for i in range(j + 1, n):
+            lij = a[i][j] / dj
+            if lij:
+                for c in range(j + 1, i + 1):
+                    a[i][c] -= lij * a[c][j]
+                z[i] -= lij * z[j]
+        quad += z[j] * z[j] / dj
+    return logdet, float(quad)
+
+
+def test_ext_exact_rational_gp_kernel():
+    """The extended truth against exact rational arithmetic on a real GP
+    matrix with the reference's 1e-10 nugget (water-like inputs, rbf on r1,
+    r2 + periodic on the angle, n = 40, cond ~8e11): the extended path is
+    within 1e-11 of the exact likelihood (measured 3.7e-12: 100 times below
+    the 1e-9 it is used to judge), while the FP64 oracle misses it by 1e3
+    times more, so the check is sharp."""
+    rng = np.random.default_rng(6)
+    n = 40
+    x = np.column_stack([rng.normal(1.81, 0.05, n), rng.normal(1.81, 0.05, n), rng.normal(1.82, 0.08, n)])
+    y = 0.2 * (1 - np.exp(-1.2 * (x[:, 0] - 1.81))) ** 2 + 0.1 * (x[:, 2] - 1.82) ** 2
+    spec = oracle.Spec([LEAF_RBF, LEAF_PERIODIC, OP_SUM], [2, 1, 0], [1, 2, 3])
+    th = np.array([0.4, 0.7, 0.2])
+    k = oracle.kernel_matrix(x, spec, th)
+    k[np.diag_indices(n)] += 1e-10
+    m = np.full(n, y.mean())
+    logdet, quad = _exact_ldl_loglik(k, y - m)
+    ll_ref = -0.5 * quad - 0.5 * logdet - 0.5 * n * np.log(2 * np.pi)
+    ll, info, out4 = oracle.loglik_ext(k, y, m)
+    assert info == 0
+    ext_err = abs(ll - ll_ref) / abs(ll_ref)
+    assert ext_err <= 1e-11, ext_err
+    assert abs(out4[1] - logdet) <= 1e-11 * abs(logdet)
+    ll64, info64, _, _ = oracle.loglik(x, y, m, spec, th, noise=1e-10)
+    assert info64 == 0
+    assert abs(ll64 - ll_ref) / abs(ll_ref) > 100 * ext_err
+
+
+@pytest.mark.parametrize("n,a,nug", [(50, 1.0, 1e-10), (300, 2.0, 1e-10), (200, 0.5, 1e-6)])
+def test_ext_closed_form_ill_conditioned(n, a, nug):
+    """K = a 11' + b I with the 1e-10 nugget of the reference (cond ~ n a / b):
+    b is the nugget as FP64 stores it on the diagonal, so the FP64 matrix is
+    exactly a 11' + b I and the Sherman-Morrison forms (evaluated in long
+    double) are the truth for both likelihoods."""
+    LD = np.longdouble
+    b = float((a + nug) - a)                      # the nugget the FP64 diagonal holds
+    k = np.full((n, n), a)
+    k[np.diag_indices(n)] = a + nug
+    assert (np.diag(k) - a == b).all()
+    rng = np.random.default_rng(n)
+    y = 0.01 * rng.normal(size=n) + 0.3
+    aL, bL, nL, yL = LD(a), LD(b), LD(n), y.astype(LD)
+    logdet = (nL - 1) * np.log(bL) + np.log(bL + nL * aL)
+    quad = (yL @ yL - aL * yL.sum() ** 2 / (bL + nL * aL)) / bL
+    ll_ref = -0.5 * quad - 0.5 * logdet - 0.5 * nL * np.log(LD(2) * LD(np.pi))
+    ll, info, out4 = oracle.loglik_ext(k, y)
+    assert info == 0
+    ext_err = abs(LD(ll) - ll_ref) / abs(ll_ref)
+    # the extended path's error is of order cond(K) x 5.4e-20 per pivot
+    # (measured 8e-13 on the quadratic form at n = 50, cond 5e11)
+    assert abs(LD(out4[1]) - logdet) <= 1e-11 * abs(logdet)
+    assert ext_err <= 1e-11, ext_err
+    llc, infoc, o4c = oracle.loglik_ext(k, y, likelihood="concentrated")
+    ybar = yL.mean()
+    s2 = ((yL - ybar) @ (yL - ybar)) / nL / bL
+    # mu_hat = 1'K^-1 y / 1'K^-1 1 is a ratio of two sums that cancel from
+    # ~1/b down to ~1/(n a): its extended error is ~cond(K) x 5.4e-20
+    # (measured 1.1e-9 at n = 300, cond 6e12)
+    assert abs(LD(o4c[2]) - ybar) <= 1e-8 * abs(ybar)
+    assert abs(LD(o4c[3]) - s2) <= 1e-8 * s2
+    assert abs(LD(llc) - (-0.5 * nL * np.log(s2) - 0.5 * logdet)) <= 1e-10 * abs(llc)
+
+
+def test_ext_agrees_with_fp64_on_conditioned_cases():
+    """On well-conditioned matrices the extended and FP64 paths agree to a few
+    ulp of FP64, both likelihoods, with the mean subtracted."""
+    rng = np.random.default_rng(4)
+    x = rng.uniform(1.5, 3.0, (90, 5))
+    y = np.sin(x).sum(axis=1)
+    spec = oracle.Spec([LEAF_RBF], [5], [1, 2, 3, 4, 5])
+    th = np.full(5, 1.5)
+    m = np.full(90, y.mean())
+    k = oracle.kernel_matrix(x, spec, th) + 1e-2 * np.eye(90)
+    for lik in ("marginal", "concentrated"):
+        ll, info, _, o4 = oracle.loglik(x, y, m, spec, th, noise=1e-2, likelihood=lik)
+        lle, infoe, o4e = oracle.loglik_ext(k, y, m, likelihood=lik)
+        assert info == infoe == 0
+        np.testing.assert_allclose(lle, ll, rtol=1e-12)
+        np.testing.assert_allclose(o4e[1:], o4[1:], rtol=1e-11, atol=1e-14)
+
+
+def test_ext_info_on_non_pd():
+    """A negative nugget on a duplicated point: INFO at that column, as the
+    FP64 oracle and LAPACK report."""
+    rng = np.random.default_rng(5)
+    x = rng.uniform(1.5, 3.0, (30, 3))
+    x[20] = x[7]
+    spec = oracle.Spec([LEAF_RBF], [3], [1, 2, 3])
+    k = oracle.kernel_matrix(x, spec, np.zeros(3)) - 1e-8 * np.eye(30)
+    _, info, _ = oracle.loglik_ext(k, rng.normal(size=30))
+    _, info64 = oracle.potrf(k)
+    assert info == info64 == 21
+
+
+@pytest.mark.parametrize("name", ["rbf", "periodic", "sum", "product", "cyclic", "linear_sum"])
+def test_kernel_matrix_ext_vs_fp64(name):
+    """The long double kernel build and the FP64 one agree to the rounding
+    the FP64 exponent carries: |err| <= 2 + 16 |x| ulp, where x is
+    the exponent (K ~ e^x; at least 1 for sums of terms), with a mean signed error below 1 ulp; a formula
+    slip in either (a dropped term, sin(D) for sin(D/2), a lost factor 1/2)
+    shows as thousands of ulps.  (The FP64 restatement of the reference's
+    formulas is itself biased by -0.1 to +0.7 ulp on average, depending on
+    the kernel: its rounding of l = exp(theta), of (1/exp(theta))**2 and of
+    sin(sqrt(D)/2) does not average out.)"""
+    rng = np.random.default_rng(8)
+    n, d = 120, 6
+    x = np.column_stack([rng.uniform(1.8, 2.9, n), rng.uniform(1.8, 2.9, n), rng.uniform(1.6, 2.2, n),
+                         rng.uniform(1.8, 2.9, n), rng.uniform(1.6, 2.2, n), rng.uniform(-np.pi, np.pi, n)])
+    specs = {
+        "rbf": oracle.Spec([LEAF_RBF], [5], [1, 2, 3, 4, 5]),
+        "periodic": oracle.Spec([LEAF_PERIODIC], [2], [3, 6]),
+        "sum": oracle.Spec([LEAF_RBF, LEAF_PERIODIC, OP_SUM], [5, 1, 0], [1, 2, 3, 4, 5, 6]),
+        "product": oracle.Spec([LEAF_RBF, LEAF_PERIODIC, OP_PROD], [5, 1, 0], [1, 2, 3, 4, 5, 6]),
+        "cyclic": oracle.Spec([LEAF_RBF_CYCLIC], [6], [1, 2, 3, 4, 5, 6]),
+        "linear_sum": oracle.Spec([LEAF_RBF, LEAF_LINEAR, OP_SUM], [3, 2, 0], [1, 2, 4, 3, 5]),
+    }
+    spec = specs[name]
+    th = rng.uniform(0.0, 1.0, oracle.nparams(spec))
+    k64 = oracle.kernel_matrix(x, spec, th)
+    hi, lo = oracle.kernel_matrix_ext(x, spec, th)
+    tri = np.tril_indices(n)
+    err = oracle.ulp_error(k64, hi, lo)[tri]
+    kabs = np.abs(hi[tri])
+    bound = 2.0 + 16.0 * np.maximum(np.abs(np.log(np.maximum(kabs, 1e-300))), 1.0)
+    assert (np.abs(err) <= bound).all(), (name, np.abs(err).max())
+    # the cyclic wrap subtracts the FP64 value of pi (as the reference does):
+    # a further bias of ~2 ulp on that kernel
+    assert abs(err.mean()) <= (3.0 if name == "cyclic" else 1.0), (name, err.mean())
+    assert np.abs(err).max() >= 0.25      # the two really are different computations
