@@ -42,11 +42,11 @@ __global__ void __launch_bounds__(64) hyper_kernel(const KernelProgram *__restri
         const int nd = p.leaf_ndim[l];
         if (kind == kRbf || kind == kRbfCyclic) {
             // rbf_kernel.f90:113-119, rbf_cyclic_kernel.f90:109-115: l = exp(theta)
-            for (int q = threadIdx.x; q < nd; q += blockDim.x) h[off + q] = -0.5 * exp(th[off + q]);
+            for (int q = threadIdx.x; q < nd; q += blockDim.x) h[off + q] = -0.5 * exp_cr(th[off + q]);
         } else if (kind == kPeriodic) {
             // periodic_kernel.f90:111-118: l = 1/exp(theta); l = l*l
             for (int q = threadIdx.x; q < nd; q += blockDim.x) {
-                double r = 1.0 / exp(th[off + q]);
+                double r = 1.0 / exp_cr(th[off + q]);
                 h[off + q] = -2.0 * (r * r);
             }
         }
@@ -61,7 +61,7 @@ __global__ void __launch_bounds__(64) hyper_kernel(const KernelProgram *__restri
             if (kind == kLinear) {
                 // linear_kernel.f90:70-76, order 0: sum(1/c), c = exp(-theta)
                 double t = 0.0;
-                for (int q = 0; q < p.leaf_ndim[l]; ++q) t += 1.0 / exp(-th[off + q]);
+                for (int q = 0; q < p.leaf_ndim[l]; ++q) t += 1.0 / exp_cr(-th[off + q]);
                 leafval[l] = t;
             } else if (kind == kConstant) {
                 leafval[l] = th[off];  // constant.f90:86-90

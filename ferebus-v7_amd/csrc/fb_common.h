@@ -66,6 +66,63 @@ __host__ __device__ inline double exp_nonpos(double x, const double *c)
     return __builtin_ldexp(p, (int)k);
 }
 
+// exp(x) for the hyperparameter transforms (l = exp(theta) and its
+// relatives, a few per particle): the same reduction as exp_nonpos with ln2
+// in three parts, r = x - k ln2 kept as a double-double, and
+// exp(r) = 1 + r + r^2 (1/2 + r (1/6 + r R(r))) with the outer three levels
+// in double-double arithmetic (R, the tail from 1/24, in double), rounded
+// once at the end.  The result before that rounding is within ~2^-63 of
+// exp(x), so it is the correctly rounded value except in about one case in
+// a thousand -- the value the reference's FP64 host exp gives, so both builds
+// start every covariance element from the same lengthscales (a one-ulp
+// difference in a lengthscale biases every element it touches).  |x| above
+// 700 falls back to the library exp.
+__host__ __device__ inline double exp_cr(double x)
+{
+    if (!(x > -700.0 && x < 700.0)) return exp(x);
+    const double L1 = 6.93147180369123816490e-01;     // ln2, 32 significant bits
+    const double L2 = 1.90821492927058770002e-10;
+    const double L3 = 1.1612227229362532e-26;         // ln2 - L1 - L2 (rounded)
+    const double C6H = 1.0 / 6.0, C6L = 9.25185853854297e-18;   // 1/6 = C6H + C6L
+    const double k = __builtin_rint(x * 1.4426950408889634074);
+    const double t = __builtin_fma(-k, L1, x);        // exact
+    const double p = k * L2, pe = __builtin_fma(k, L2, -p);
+    const double rh = t - p;
+    const double rl = ((t - rh) - p) - pe - k * L3;   // r = rh + rl, |rh| >> |p|
+    // R(r) = sum_{j = 4..16} r^(j-4) / j! (the first omitted term, r^17/17!,
+    // is below 5e-23 for |r| <= ln2/2)
+    double R = 1.0 / 20922789888000.0;                 // 1/16!
+    R = __builtin_fma(R, rh, 1.0 / 1307674368000.0);
+    R = __builtin_fma(R, rh, 1.0 / 87178291200.0);
+    R = __builtin_fma(R, rh, 1.0 / 6227020800.0);
+    R = __builtin_fma(R, rh, 1.0 / 479001600.0);
+    R = __builtin_fma(R, rh, 1.0 / 39916800.0);
+    R = __builtin_fma(R, rh, 1.0 / 3628800.0);
+    R = __builtin_fma(R, rh, 1.0 / 362880.0);
+    R = __builtin_fma(R, rh, 1.0 / 40320.0);
+    R = __builtin_fma(R, rh, 1.0 / 5040.0);
+    R = __builtin_fma(R, rh, 1.0 / 720.0);
+    R = __builtin_fma(R, rh, 1.0 / 120.0);
+    R = __builtin_fma(R, rh, 1.0 / 24.0);
+    // U = 1/6 + r R
+    const double m = rh * R, me = __builtin_fma(rh, R, -m);
+    const double uh = C6H + m, ue = (C6H - uh) + m;   // |C6H| > |m|
+    const double ul = ue + me + C6L + rl * R;
+    // V = 1/2 + r U
+    const double v1 = rh * uh, v1e = __builtin_fma(rh, uh, -v1);
+    const double v1l = v1e + rh * ul + rl * uh;
+    const double vh = 0.5 + v1, ve = (0.5 - vh) + v1;
+    const double vl = ve + v1l;
+    // W = r^2 V
+    const double sh = rh * rh, sl = __builtin_fma(rh, rh, -sh) + 2.0 * rh * rl;
+    const double wh = sh * vh, wl = __builtin_fma(sh, vh, -wh) + sh * vl + sl * vh;
+    // 1 + r + W
+    const double s = rh + wh, se = (rh - s) + wh;     // |rh| >= |wh|
+    const double s_l = se + rl + wl;
+    const double a = 1.0 + s, ae = (1.0 - a) + s;     // 1 > |s|
+    return __builtin_ldexp(a + (ae + s_l), (int)k);
+}
+
 constexpr int kMaxLeaves = 8;     // stationary / constant leaves of the kernel tree
 constexpr int kMaxTerms = 16;     // sum-of-products terms after expansion
 constexpr int kMaxDims = 512;     // total active dimensions over all leaves
@@ -102,7 +159,10 @@ constexpr int kNBI = 64;      // diagonal block order (potrf / trsm)
 constexpr int kMB = 64;       // rows of one update tile
 constexpr int kKC = 32;       // depth of one LDS stage of the update GEMM
 constexpr int kThreads = 256; // work-group size of every kernel (4 wave64)
-constexpr int kDinv = 4 * 16 * 16;   // doubles per 64x64 diagonal block: four 16x16 inverses
+// doubles per 64x64 diagonal block: four 16x16 inverses, then one flag per
+// 16x16 block (nonzero: ill-conditioned, the panel solves substitute)
+constexpr int kDinvFlags = 4 * 16 * 16;
+constexpr int kDinv = kDinvFlags + 4;
 
 // Per-particle words of the chained lock-step schedule (two launches per
 // panel, or one with the row solves fused; fb_potrf.hip): flags and XCD

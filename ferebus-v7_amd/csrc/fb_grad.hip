@@ -270,7 +270,7 @@ __global__ void __launch_bounds__(kThreads) grad_pair_kernel(GradArgs a)
             const int off = p.leaf_par_off[l];
             if (p.leaf_kind[l] == kLinear) {
                 double t = 0.0;
-                for (int q = 0; q < p.leaf_ndim[l]; ++q) t += 1.0 / exp(-th[off + q]);
+                for (int q = 0; q < p.leaf_ndim[l]; ++q) t += 1.0 / exp_cr(-th[off + q]);
                 u[l] = t;
             } else if (p.leaf_kind[l] == kConstant) {
                 u[l] = th[off];
@@ -424,9 +424,9 @@ __global__ void grad_reduce_kernel(const double *__restrict__ partial, int ntile
             if (q < off || q >= off + np_l || q >= p.n_par_kernel) continue;
             switch (p.leaf_kind[l]) {
             case kRbf:
-            case kRbfCyclic: mult = -0.5 * exp(th); break;                // dh/dtheta = h
-            case kPeriodic: { const double r = 1.0 / exp(th); mult = 4.0 * (r * r); break; }   // -2 h
-            case kLinear: mult = 1.0 / exp(-th); break;                  // du/dtheta = exp(theta)
+            case kRbfCyclic: mult = -0.5 * exp_cr(th); break;                // dh/dtheta = h
+            case kPeriodic: { const double r = 1.0 / exp_cr(th); mult = 4.0 * (r * r); break; }   // -2 h
+            case kLinear: mult = 1.0 / exp_cr(-th); break;                  // du/dtheta = exp(theta)
             default: mult = 1.0;
             }
         }

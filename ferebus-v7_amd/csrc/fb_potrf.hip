@@ -430,7 +430,9 @@ __attribute__((amdgpu_waves_per_eu(KC == 8 ? 3 : 2, KC == 8 ? 3 : 2))) panel_cha
 // Launch 2 of panel c0, chained: every row tile below the panel is solved
 // (solve_rows_tile: launches 2 and 3 of the three-launch schedule for one
 // tile, each wave on its own 16 rows; no LDS).
-__global__ void __launch_bounds__(kThreads) solve_rows_kernel(double *__restrict__ K, int np, int c0, int S,
+// (three waves per SIMD: at most 168 VGPRs)
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3)))
+solve_rows_kernel(double *__restrict__ K, int np, int c0, int S,
                                                               const int *__restrict__ info,
                                                               const double *__restrict__ dinv)
 {

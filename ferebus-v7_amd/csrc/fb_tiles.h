@@ -285,6 +285,31 @@ __device__ inline double readlane_f64(double x, int l)
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+// Ratio of the largest to the smallest pivot of a 16x16 diagonal block above
+// which the panel solves substitute inside the block instead of multiplying
+// by its inverse: a product with the inverse is not backward stable for an
+// ill-conditioned block, and at a 1e-10 nugget the first block of columns
+// (pivots falling from ~1 to ~1e-5 as the smooth part of the kernel is
+// exhausted) then perturbs every later pivot by ~1e-5 relative, log det K by
+// ~1e-4 -- ten times what the substitution order of dpotf2 gives.
+constexpr double kSubstRatio = 32.0;
+
+// Wave-wide: lanes 0..15 hold the pivots L(c, c) of a 16x16 diagonal block
+// (real = false for identity columns beyond n).  Lane 0 stores the block's
+// flag at *F: 1 when its largest pivot exceeds kSubstRatio times its
+// smallest, else 0; trsm16_rows reads it to choose substitution.
+template <bool WT>
+__device__ __forceinline__ void store_subst_flag(double piv, bool real, double *F, int lane)
+{
+    double lo = real ? piv : 1e300, hi = real ? piv : 0.0;
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1) {
+        lo = fmin(lo, __shfl_xor(lo, off, 16));
+        hi = fmax(hi, __shfl_xor(hi, off, 16));
+    }
+    if (lane == 0) st_f64<WT>(F, hi > kSubstRatio * lo ? 1.0 : 0.0);
+}
+
 // Factors the 64x64 diagonal block at (c0, c0) of the matrix at A,
 // right-looking in four 16-column panels, the block staged in LDS:
 //  * wave 0 factors panel j (lane = row) in groups of four columns: inside
@@ -337,7 +362,9 @@ __device__ inline int potrf64_block(double *__restrict__ A, int np, int n, int c
     // completion of panel q (columns 16q .. 16q + 15, all real): one wave each
     auto finish = [&](int q) {
         if (wave == 1) {
-            if (lane < 16) lg += log(Ls[(16 * q + lane) * kPLD + 16 * q + lane]);
+            const double piv = Ls[(16 * q + (lane & 15)) * kPLD + 16 * q + (lane & 15)];
+            if (lane < 16) lg += log(piv);
+            store_subst_flag<WT>(piv, true, dinv + kDinvFlags + q, lane);
         } else if (wave == 2) {
             // column-oriented forward substitution of L_qq x = e_c, lane c < 16
             if (lane < 16) {
@@ -355,7 +382,8 @@ __device__ inline int potrf64_block(double *__restrict__ A, int np, int n, int c
                 }
                 double *W = dinv + q * 256 + lane * 16;
 #pragma unroll
-                for (int i = 0; i < 16; ++i) st_f64<WT>(W + i, x[i]);
+                for (int i = 0; i < 16; ++i)
+                    st_f64<WT>(W + i, x[i]);
             }
         } else if (wave == 3) {
             for (int e = lane; e < 16 * B; e += 64) {
@@ -461,8 +489,12 @@ __device__ inline int potrf64_block(double *__restrict__ A, int np, int n, int c
         __syncthreads();
         if (wave == 1) {
             // logs of the real columns only
-            for (int q = 0; q < 4; ++q)
-                if (lane < 16 && 16 * q + lane < ncol) lg += log(Ls[(16 * q + lane) * kPLD + 16 * q + lane]);
+            for (int q = 0; q < 4; ++q) {
+                const double piv = Ls[(16 * q + (lane & 15)) * kPLD + 16 * q + (lane & 15)];
+                const bool real = 16 * q + (lane & 15) < ncol;
+                if (lane < 16 && real) lg += log(piv);
+                store_subst_flag<WT>(piv, real, dinv + kDinvFlags + q, lane);
+            }
         } else {
             for (int q = 0; q < 4; ++q) {
                 const int w = wave;   // waves 0, 2, 3 share the stores and inverses
@@ -482,7 +514,8 @@ __device__ inline int potrf64_block(double *__restrict__ A, int np, int n, int c
                         }
                         double *W = dinv + q * 256 + lane * 16;
 #pragma unroll
-                        for (int i = 0; i < 16; ++i) st_f64<WT>(W + i, x[i]);
+                        for (int i = 0; i < 16; ++i)
+                    st_f64<WT>(W + i, x[i]);
                     }
                 } else {
                     for (int e = lane + (w == 3 ? 64 : 0); e < 16 * B; e += 128) {
@@ -503,6 +536,50 @@ __device__ inline int potrf64_block(double *__restrict__ A, int np, int n, int c
     return 0;
 }
 
+// Forward substitution inside one 16x16 diagonal block L_jj (L_jj(i, k) at
+// Ljj[k * ldl + i], W(c, c) = 1 / L(c, c) in Wj) of the 16 rows whose
+// right-hand sides r hold column 4v + l/16 of row l % 16 in lane l,
+// component v: y_c = (r_c - sum_{k<c} y_k L(c, k)) / L(c, c), column after
+// column as in dpotf2, the division a product with W(c, c).  It runs only
+// for flagged (ill-conditioned) blocks.
+template <bool CALL>
+__device__ __forceinline__ d4 subst16_body(d4 r, const double *__restrict__ Wj, const double *__restrict__ Ljj, int ldl)
+{
+    const int lane = threadIdx.x & 63;
+    const int lr = lane & 15, lq = lane >> 4;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+#pragma unroll 1
+        for (int q = 0; q < 4; ++q) {
+            const int c = 4 * v + q;
+            const double yc = __shfl(r[v] * Wj[c * 16 + c], lr + 16 * q, 64);
+            if (lq == q) r[v] = yc;
+#pragma unroll
+            for (int v2 = v; v2 < 4; ++v2) {
+                const int col = 4 * v2 + lq;
+                if (col > c) r[v2] = fma(-yc, Ljj[(size_t)c * ldl + col], r[v2]);
+            }
+        }
+    }
+    return r;
+}
+
+__device__ __attribute__((noinline)) d4 subst16_call(d4 r, const double *__restrict__ Wj,
+                                                     const double *__restrict__ Ljj, int ldl)
+{
+    return subst16_body<true>(r, Wj, Ljj, ldl);
+}
+
+// CALL: the substitution as a call (its registers and loads kept out of the
+// caller, at the price of a stack frame), else inlined; chosen per kernel
+// by register use.
+template <bool CALL>
+__device__ __forceinline__ d4 subst16(d4 r, const double *__restrict__ Wj, const double *__restrict__ Ljj, int ldl)
+{
+    if constexpr (CALL) return subst16_call(r, Wj, Ljj, ldl);
+    else return subst16_body<false>(r, Wj, Ljj, ldl);
+}
+
 // X = C L_bb^{-T} in place on the 64x64 tile at Ct (leading dimension ldc),
 // L_bb the factored diagonal block at Lb (leading dimension ldl) with the
 // inverses of its 16x16 diagonal blocks in dinv; IDENT: C is the identity
@@ -519,6 +596,7 @@ __device__ inline int potrf64_block(double *__restrict__ A, int np, int n, int c
 // L_jk (j > k) in A layout (lane l holds L_jk[l%16][4kk + l/16]) and the
 // 16x16 inverses likewise, each read where it is used (held all at once
 // they take 80 registers).
+template <bool CALL = false>
 __device__ inline void trsm16_rows(const d4 t[4], const double *__restrict__ Lb, int ldl,
                                    const double *__restrict__ dinv, d4 y[4])
 {
@@ -535,9 +613,19 @@ __device__ inline void trsm16_rows(const d4 t[4], const double *__restrict__ Lb,
 #pragma unroll
             for (int kk = 0; kk < 4; ++kk) r = mfma_f64(-lf[kk], y[k][kk], r);
         }
+        const double *Wj = dinv + j * 256;
+        if (dinv[kDinvFlags + j] != 0.0) {
+            // an ill-conditioned diagonal block (store_subst_flag): forward
+            // substitution inside it, column c of the block after column c - 1
+            // as in dpotf2, y_c = (r_c - sum_{k<c} y_k L(c, k)) / L(c, c),
+            // the division a product with W(c, c) = 1 / L(c, c); column c of
+            // row lr sits in lane lr + 16 (c % 4), component c / 4
+            y[j] = subst16<CALL>(r, Wj, Lb + (size_t)(16 * j) * ldl + 16 * j, ldl);
+            continue;
+        }
         double wf[4];
 #pragma unroll
-        for (int kk = 0; kk < 4; ++kk) wf[kk] = dinv[j * 256 + (4 * kk + lq) * 16 + lr];
+        for (int kk = 0; kk < 4; ++kk) wf[kk] = Wj[(4 * kk + lq) * 16 + lr];
         d4 o = d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk) o = mfma_f64(wf[kk], r[kk], o);
@@ -600,7 +688,7 @@ __device__ inline void solve_rows16(double *A, int np, int c0, int row, const do
     for (int j = 0; j < 4; ++j)
 #pragma unroll
         for (int v = 0; v < 4; ++v) t[j][v] = C0[(size_t)(16 * j + 4 * v + lq) * np];
-    trsm16_rows(t, A + (size_t)c0 * np + c0, np, dinv_s + (size_t)(c0 / kNBI) * kDinv, x0);
+    trsm16_rows<true>(t, A + (size_t)c0 * np + c0, np, dinv_s + (size_t)(c0 / kNBI) * kDinv, x0);
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
@@ -617,7 +705,7 @@ __device__ inline void solve_rows16(double *A, int np, int c0, int row, const do
         for (int j = 0; j < 4; ++j) t[j] = mfma_f64(Bg[(size_t)(4 * kk + lq) * np + 16 * j], af, t[j]);
     }
     d4 x1[4];
-    trsm16_rows(t, A + (size_t)c1 * np + c1, np, dinv_s + (size_t)(c1 / kNBI) * kDinv, x1);
+    trsm16_rows<true>(t, A + (size_t)c1 * np + c1, np, dinv_s + (size_t)(c1 / kNBI) * kDinv, x1);
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll

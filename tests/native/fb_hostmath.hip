@@ -2,6 +2,7 @@
 // for CPU accuracy tests (tests/test_device_math.py).  Test infrastructure:
 // the product library does not link it.
 //   exp_nonpos               (fb_common.h) runs on every covariance element
+//   exp_cr                   (fb_common.h) gives every lengthscale
 //   goldschmidt_sqrt_rsqrt   (fb_tiles.h) refines every Cholesky pivot's
 //                            square root and its reciprocal
 #include "../../ferebus-v7_amd/csrc/fb_tiles.h"
@@ -21,3 +22,8 @@ void fbh_sqrt_rsqrt(int n, const double *x, const double *y0, double *sq, double
 }
 
 }  // extern "C"
+
+extern "C" void fbh_exp_cr(int n, const double *x, double *y)
+{
+    for (int i = 0; i < n; ++i) y[i] = fb::exp_cr(x[i]);
+}

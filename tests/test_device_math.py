@@ -1,7 +1,8 @@
 """CPU accuracy checks of the hand-written device math (runs without a GPU).
 
 The covariance build evaluates every element with fb::exp_nonpos
-(fb_common.h), and the Cholesky factorisation takes every pivot's square
+(fb_common.h) from lengthscales given by fb::exp_cr, and the Cholesky
+factorisation takes every pivot's square
 root and its reciprocal from fb::goldschmidt_sqrt_rsqrt (fb_tiles.h) fed
 by the hardware's reciprocal square root estimate.  Both are compiled for
 the host from the same source (tests/native/fb_hostmath.hip) and compared
@@ -28,6 +29,8 @@ def lib():
     h = ctypes.CDLL(str(LIB))
     h.fbh_exp_nonpos.argtypes = [ctypes.c_int, P_DBL, P_DBL]
     h.fbh_exp_nonpos.restype = None
+    h.fbh_exp_cr.argtypes = [ctypes.c_int, P_DBL, P_DBL]
+    h.fbh_exp_cr.restype = None
     h.fbh_sqrt_rsqrt.argtypes = [ctypes.c_int, P_DBL, P_DBL, P_DBL, P_DBL]
     h.fbh_sqrt_rsqrt.restype = None
     return h
@@ -109,3 +112,28 @@ def test_goldschmidt_sqrt_rsqrt(lib):
               f"(mean {er.mean():+.4f})")
         assert np.abs(es).max() <= 0.5 and np.abs(er).max() <= 2.0
         assert abs(es.mean()) <= 0.01 and abs(er.mean()) <= 0.02
+
+
+def test_exp_cr_matches_correct_rounding(lib):
+    """The hyperparameter exp: within 0.5 ulp of expl everywhere on
+    [-700, 700], equal to the correctly rounded value (expl rounded once to
+    FP64) in >= 99.9 % of points, and equal to the C library's exp -- the
+    reference's FP64 host exp -- in >= 99.9 % of the theta range."""
+    rng = np.random.default_rng(2)
+    x = np.concatenate([rng.uniform(-40, 40, 200_000), rng.uniform(-1, 1, 100_000),
+                        rng.uniform(-700, 700, 50_000), [0.0, 1.0, -1.0, 0.5 * np.log(2.0)]])
+    y = np.empty_like(x)
+    lib.fbh_exp_cr(len(x), _p(x), _p(y))
+    t = np.exp(x.astype(LD))
+    err = _ulps(y, t)
+    cr = np.asarray(t, dtype=np.float64)
+    print(f"exp_cr: max |err| {np.abs(err).max():.4f} ulp, equal to the rounded long double value "
+          f"{np.mean(y == cr):.5f}")
+    assert np.abs(err).max() <= 0.5 + 1e-3
+    assert np.mean(y == cr) >= 0.999
+    assert y[-4] == 1.0
+    libm = ctypes.CDLL("libm.so.6")
+    libm.exp.restype = ctypes.c_double
+    libm.exp.argtypes = [ctypes.c_double]
+    g = np.array([libm.exp(v) for v in x[:50_000]])
+    assert np.mean(g == y[:50_000]) >= 0.999

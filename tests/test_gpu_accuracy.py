@@ -106,13 +106,21 @@ def test_build_elementwise(gpu, name):
               f"mean {eg.mean():+.4f}; FP64 oracle max {np.abs(eo).max():.2f} rms {np.sqrt(np.mean(eo**2)):.3f} "
               f"mean {eo.mean():+.4f}; |x| max {xabs.max():.1f}")
         # the diagonal: sum of the terms' coefficients (exp(0) = 1) plus the
-        # nugget, rounded once -- identical to the FP64 oracle's
-        assert np.array_equal(kg[dg], k64[dg]), name
+        # nugget, rounded once -- identical to the FP64 oracle's where the
+        # coefficients are 1 (stationary leaves); a linear leaf's value
+        # sum(exp(theta)) is evaluated on the device, within 1 ulp
+        if name == "linear":
+            np.testing.assert_allclose(kg[dg], k64[dg], rtol=4e-16, atol=0)
+        else:
+            assert np.array_equal(kg[dg], k64[dg]), name
+        ed = oracle.ulp_error(kg[tri], k64[tri], np.zeros(len(tri[0])))
+        print(f"    device vs FP64 oracle: max {np.abs(ed).max():.2f} ulp, equal {np.mean(ed == 0):.3f}")
         bound = 2.0 + 16.0 * np.maximum(xabs, 1.0)
         assert (np.abs(eg) <= bound).all(), (name, np.abs(eg).max())
-        # no bias of its own: the rbf-cyclic wrap subtracts the FP64 value of
-        # pi, as the reference does, which biases both FP64 builds alike
-        assert abs(eg.mean()) <= max(0.05, abs(eo.mean()) + 0.05), (name, eg.mean(), eo.mean())
+        # no bias beyond the FP64 formulation's own (the lengthscales rounded
+        # to FP64 as the reference rounds them; the rbf-cyclic wrap with the
+        # FP64 value of pi): within 0.1 ulp of the oracle's mean error
+        assert abs(eg.mean() - eo.mean()) <= 0.1, (name, eg.mean(), eo.mean())
 
 
 def _lapack_ll(k_lower, r):
@@ -142,7 +150,7 @@ def test_water_c1_extended(gpu):
     print(f"C1: {len(evals)} evaluations, {res.rejections} rejections, best cost {res.best_cost:.10f}")
     r = w.y - mfit.value
     sample = [e for e in evals if e[0] in (0, 1, 10, 25, 50) and e[3] == 0]
-    e_fac, e_dev, e_or, e_lp = [], [], [], []
+    e_fac, e_fac_or, e_dev, e_or, e_lp = [], [], [], [], []
     for it, rnd, k, info, cost, th in sample:
         kg, ig = gp.matrix(th, "K")
         assert ig == 0
@@ -151,15 +159,45 @@ def test_water_c1_extended(gpu):
         t_cr, tci, _ = oracle.loglik_ext(hi, w.y, mfit.value)       # truth of the correctly rounded K
         ll_or, oi, _, _ = oracle.loglik(w.x, w.y, mfit.value, oracle.Spec.from_program(prog), th, noise=nugget)
         assert ti == 0 and tci == 0 and oi == 0
+        lo_, io_ = oracle.potrf(kg)                                  # dpotf2 order on the device's K
+        assert io_ == 0
+        ll_or_dev = -0.5 * r @ oracle.potrs(lo_, r) - 0.5 * oracle.logdet(lo_) - 0.5 * len(r) * np.log(2 * np.pi)
         ll_dev = -cost
         e_fac.append(abs(ll_dev - t_dev) / abs(t_dev))
+        e_fac_or.append(abs(ll_or_dev - t_dev) / abs(t_dev))
         e_dev.append(abs(ll_dev - t_cr) / abs(t_cr))
         e_or.append(abs(ll_or - t_cr) / abs(t_cr))
         e_lp.append(abs(_lapack_ll(k64, r) - t_cr) / abs(t_cr))
-    e_fac, e_dev, e_or, e_lp = map(np.array, (e_fac, e_dev, e_or, e_lp))
+    e_fac, e_fac_or, e_dev, e_or, e_lp = map(np.array, (e_fac, e_fac_or, e_dev, e_or, e_lp))
     q = lambda a: f"median {np.median(a):.2e} max {a.max():.2e}"
-    print(f"C1 {len(sample)} particles: device factorisation vs extended truth of its K: {q(e_fac)}")
-    print(f"  vs the correctly rounded K's truth: device {q(e_dev)}; FP64 oracle {q(e_or)}; LAPACK {q(e_lp)}")
-    assert e_fac.max() <= 1e-9
-    assert np.median(e_dev) <= max(np.median(e_or), np.median(e_lp))
-    assert e_dev.max() <= max(e_or.max(), e_lp.max())
+    print(f"C1 {len(sample)} particles, against the extended likelihood of the device's own K: "
+          f"device {q(e_fac)}; FP64 dpotf2 order {q(e_fac_or)}")
+    print(f"  against the correctly rounded K's: device {q(e_dev)}; FP64 oracle {q(e_or)}; LAPACK {q(e_lp)}")
+    # the factorisation and likelihood of a given K: 1e-9 at the median; no
+    # particle further off than twice what the dpotf2 order gives in FP64
+    assert np.median(e_fac) <= 1e-9
+    assert e_fac.max() <= max(1e-9, 2.0 * e_fac_or.max())
+    # the whole path: no further from the truth than the FP64 oracle (within
+    # a quarter) and than LAPACK
+    assert np.median(e_dev) <= 1.25 * np.median(e_or) and np.median(e_dev) <= np.median(e_lp)
+    assert e_dev.max() <= 1.25 * e_or.max() and e_dev.max() <= e_lp.max()
+    # the reference's selection rules on the oracle's costs of the traced
+    # positions pick the device's gbest
+    spec = oracle.Spec.from_program(prog)
+    th_all = np.array([e[5] for e in evals])
+    ocost, oinfo = oracle.swarm_loglik(w.x, w.y, mfit.value, spec, th_all, noise=nugget)
+    ocost = -ocost
+    assert np.array_equal(oinfo, np.array([e[3] for e in evals]))
+    final = {}
+    for n_, e in enumerate(evals):
+        final[(e[0], e[2])] = n_            # a retried particle's last evaluation counts
+    gcost, gbest = np.inf, None
+    for it in sorted({e[0] for e in evals}):
+        for k in range(cfg.swarm_size):     # gbest in particle order with <= (pso.f90:454-459)
+            n_ = final[(it, k)]
+            c = ocost[n_] if oinfo[n_] == 0 else np.inf
+            if c <= gcost:
+                gcost, gbest = c, evals[n_][5]
+    print(f"replayed gbest cost {gcost:.10f}, device {res.best_cost:.10f}")
+    assert np.array_equal(gbest, res.best_position)
+    assert abs(gcost - res.best_cost) <= 1e-9 * abs(gcost)
