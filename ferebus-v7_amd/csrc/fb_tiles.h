@@ -541,9 +541,11 @@ __device__ inline int potrf64_block(double *__restrict__ A, int np, int n, int c
 // right-hand sides r hold column 4v + l/16 of row l % 16 in lane l,
 // component v: y_c = (r_c - sum_{k<c} y_k L(c, k)) / L(c, c), column after
 // column as in dpotf2, the division a product with W(c, c).  It runs only
-// for flagged (ill-conditioned) blocks.
-template <bool CALL>
-__device__ __forceinline__ d4 subst16_body(d4 r, const double *__restrict__ Wj, const double *__restrict__ Ljj, int ldl)
+// for flagged (ill-conditioned) blocks, as a call: one out-of-line copy
+// keeps its registers and code out of the solves' hot path (inlined into
+// every solve it cost the factorisation 2-4 %).
+__device__ __attribute__((noinline)) d4 subst16(d4 r, const double *__restrict__ Wj, const double *__restrict__ Ljj,
+                                                int ldl)
 {
     const int lane = threadIdx.x & 63;
     const int lr = lane & 15, lq = lane >> 4;
@@ -564,22 +566,6 @@ __device__ __forceinline__ d4 subst16_body(d4 r, const double *__restrict__ Wj, 
     return r;
 }
 
-__device__ __attribute__((noinline)) d4 subst16_call(d4 r, const double *__restrict__ Wj,
-                                                     const double *__restrict__ Ljj, int ldl)
-{
-    return subst16_body<true>(r, Wj, Ljj, ldl);
-}
-
-// CALL: the substitution as a call (its registers and loads kept out of the
-// caller, at the price of a stack frame), else inlined; chosen per kernel
-// by register use.
-template <bool CALL>
-__device__ __forceinline__ d4 subst16(d4 r, const double *__restrict__ Wj, const double *__restrict__ Ljj, int ldl)
-{
-    if constexpr (CALL) return subst16_call(r, Wj, Ljj, ldl);
-    else return subst16_body<false>(r, Wj, Ljj, ldl);
-}
-
 // X = C L_bb^{-T} in place on the 64x64 tile at Ct (leading dimension ldc),
 // L_bb the factored diagonal block at Lb (leading dimension ldl) with the
 // inverses of its 16x16 diagonal blocks in dinv; IDENT: C is the identity
@@ -596,12 +582,15 @@ __device__ __forceinline__ d4 subst16(d4 r, const double *__restrict__ Wj, const
 // L_jk (j > k) in A layout (lane l holds L_jk[l%16][4kk + l/16]) and the
 // 16x16 inverses likewise, each read where it is used (held all at once
 // they take 80 registers).
-template <bool CALL = false>
 __device__ inline void trsm16_rows(const d4 t[4], const double *__restrict__ Lb, int ldl,
                                    const double *__restrict__ dinv, d4 y[4])
 {
     const int lane = threadIdx.x & 63;
     const int lr = lane & 15, lq = lane >> 4;
+    // the four blocks' flags up front: one load latency, not four in the chain
+    double flag[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) flag[j] = dinv[kDinvFlags + j];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         d4 r = t[j];
@@ -614,13 +603,13 @@ __device__ inline void trsm16_rows(const d4 t[4], const double *__restrict__ Lb,
             for (int kk = 0; kk < 4; ++kk) r = mfma_f64(-lf[kk], y[k][kk], r);
         }
         const double *Wj = dinv + j * 256;
-        if (dinv[kDinvFlags + j] != 0.0) {
+        if (flag[j] != 0.0) {
             // an ill-conditioned diagonal block (store_subst_flag): forward
             // substitution inside it, column c of the block after column c - 1
             // as in dpotf2, y_c = (r_c - sum_{k<c} y_k L(c, k)) / L(c, c),
             // the division a product with W(c, c) = 1 / L(c, c); column c of
             // row lr sits in lane lr + 16 (c % 4), component c / 4
-            y[j] = subst16<CALL>(r, Wj, Lb + (size_t)(16 * j) * ldl + 16 * j, ldl);
+            y[j] = subst16(r, Wj, Lb + (size_t)(16 * j) * ldl + 16 * j, ldl);
             continue;
         }
         double wf[4];
@@ -688,7 +677,7 @@ __device__ inline void solve_rows16(double *A, int np, int c0, int row, const do
     for (int j = 0; j < 4; ++j)
 #pragma unroll
         for (int v = 0; v < 4; ++v) t[j][v] = C0[(size_t)(16 * j + 4 * v + lq) * np];
-    trsm16_rows<true>(t, A + (size_t)c0 * np + c0, np, dinv_s + (size_t)(c0 / kNBI) * kDinv, x0);
+    trsm16_rows(t, A + (size_t)c0 * np + c0, np, dinv_s + (size_t)(c0 / kNBI) * kDinv, x0);
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
@@ -705,7 +694,7 @@ __device__ inline void solve_rows16(double *A, int np, int c0, int row, const do
         for (int j = 0; j < 4; ++j) t[j] = mfma_f64(Bg[(size_t)(4 * kk + lq) * np + 16 * j], af, t[j]);
     }
     d4 x1[4];
-    trsm16_rows<true>(t, A + (size_t)c1 * np + c1, np, dinv_s + (size_t)(c1 / kNBI) * kDinv, x1);
+    trsm16_rows(t, A + (size_t)c1 * np + c1, np, dinv_s + (size_t)(c1 / kNBI) * kDinv, x1);
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
