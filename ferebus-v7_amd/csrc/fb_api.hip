@@ -16,6 +16,7 @@
 #include <cstring>
 #include <limits>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1115,6 +1116,110 @@ int fb_swarm_loglik_grad(fb_handle *h, int s, const double *theta, double *ll, d
     std::memcpy(info, h->h_info, sizeof(int) * s);
     std::memcpy(grad, h->h_grad, sizeof(double) * P * s);
     return h->profile ? drain_profile(h) : FB_OK;
+}
+
+// ---- evaluation with no allocation: the library's own scratch array ---------
+
+}  // extern "C"
+
+namespace {
+
+// Work buffers of fb_swarm_loglik_static: hyperparameters, chain words,
+// the swarm's K matrices, partial log-determinants, 16x16 inverses and
+// flags, likelihood extras and residuals, for every call on this device.
+// In host code the name is a stand-in, not a device address: the address
+// comes from hipGetSymbolAddress, once per device.
+constexpr size_t kStaticScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// The scratch array's address on the current device, looked up on the
+// device's first call (with that device current) and kept.
+int scratch_address(void **out)
+{
+    int dev = 0;
+    FB_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(FB_E_NOGPU, "fb_swarm_loglik_static: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(FB_E_HIP, std::string("fb_swarm_loglik_static: hipGetSymbolAddress: ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return FB_OK;
+}
+
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+}  // namespace
+
+extern "C" {
+
+int fb_swarm_loglik_static(fb_handle *h, int s, const double *d_theta, double *d_ll, int *d_info, void *stream)
+{
+    if (!h || !d_theta || !d_ll || !d_info) return fail(FB_E_ARG, "fb_swarm_loglik_static: null argument");
+    if (s < 1) return fail(FB_E_ARG, "fb_swarm_loglik_static: empty swarm");
+    const size_t S = (size_t)s, np2 = (size_t)h->np * h->np;
+    const size_t hs = (size_t)h->prog.n_par_kernel + fb::kMaxTerms + 1;
+    const size_t off_hyp = 0;
+    const size_t off_chain = align256(off_hyp + sizeof(double) * hs * S);
+    const size_t off_K = align256(off_chain + sizeof(int) * fb::kChainStride * S);
+    const size_t off_logdet = align256(off_K + sizeof(double) * np2 * S);
+    const size_t off_dinv = align256(off_logdet + sizeof(double) * h->nsteps * S);
+    const size_t off_extra = align256(off_dinv + sizeof(double) * h->nsteps * fb::kDinv * S);
+    const size_t off_z = align256(off_extra + sizeof(double) * 4 * S);
+    const size_t total = align256(off_z + sizeof(double) * h->n * S);
+    if (total > kStaticScratchBytes)
+        return fail(FB_E_CAPACITY, "fb_swarm_loglik_static: swarm needs " + std::to_string(total >> 20) +
+                                       " MiB of scratch, the library holds " +
+                                       std::to_string(kStaticScratchBytes >> 20) + " MiB per device");
+    void *base = nullptr;
+    if (int rc = scratch_address(&base)) return rc;
+    unsigned char *w = static_cast<unsigned char *>(base);
+    fb::SwarmBuffers b;
+    b.n = h->n; b.d = h->d; b.np = h->np; b.nrhs = h->nrhs; b.likelihood = h->likelihood;
+    b.S = s;
+    b.n_slots = h->n_slots; b.n_leaves = h->prog.n_leaves;
+    b.slots = h->d_slots; b.RHS = h->d_rhs; b.prog = h->d_prog; b.prog_host = &h->prog;
+    b.theta = d_theta; b.ll = d_ll; b.info = d_info;
+    b.hyp = reinterpret_cast<double *>(w + off_hyp);
+    b.K = reinterpret_cast<double *>(w + off_K);
+    b.logdet_part = reinterpret_cast<double *>(w + off_logdet);
+    b.dinv = reinterpret_cast<double *>(w + off_dinv);
+    b.extra = reinterpret_cast<double *>(w + off_extra);
+    b.zbuf = reinterpret_cast<double *>(w + off_z);
+    // one fixed lock-step schedule (16-deep chunks, 64 x 128 tiles; chained
+    // up to kChainMaxSwarm particles, three launches per panel beyond): the
+    // same tile routines in the same order as every other path, so the same
+    // bits; no tuning, no graph, no persistent-schedule descriptors
+    b.gemm_kc = 16;
+    b.tile = 0;
+    b.panel_tile = nullptr;
+    b.chain = (h->chain && s <= fb::kChainMaxSwarm) ? reinterpret_cast<int *>(w + off_chain) : nullptr;
+    b.tail = b.chain ? h->tail : 0;
+    b.lookahead = 0;
+    b.wait_limit = h->wait_limit;
+    hipStream_t st = stream ? (hipStream_t)stream : h->stream;
+    fb::launch_build(b, st);
+    for (int c0 = 0; c0 < h->np; c0 += fb::kNBO) {
+        if (b.chain) {
+            fb::launch_panel_chain(b, c0, st);
+            if (c0 + fb::kNBO < h->np && !fb::panel_fused(b, c0)) fb::launch_solve_rows(b, c0, st);
+            continue;
+        }
+        fb::launch_panel_update_factor(b, c0, st);
+        fb::launch_trsm_factor(b, c0, st);
+        if (c0 + fb::kNBI + fb::kNBI < h->np) fb::launch_update_trsm(b, c0 + fb::kNBI, st);
+    }
+    fb::launch_likelihood(b, st);
+    FB_HIP(hipGetLastError());
+    return FB_OK;
 }
 
 // ---- several handles in one persistent factorisation ------------------------

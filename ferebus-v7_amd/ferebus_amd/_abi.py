@@ -25,7 +25,7 @@ EXPORTS = (
     "fb_profile_enable", "fb_profile_read", "fb_factor_path", "fb_device_name", "fb_last_error",
     "fb_multi_create", "fb_multi_swarm_loglik", "fb_multi_destroy",
     "fb_pso_create", "fb_pso_destroy", "fb_pso_init", "fb_pso_step", "fb_pso_retry", "fb_pso_select",
-    "fb_pso_set_gbest", "fb_pso_positions", "fb_debug_matrix",
+    "fb_pso_set_gbest", "fb_pso_positions", "fb_debug_matrix", "fb_swarm_loglik_static",
 )
 
 
@@ -72,6 +72,7 @@ def load() -> ctypes.CDLL:
         "fb_set_stream": (ctypes.c_int, [vp, vp]),
         "fb_swarm_loglik": (ctypes.c_int, [vp, ctypes.c_int, _P_DBL, _P_DBL, _P_INT]),
         "fb_swarm_loglik_async": (ctypes.c_int, [vp, ctypes.c_int, vp, vp, vp]),
+        "fb_swarm_loglik_static": (ctypes.c_int, [vp, ctypes.c_int, vp, vp, vp, vp]),
         "fb_particle_loglik": (ctypes.c_int, [vp, ctypes.c_int, _P_DBL, _P_DBL, _P_INT]),
         "fb_get_alpha": (ctypes.c_int, [vp, _P_DBL, _P_DBL, _P_DBL, _P_INT]),
         "fb_swarm_loglik_grad": (ctypes.c_int, [vp, ctypes.c_int, _P_DBL, _P_DBL, _P_DBL, _P_INT]),

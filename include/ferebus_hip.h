@@ -116,6 +116,26 @@ int fb_swarm_loglik(fb_handle *h, int s, const double *theta, double *ll, int *i
 int fb_swarm_loglik_async(fb_handle *h, int s, const double *d_theta,
                           double *d_ll, int *d_info);
 
+/* The same evaluation as fb_swarm_loglik_async with no allocation at all:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  Every work buffer (the swarm's K matrices,
+ * hyperparameters, partial log-dets, 16x16 inverses, hand-over words,
+ * residuals) lives in one 64 MiB array the library holds on each device
+ * until the process ends; a swarm that needs more (about 64 MiB / (8 np^2),
+ * np = n + right-hand sides rounded up to 128: 30 particles at n = 500, one
+ * at n = 2000) is rejected with FB_E_CAPACITY.  The handle's inputs (x, y,
+ * mean, kernel) are read, none of its buffers written.  One fixed lock-step
+ * schedule (no tuning, no graph), bit-identical to fb_swarm_loglik.
+ *   d_theta   P*s FP64 device array (layout of fb_swarm_loglik)
+ *   d_ll, d_info  s entries each, device arrays (outputs)
+ *   stream    hipStream_t as void*, NULL = the handle's stream; enqueued
+ *             without host synchronisation.
+ * Calls that overlap on one device (any handles, any streams) share the
+ * array and race: serialise them. */
+int fb_swarm_loglik_static(fb_handle *h, int s, const double *d_theta, double *d_ll, int *d_info,
+                           void *stream);
+
 /* Plain per-particle path: one launch sequence per particle, particles one after
  * the other, exactly what a line-by-line offload of get_cost would do.  Same
  * arguments and results as fb_swarm_loglik.  Kept as the comparison path. */
