@@ -71,6 +71,15 @@ program ferebus_amd
   pso%cognitive_learning_rate = cfg%cognitive
   pso%social_learning_rate = cfg%social
   pso%relative_change = cfg%stopping == "relative_change"
+  pso%swarm_updater = cfg%swarm_updater
+  pso%delta = cfg%delta
+  pso%parameter_updater = cfg%parameter_updater
+  pso%amin = [cfg%min_w, cfg%min_c1, cfg%min_c2]
+  pso%amax = [cfg%max_w, cfg%max_c1, cfg%max_c2]
+  pso%an = [cfg%n_w, cfg%n_c1, cfg%n_c2]
+  pso%reward_step = cfg%reward_step
+  pso%penalty_step = cfg%penalty_step
+  pso%success_threshold = cfg%success_threshold
   pso%tolerance = cfg%tolerance
   pso%stall_iterations = cfg%stall_iterations
   pso%lbfgs_every = cfg%lbfgs_every

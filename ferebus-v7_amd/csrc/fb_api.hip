@@ -1433,7 +1433,8 @@ int fb_multi_swarm_loglik(fb_multi *m, const int *s, const double *const *theta,
 struct fb_pso {
     fb_handle *h = nullptr;
     fb::PsoState st;
-    double *d_r = nullptr, *h_r = nullptr;            // r1 | r2, device and pinned staging
+    double coef[3] = {0.0, 0.0, 0.0};                 // w, c1, c2 of the next step
+    double *d_r = nullptr, *h_r = nullptr;            // r1 | r2 | w c1 c2, device and pinned staging
     double *h_cost = nullptr, *h_state = nullptr;     // pinned read-back
     int *h_info = nullptr, *h_gidx = nullptr;
     int *d_idx = nullptr, *h_idx = nullptr;           // retry lists
@@ -1537,7 +1538,8 @@ int fb_pso_create(fb_pso **out, fb_handle *h, int s, double w, double c1, double
     if (P < 1) return fail(FB_E_ARG, "fb_pso_create: the model has no parameters to optimise");
     fb_pso *p = new fb_pso;
     p->h = h;
-    p->st.S = s; p->st.P = P; p->st.w = w; p->st.c1 = c1; p->st.c2 = c2;
+    p->st.S = s; p->st.P = P;
+    p->coef[0] = w; p->coef[1] = c1; p->coef[2] = c2;
     const size_t SP = (size_t)s * P;
     bool ok = true;
     auto dev = [&](void **q, size_t bytes) { ok = ok && hipMalloc(q, bytes) == hipSuccess; };
@@ -1552,10 +1554,10 @@ int fb_pso_create(fb_pso **out, fb_handle *h, int s, double w, double c1, double
     dev((void **)&p->st.info, sizeof(int) * s);
     dev((void **)&p->st.pending, sizeof(int));
     dev((void **)&p->st.gidx, sizeof(int));
-    dev((void **)&p->d_r, sizeof(double) * 2 * SP);
+    dev((void **)&p->d_r, sizeof(double) * (2 * SP + 3));
     dev((void **)&p->d_idx, sizeof(int) * s);
     dev((void **)&p->d_xs, sizeof(double) * 2 * SP);
-    pin((void **)&p->h_r, sizeof(double) * 2 * SP);
+    pin((void **)&p->h_r, sizeof(double) * (2 * SP + 3));
     pin((void **)&p->h_cost, sizeof(double) * s);
     pin((void **)&p->h_state, sizeof(double) * (2 + P));
     pin((void **)&p->h_info, sizeof(int) * s);
@@ -1622,10 +1624,11 @@ int fb_pso_step(fb_pso *p, const double *r1, const double *r2, double *cost, int
     if (rc) return rc;
     std::memcpy(p->h_r, r1, sizeof(double) * SP);
     std::memcpy(p->h_r + SP, r2, sizeof(double) * SP);
+    std::memcpy(p->h_r + 2 * SP, p->coef, sizeof(double) * 3);
     auto enqueue = [&](hipStream_t s) {
-        (void)hipMemcpyAsync(p->d_r, p->h_r, sizeof(double) * 2 * SP, hipMemcpyHostToDevice, s);
+        (void)hipMemcpyAsync(p->d_r, p->h_r, sizeof(double) * (2 * SP + 3), hipMemcpyHostToDevice, s);
         fb::launch_pso_select(p->st, s);
-        fb::launch_pso_update(p->st, p->d_r, p->d_r + SP, s);
+        fb::launch_pso_update(p->st, p->d_r, p->d_r + SP, p->d_r + 2 * SP, s);
         pso_enqueue_eval(p, s);
     };
     if (h->profile || !h->use_graph) {
@@ -1691,6 +1694,13 @@ int fb_pso_retry(fb_pso *p, int k, const int *idx, const double *x, const double
         if (info) info[r] = h->h_info[r];
     }
     return h->profile ? drain_profile(h) : FB_OK;
+}
+
+int fb_pso_set_coefficients(fb_pso *p, double w, double c1, double c2)
+{
+    if (!p) return fail(FB_E_ARG, "fb_pso_set_coefficients: null argument");
+    p->coef[0] = w; p->coef[1] = c1; p->coef[2] = c2;
+    return FB_OK;
 }
 
 int fb_pso_select(fb_pso *p, double *gcost, double *gbest, int *gidx)

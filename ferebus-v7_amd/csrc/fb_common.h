@@ -285,14 +285,14 @@ void launch_backsolve(const SwarmBuffers &b, int particle, double *alpha, hipStr
 // at x[k * P + j]; state[0] = gbest cost.
 struct PsoState {
     int S = 0, P = 0;
-    double w = 0.0, c1 = 0.0, c2 = 0.0;
     double *x = nullptr, *v = nullptr, *pbest = nullptr, *pcost = nullptr, *gbest = nullptr;
     double *cost = nullptr, *state = nullptr;
     int *info = nullptr, *pending = nullptr, *gidx = nullptr;
 };
 void launch_pso_cost(const PsoState &p, const double *ll, const int *info_src, hipStream_t st);
 void launch_pso_select(const PsoState &p, hipStream_t st);
-void launch_pso_update(const PsoState &p, const double *r1, const double *r2, hipStream_t st);
+void launch_pso_update(const PsoState &p, const double *r1, const double *r2, const double *coef,
+                       hipStream_t st);
 void launch_pso_scatter(const PsoState &p, int k, const int *idx, const double *xs, const double *vs,
                         hipStream_t st);
 void launch_pso_gather(const PsoState &p, int k, const int *idx, double *out, hipStream_t st);

@@ -6,7 +6,9 @@
 // Rules of the reference PSO (src/optimiser/pso/pso.f90):
 //   * update (pso.f90:422-434): v = w v + c1 r1 (pbest - x) + c2 r2 (gbest - x),
 //     x = x + v, no bounds; r1, r2 ~ U[0,1)^(P x S) drawn on the host before
-//     the swarm loop (pso.f90:405-409) and handed over per iteration;
+//     the swarm loop (pso.f90:405-409) and handed over per iteration, with
+//     the swarm's current (w, c1, c2), which a parameter updater may change
+//     between iterations (pso.f90:638-825);
 //   * pbest (pso.f90:440-443): replaced when cost <= pbest cost;
 //   * gbest (pso.f90:454-459): particles in index order, replaced when
 //     cost <= running gbest cost, so the last particle holding the minimum
@@ -86,8 +88,10 @@ __global__ void __launch_bounds__(256) pso_select_kernel(int S, int P, const dou
     if (t == 0) *pending = 0;
 }
 
-// Velocity and position update of every particle.
-__global__ void pso_update_kernel(int S, int P, double w, double c1, double c2,
+// Velocity and position update of every particle with the coefficients
+// (w, c1, c2) at coef -- device memory, so that a parameter updater can
+// change them from one replay of the captured iteration to the next.
+__global__ void pso_update_kernel(int S, int P, const double *__restrict__ coef,
                                   const double *__restrict__ r1, const double *__restrict__ r2,
                                   const double *__restrict__ pbest, const double *__restrict__ gbest,
                                   double *__restrict__ x, double *__restrict__ v)
@@ -95,6 +99,7 @@ __global__ void pso_update_kernel(int S, int P, double w, double c1, double c2,
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= S * P) return;
     const int j = e % P;
+    const double w = coef[0], c1 = coef[1], c2 = coef[2];
     const double xe = x[e];
     const double ve = w * v[e] + c1 * r1[e] * (pbest[e] - xe) + c2 * r2[e] * (gbest[j] - xe);
     v[e] = ve;
@@ -148,10 +153,10 @@ void launch_pso_select(const PsoState &p, hipStream_t st)
                        p.gbest, p.state, p.pending, p.gidx);
 }
 
-void launch_pso_update(const PsoState &p, const double *r1, const double *r2, hipStream_t st)
+void launch_pso_update(const PsoState &p, const double *r1, const double *r2, const double *coef, hipStream_t st)
 {
-    hipLaunchKernelGGL(pso_update_kernel, dim3(blocks((long long)p.S * p.P)), dim3(256), 0, st, p.S, p.P, p.w,
-                       p.c1, p.c2, r1, r2, p.pbest, p.gbest, p.x, p.v);
+    hipLaunchKernelGGL(pso_update_kernel, dim3(blocks((long long)p.S * p.P)), dim3(256), 0, st, p.S, p.P, coef,
+                       r1, r2, p.pbest, p.gbest, p.x, p.v);
 }
 
 void launch_pso_scatter(const PsoState &p, int k, const int *idx, const double *xs, const double *vs,

@@ -26,6 +26,7 @@ EXPORTS = (
     "fb_multi_create", "fb_multi_swarm_loglik", "fb_multi_destroy",
     "fb_pso_create", "fb_pso_destroy", "fb_pso_init", "fb_pso_step", "fb_pso_retry", "fb_pso_select",
     "fb_pso_set_gbest", "fb_pso_positions", "fb_debug_matrix", "fb_swarm_loglik_static",
+    "fb_pso_set_coefficients",
 )
 
 
@@ -86,6 +87,7 @@ def load() -> ctypes.CDLL:
         "fb_pso_init": (ctypes.c_int, [vp, _P_DBL, _P_DBL, _P_DBL, _P_INT]),
         "fb_pso_step": (ctypes.c_int, [vp, _P_DBL, _P_DBL, _P_DBL, _P_INT]),
         "fb_pso_retry": (ctypes.c_int, [vp, ctypes.c_int, _P_INT, _P_DBL, _P_DBL, _P_DBL, _P_INT]),
+        "fb_pso_set_coefficients": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
         "fb_pso_select": (ctypes.c_int, [vp, _P_DBL, _P_DBL, _P_INT]),
         "fb_pso_set_gbest": (ctypes.c_int, [vp, _P_DBL, ctypes.c_double]),
         "fb_pso_positions": (ctypes.c_int, [vp, _P_DBL, _P_DBL]),

@@ -28,7 +28,7 @@ module ferebus_hip
             fb_swarm_loglik_async, fb_particle_loglik, fb_get_alpha, fb_swarm_loglik_grad, &
             fb_multi_create, fb_multi_swarm_loglik, fb_multi_destroy, &
             fb_pso_create, fb_pso_destroy, fb_pso_init, fb_pso_step, fb_pso_retry, fb_pso_select, &
-            fb_pso_set_gbest, fb_pso_positions, &
+            fb_pso_set_gbest, fb_pso_positions, fb_pso_set_coefficients, &
             fb_profile_enable, fb_profile_read, fb_factor_path, fb_device_name, fb_last_error, fb_error_message
 
   interface
@@ -179,6 +179,12 @@ module ferebus_hip
       real(c_double), intent(in) :: gbest(*)
       real(c_double), value :: gcost
     end function fb_pso_set_gbest
+
+    integer(c_int) function fb_pso_set_coefficients(p, w, c1, c2) bind(c, name="fb_pso_set_coefficients")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: p
+      real(c_double), value :: w, c1, c2
+    end function fb_pso_set_coefficients
 
     integer(c_int) function fb_pso_positions(p, x, v) bind(c, name="fb_pso_positions")
       import :: c_int, c_double, c_ptr

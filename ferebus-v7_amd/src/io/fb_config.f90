@@ -15,6 +15,19 @@
 !>     (src/config/optimiser/pso/pso.f90:48-56)
 !>   [optimiser.pso.relative_change] tolerance=1e-6, stall_iterations=20
 !>     (src/config/optimiser/pso/stopping_criteria.f90:60-62)
+!>   [optimiser.pso] swarm_updater="global" ("default"; "forced" with
+!>     [optimiser.pso.forced] delta=1e-7), parameter_updater="constant"
+!>     ("none"; "unified_adaptive"/"uapso"/"ua", "individual_adaptive"/
+!>     "iapso"/"ia" with their table -- or [optimiser.pso.adaptive], or the
+!>     other adaptive variant's -- holding min/max/n of the three
+!>     coefficients, reward/penalty step lengths and success_threshold;
+!>     "random_unified"/"random"/"ru", "random_individual"/"ri")
+!>     (src/config/optimiser/pso/pso.f90:92-157, parameter_updater.f90:62-75,
+!>     swarm_updater.f90:58).  The three names are sanitised as the reference
+!>     does (spaces to underscores, lower case; src/utils/str.f90:85-98), and
+!>     an unknown name -- or a stopping criterion the reference parses but
+!>     never implements (potential_based, sign_test; pso.f90:360-366) -- is
+!>     an error, not a silent default.
 !>   [kernels.<name>] type="rbf", active_dimensions (src/config/config_module.f90:165-190)
 !>   [notes] key = value
 !> The CSV reader takes the header convention of src/csv.f90:37-177: an empty
@@ -66,14 +79,35 @@ module fb_config
     character(len=:), allocatable :: stopping
     real(wp) :: tolerance = 1e-6_wp
     integer :: stall_iterations = 20
+    character(len=:), allocatable :: swarm_updater, parameter_updater
+    real(wp) :: delta = 1e-7_wp
+    real(wp) :: min_w = 0.0_wp, max_w = 1.0_wp, min_c1 = 0.0_wp, max_c1 = 4.0_wp, min_c2 = 0.0_wp, max_c2 = 4.0_wp
+    integer :: n_w = 10, n_c1 = 10, n_c2 = 10
+    real(wp) :: reward_step = 0.01_wp, penalty_step = 0.01_wp, success_threshold = 0.5_wp
     integer :: lbfgs_every = 0, lbfgs_steps = 5
     type(KernelConfigEntry), allocatable :: kernels(:)
     character(len=256), allocatable :: note_keys(:), note_values(:)
   end type FerebusConfig
 
-  public :: read_config, read_training_set
+  public :: read_config, read_training_set, sanitise
 
 contains
+
+  !> Spaces to underscores and lower case (src/utils/str.f90:85-98).
+  function sanitise(s) result(r)
+    character(len=*), intent(in) :: s
+    character(len=:), allocatable :: r
+    integer :: i, c
+    r = s
+    do i = 1, len(r)
+      if (r(i:i) == " ") then
+        r(i:i) = "_"
+      else
+        c = iachar(r(i:i))
+        if (c >= iachar("A") .and. c <= iachar("Z")) r(i:i) = achar(c + 32)
+      end if
+    end do
+  end function sanitise
 
   function strip(s) result(r)
     character(len=*), intent(in) :: s
@@ -313,7 +347,7 @@ contains
     logical, intent(out) :: ok
     character(len=:), allocatable, intent(out) :: msg
     type(TomlDoc) :: doc
-    character(len=:), allocatable :: atoms, item, props
+    character(len=:), allocatable :: atoms, item, props, ptable, pre
     character(len=64), allocatable :: knames(:)
     integer :: k, p0, p1, a0, b0
     call doc%load(path, ok)
@@ -321,6 +355,7 @@ contains
       msg = doc%error
       return
     end if
+    ok = .false.            ! until every key has been read
     cfg%system_name = doc%get_str("system.name", "")
     cfg%natoms = doc%get_int("system.natoms", 0)
     cfg%nfeats = doc%get_int("system.nfeats", 3 * cfg%natoms - 6)
@@ -369,9 +404,75 @@ contains
     cfg%inertia_weight = doc%get_real("optimiser.pso.inertia_weight", 0.789_wp)
     cfg%cognitive = doc%get_real("optimiser.pso.cognitive_learning_rate", 1.494_wp)
     cfg%social = doc%get_real("optimiser.pso.social_learning_rate", 1.494_wp)
-    cfg%stopping = doc%get_str("optimiser.pso.stopping_criteria", "none")
-    cfg%tolerance = doc%get_real("optimiser.pso.relative_change.tolerance", 1e-6_wp)
-    cfg%stall_iterations = doc%get_int("optimiser.pso.relative_change.stall_iterations", 20)
+    cfg%stopping = sanitise(doc%get_str("optimiser.pso.stopping_criteria", "none"))
+    select case (cfg%stopping)
+    case ("none", "relative_change")
+    case ("potential_based", "sign_test")
+      msg = "stopping_criteria " // cfg%stopping // " has no implementation (the reference parses it but " // &
+            "never implements it)"
+      return
+    case default
+      msg = "unknown stopping_criteria: " // cfg%stopping
+      return
+    end select
+    cfg%tolerance = doc%get_real("optimiser.pso." // cfg%stopping // ".tolerance", 1e-6_wp)
+    cfg%stall_iterations = doc%get_int("optimiser.pso." // cfg%stopping // ".stall_iterations", 20)
+    cfg%swarm_updater = sanitise(doc%get_str("optimiser.pso.swarm_updater", "global"))
+    select case (cfg%swarm_updater)
+    case ("global", "default")
+      cfg%swarm_updater = "global"
+    case ("forced")
+      cfg%delta = doc%get_real("optimiser.pso.forced.delta", 1e-7_wp)
+    case default
+      msg = "unknown swarm_updater: " // cfg%swarm_updater
+      return
+    end select
+    ptable = sanitise(doc%get_str("optimiser.pso.parameter_updater", "constant"))
+    select case (ptable)
+    case ("constant", "none")
+      cfg%parameter_updater = "constant"
+    case ("unified_adaptive", "uapso", "ua")
+      cfg%parameter_updater = "unified_adaptive"
+    case ("individual_adaptive", "iapso", "ia")
+      cfg%parameter_updater = "individual_adaptive"
+    case ("random_unified", "random", "ru")
+      cfg%parameter_updater = "random_unified"
+    case ("random_individual", "ri")
+      cfg%parameter_updater = "random_individual"
+    case default
+      msg = "unknown parameter_updater: " // ptable
+      return
+    end select
+    if (cfg%parameter_updater == "unified_adaptive" .or. cfg%parameter_updater == "individual_adaptive") then
+      ! the table named as given, else [adaptive], else the other variant's
+      ! (src/config/optimiser/pso/pso.f90:124-139)
+      if (size(doc%keys_under("optimiser.pso." // ptable)) == 0) then
+        if (size(doc%keys_under("optimiser.pso.adaptive")) > 0) then
+          ptable = "adaptive"
+        else if (cfg%parameter_updater == "unified_adaptive") then
+          ptable = "individual_adaptive"
+        else
+          ptable = "unified_adaptive"
+        end if
+      end if
+      pre = "optimiser.pso." // ptable // "."
+      cfg%min_w = doc%get_real(pre // "min_inertia_weight", 0.0_wp)
+      cfg%max_w = doc%get_real(pre // "max_inertia_weight", 1.0_wp)
+      cfg%n_w = doc%get_int(pre // "n_inertia_weight", 10)
+      cfg%min_c1 = doc%get_real(pre // "min_cognitive_learning_rate", 0.0_wp)
+      cfg%max_c1 = doc%get_real(pre // "max_cognitive_learning_rate", 4.0_wp)
+      cfg%n_c1 = doc%get_int(pre // "n_cognitive_learning_rate", 10)
+      cfg%min_c2 = doc%get_real(pre // "min_social_learning_rate", 0.0_wp)
+      cfg%max_c2 = doc%get_real(pre // "max_social_learning_rate", 4.0_wp)
+      cfg%n_c2 = doc%get_int(pre // "n_social_learning_rate", 10)
+      cfg%reward_step = doc%get_real(pre // "reward_step_length", 0.01_wp)
+      cfg%penalty_step = doc%get_real(pre // "penalty_step_length", 0.01_wp)
+      cfg%success_threshold = doc%get_real(pre // "success_threshold", 0.5_wp)
+      if (min(cfg%n_w, cfg%n_c1, cfg%n_c2) < 2) then
+        msg = "adaptive parameter updater: every n_* must be at least 2"
+        return
+      end if
+    end if
     ! build-side extension (no reference key): L-BFGS polishing of gbest
     cfg%lbfgs_every = doc%get_int("optimiser.pso.lbfgs_every", 0)
     cfg%lbfgs_steps = doc%get_int("optimiser.pso.lbfgs_steps", 5)

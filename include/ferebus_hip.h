@@ -227,6 +227,10 @@ int fb_multi_destroy(fb_multi *m);
  *                   and INFO of every particle
  *   fb_pso_retry    new positions / velocities for the k particles idx
  *                   (0-based), re-evaluated together; their costs / INFO
+ *   fb_pso_set_coefficients  (w, c1, c2) of the following steps: the swarm's
+ *                   inertia weight and learning rates as a parameter updater
+ *                   sets them (pso.f90:638-825; the update reads them from
+ *                   device memory, so the captured iteration is kept)
  *   fb_pso_select   applies pending selections; gbest cost, position and
  *                   particle index (gidx may be NULL)
  *   fb_pso_set_gbest  replaces gbest (after polishing), pending costs
@@ -238,6 +242,7 @@ int fb_pso_destroy(fb_pso *p);
 int fb_pso_init(fb_pso *p, const double *x0, const double *v0, double *cost, int *info);
 int fb_pso_step(fb_pso *p, const double *r1, const double *r2, double *cost, int *info);
 int fb_pso_retry(fb_pso *p, int k, const int *idx, const double *x, const double *v, double *cost, int *info);
+int fb_pso_set_coefficients(fb_pso *p, double w, double c1, double c2);
 int fb_pso_select(fb_pso *p, double *gcost, double *gbest, int *gidx);
 int fb_pso_set_gbest(fb_pso *p, const double *gbest, double gcost);
 int fb_pso_positions(fb_pso *p, double *x, double *v);

@@ -43,6 +43,13 @@ program test_host
   print '(A,A)', "pso.stopping=", cfg%stopping
   print '(A,ES24.16)', "pso.tolerance=", cfg%tolerance
   print '(A,I0)', "pso.stall_iterations=", cfg%stall_iterations
+  print '(A,A)', "pso.swarm_updater=", cfg%swarm_updater
+  print '(A,ES24.16)', "pso.delta=", cfg%delta
+  print '(A,A)', "pso.parameter_updater=", cfg%parameter_updater
+  print '(A,3(ES24.16,1X))', "pso.adaptive_min=", cfg%min_w, cfg%min_c1, cfg%min_c2
+  print '(A,3(ES24.16,1X))', "pso.adaptive_max=", cfg%max_w, cfg%max_c1, cfg%max_c2
+  print '(A,3(I0,1X))', "pso.adaptive_n=", cfg%n_w, cfg%n_c1, cfg%n_c2
+  print '(A,3(ES24.16,1X))', "pso.adaptive_steps=", cfg%reward_step, cfg%penalty_step, cfg%success_threshold
   print '(A,I0)', "kernels=", size(cfg%kernels)
   do k = 1, size(cfg%kernels)
     if (allocated(cfg%kernels(k)%active_dims)) then

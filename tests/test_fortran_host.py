@@ -122,3 +122,63 @@ def test_csv_and_means(host_exe):
         m, *_ = oracle.mean(name, x, y)
         got = [float(v) for v in field(lines, f"mean{kind}").split()]
         np.testing.assert_allclose(got, [m[0], m[1], m.sum()], rtol=1e-11, atol=1e-12)
+
+
+def _run_toml(exe, tmp_path, pso_block):
+    base = (EX / "WATER_ferebus.toml").read_text()
+    text = base + "\n" + pso_block
+    path = tmp_path / "ferebus.toml"
+    path.write_text(text)
+    return subprocess.run([str(exe), str(path), str(EX / "WATER_O1_TRAINING_SET.csv")],
+                          capture_output=True, text=True)
+
+
+def test_updater_keys_sanitised(host_exe, tmp_path):
+    """swarm_updater / parameter_updater / stopping_criteria are sanitised as
+    the reference does (spaces to underscores, lower case, str.f90:85-98),
+    aliases map to their updater, and the adaptive table is looked up under
+    the given name, then [adaptive] (config/optimiser/pso/pso.f90:124-139)."""
+    base = (EX / "WATER_ferebus.toml").read_text()
+    assert "[optimiser.pso]" in base and "stopping_criteria" not in base
+    out = _run_toml(host_exe, tmp_path, "")
+    assert out.returncode == 0, out.stderr
+    lines = [l.strip() for l in out.stdout.splitlines()]
+    assert field(lines, "pso.swarm_updater") == "global"
+    assert field(lines, "pso.parameter_updater") == "constant"
+    assert float(field(lines, "pso.delta")) == 1e-7
+    # keys of [optimiser.pso] after the file's own section: re-open it
+    text = base.replace("[optimiser.pso]", "[optimiser.pso]\nswarm_updater = \"Forced\"\n"
+                        "parameter_updater = \"UAPSO\"\nstopping_criteria = \"Relative Change\"", 1)
+    text += ("\n[optimiser.pso.forced]\ndelta = 1e-3\n[optimiser.pso.adaptive]\nmax_inertia_weight = 0.9\n"
+             "n_social_learning_rate = 7\nsuccess_threshold = 0.25\n"
+             "[optimiser.pso.relative_change]\ntolerance = 1e-5\nstall_iterations = 9\n")
+    (tmp_path / "f.toml").write_text(text)
+    out = subprocess.run([str(host_exe), str(tmp_path / "f.toml"), str(EX / "WATER_O1_TRAINING_SET.csv")],
+                         capture_output=True, text=True)
+    assert out.returncode == 0, out.stderr
+    lines = [l.strip() for l in out.stdout.splitlines()]
+    assert field(lines, "pso.swarm_updater") == "forced"
+    assert float(field(lines, "pso.delta")) == 1e-3
+    assert field(lines, "pso.parameter_updater") == "unified_adaptive"
+    assert field(lines, "pso.stopping") == "relative_change"
+    assert float(field(lines, "pso.tolerance")) == 1e-5 and field(lines, "pso.stall_iterations") == "9"
+    assert [float(v) for v in field(lines, "pso.adaptive_max").split()] == [0.9, 4.0, 4.0]
+    assert field(lines, "pso.adaptive_n").split() == ["10", "10", "7"]
+    assert [float(v) for v in field(lines, "pso.adaptive_steps").split()] == [0.01, 0.01, 0.25]
+
+
+@pytest.mark.parametrize("key,value,needle", [
+    ("stopping_criteria", "potential_based", "no implementation"),
+    ("stopping_criteria", "Sign Test", "no implementation"),
+    ("stopping_criteria", "relative-change", "unknown stopping_criteria"),
+    ("swarm_updater", "local", "unknown swarm_updater"),
+    ("parameter_updater", "adaptive", "unknown parameter_updater"),
+])
+def test_unknown_names_fail_closed(host_exe, tmp_path, key, value, needle):
+    base = (EX / "WATER_ferebus.toml").read_text()
+    text = base.replace("[optimiser.pso]", f"[optimiser.pso]\n{key} = \"{value}\"", 1)
+    (tmp_path / "bad.toml").write_text(text)
+    out = subprocess.run([str(host_exe), str(tmp_path / "bad.toml"), str(EX / "WATER_O1_TRAINING_SET.csv")],
+                         capture_output=True, text=True)
+    assert out.returncode != 0
+    assert needle in out.stderr + out.stdout

@@ -70,12 +70,13 @@ active_dimensions = [3]
 """
 
 
-def _run(tmp_path, iterations=15, lbfgs_every=0, noise=1e-8, seed=7):
+def _run(tmp_path, iterations=15, lbfgs_every=0, noise=1e-8, seed=7, extra=""):
     if not EXE.exists():
         pytest.skip("bin/ferebus_amd not built")
     shutil.copy(EX / "WATER_O1_TRAINING_SET.csv", tmp_path)
-    (tmp_path / "ferebus.toml").write_text(WATER_TOML.format(iterations=iterations, lbfgs_every=lbfgs_every,
-                                                             noise=noise))
+    text = WATER_TOML.format(iterations=iterations, lbfgs_every=lbfgs_every, noise=noise)
+    text = text.replace("lbfgs_steps = 4\n", "lbfgs_steps = 4\n" + extra)
+    (tmp_path / "ferebus.toml").write_text(text)
     out = subprocess.run([str(EXE), "-c", "ferebus.toml", "--trace", "trace.txt", "--seed", str(seed)],
                          cwd=tmp_path, capture_output=True, text=True, timeout=240)
     print(out.stdout[-3000:])
@@ -204,3 +205,35 @@ def test_host_lbfgs_polish(tmp_path):
         before = min(e[4] for e in evals if e[0] <= it and e[3] == 0)
         assert fp <= before + 1e-12 * abs(before)
     assert best_cost <= min(p[3] for p in polish) + 1e-12 * abs(best_cost)
+
+
+@pytest.mark.parametrize("swarm,param", [("Forced", "individual_adaptive"), ("global", "UAPSO"),
+                                         ("global", "random"), ("forced", "RI")])
+def test_host_updaters(tmp_path, swarm, param):
+    """The Fortran program with the reference's other swarm and parameter
+    updaters (config/optimiser/pso/pso.f90:115-157): every traced cost is
+    the oracle's, and gbest is the lowest traced cost (the forced updater
+    moves gbest only when a particle's pbest improves, which every new
+    overall minimum does)."""
+    noise = 1e-8
+    extra = f'swarm_updater = "{swarm}"\nparameter_updater = "{param}"\n'
+    best_cost, best_params, evals, _ = _run(tmp_path, iterations=5, noise=noise, extra=extra)
+    x, y, prog, m = _water()
+    th = np.array([e[5] for e in evals])
+    gpu_cost = np.array([e[4] for e in evals])
+    gpu_info = np.array([e[3] for e in evals])
+    ocost, oinfo = _oracle_cost(x, y, m, prog, th, noise)
+    assert np.array_equal(gpu_info, oinfo)
+    ok = gpu_info == 0
+    rel = np.abs(gpu_cost[ok] - ocost[ok]) / np.abs(ocost[ok])
+    # the 59-point reference set with the k1*k2 kernel at a 1e-8 nugget is
+    # ill-conditioned: the tolerance of test_host_pso_replay (10 x the
+    # oracle-vs-LAPACK gap, at least 1e-9)
+    lref, linfo = _lapack(x, y, m, prog, th, noise, False, "marginal")
+    both = ok & (linfo == 0)
+    tol = max(1e-9, 10.0 * np.max(np.abs(-lref[both] - ocost[both]) / np.abs(ocost[both])))
+    print(f"{swarm}/{param}: {len(evals)} evaluations, max relative cost error {rel.max():.2e} (tolerance {tol:.1e})")
+    assert rel.max() <= tol
+    per_iter = [sum(1 for e in evals if e[0] == it and e[1] == 0) for it in range(1, 6)]
+    assert per_iter == [20] * 5
+    assert best_cost == pytest.approx(gpu_cost[ok].min(), rel=1e-14)
