@@ -223,6 +223,7 @@ struct fb_handle {
     std::vector<hipEvent_t> event_pool;
     double prof_ms[7] = {0, 0, 0, 0, 0, 0, 0};
     long long prof_n[7] = {0, 0, 0, 0, 0, 0, 0};
+    long long tune_evals = 0;        // evaluations the autotuner has timed (read as profile kind 7)
 };
 
 namespace {
@@ -833,6 +834,7 @@ static int choose_path(fb_handle *h, int s)
         for (int rep = 0; rep < 2; ++rep) {
             (void)hipEventRecord(e0, st);
             enqueue_pipeline(h, s, h->d_theta, h->d_ll, h->d_info, st);
+            ++h->tune_evals;
             (void)hipEventRecord(e1, st);
             if (hipEventSynchronize(e1) != hipSuccess) break;
             float ms = 0.f;
@@ -872,6 +874,7 @@ static int choose_path(fb_handle *h, int s)
         evs.clear();
         h->k1_events = &evs;
         enqueue_pipeline(h, s, h->d_theta, h->d_ll, h->d_info, st);
+        ++h->tune_evals;
         h->k1_events = nullptr;
         if (hipStreamSynchronize(st) != hipSuccess) break;
         for (size_t k = 0; k < evs.size() && (int)k < npan; ++k) {
@@ -899,6 +902,7 @@ static int choose_path(fb_handle *h, int s)
         for (int rep = 0; rep < 2; ++rep) {
             (void)hipEventRecord(e0, st);
             enqueue_pipeline(h, s, h->d_theta, h->d_ll, h->d_info, st);
+            ++h->tune_evals;
             (void)hipEventRecord(e1, st);
             if (hipEventSynchronize(e1) != hipSuccess) break;
             float ms = 0.f;
@@ -915,6 +919,33 @@ static int choose_path(fb_handle *h, int s)
     set_mode(h, best_mode, s);
     if (!h->tune_file.empty()) write_tune_file(h, s, best_mode, best);
     return FB_OK;
+}
+
+// The path for a swarm of s without timing anything (the re-evaluation of a
+// PSO iteration's failed particles, whose count changes from one iteration
+// to the next): the size's own choice when it has one, else the choice of
+// the nearest tuned size -- its per-panel shapes included -- on the
+// lock-step launches (the persistent launch would need a schedule
+// allocated per size), else 32 x 64 tiles for a few particles and 64 x 128
+// ones beyond.  All paths give the same bits.
+static void choose_path_untimed(fb_handle *h, int s)
+{
+    if (h->forced_mode >= 0) { set_mode(h, h->forced_mode, s); return; }
+    auto it = h->tuned.find(s);
+    if (it != h->tuned.end()) { set_mode(h, it->second, s); return; }
+    int best_s = -1;
+    for (const auto &t : h->tuned)
+        if (best_s < 0 || std::abs(t.first - s) < std::abs(best_s - s)) best_s = t.first;
+    int mode = s <= 4 ? 4 : 2;
+    if (best_s >= 0 && h->tuned[best_s] != 0) {
+        mode = h->tuned[best_s];
+        if (mode == kModePerPanel) {
+            auto pt = h->panel_tiles.find(best_s);
+            if (pt != h->panel_tiles.end()) h->panel_tiles[s] = pt->second;
+            else mode = 2;
+        }
+    }
+    set_mode(h, mode, s);
 }
 
 // INFO kInfoWaitTimeout: a hand-over wait of the chained factorisation
@@ -1670,8 +1701,8 @@ int fb_pso_retry(fb_pso *p, int k, const int *idx, const double *x, const double
     fb::launch_pso_scatter(p->st, k, p->d_idx, p->d_xs, p->d_xs + kP, st);
     // the listed particles as a swarm of k on the ordinary batched path
     fb::launch_pso_gather(p->st, k, p->d_idx, h->d_theta, st);
-    int rc = choose_path(h, k);
-    if (rc) return rc;
+    choose_path_untimed(h, k);
+    int rc = FB_OK;
     if (h->use_dag) {
         rc = dag_plan_for(h, k);
         if (rc) return rc;
@@ -1767,7 +1798,12 @@ int fb_profile_enable(fb_handle *h, int enable)
 
 int fb_profile_read(fb_handle *h, int kind, double *total_ms, long long *launches)
 {
-    if (!h || kind < 0 || kind > 6 || !total_ms || !launches) return fail(FB_E_ARG, "fb_profile_read: bad argument");
+    if (!h || kind < 0 || kind > 7 || !total_ms || !launches) return fail(FB_E_ARG, "fb_profile_read: bad argument");
+    if (kind == 7) {   // the autotuner's timed evaluations so far (not reset)
+        *total_ms = 0.0;
+        *launches = h->tune_evals;
+        return FB_OK;
+    }
     int rc = drain_profile(h);
     *total_ms = h->prof_ms[kind];
     *launches = h->prof_n[kind];

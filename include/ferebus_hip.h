@@ -253,7 +253,11 @@ int fb_pso_positions(fb_pso *p, double *x, double *v);
  *       1 trailing update + first diagonal block, 2 panel solve + second
  *       diagonal block, 3 second block's update + solve; 4 likelihood
  *       reduction, 5 whole swarm evaluation, 6 persistent (single-launch)
- *       factorisation. */
+ *       factorisation; 7: *launches = the swarm evaluations the autotuner
+ *       has timed on this handle so far (counted whether or not profiling
+ *       is enabled, not reset; *total_ms = 0).  fb_pso_retry never tunes:
+ *       its swarms of re-randomised particles take the nearest tuned size's
+ *       path. */
 int fb_profile_enable(fb_handle *h, int enable);
 int fb_profile_read(fb_handle *h, int kind, double *total_ms, long long *launches);
 

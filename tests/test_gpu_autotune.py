@@ -258,3 +258,33 @@ def test_chained_schedule_stream_groups(gpu, monkeypatch):
         out[streams] = (ll, info)
         gp.close()
     assert np.array_equal(out[1][1], out[3][1]) and np.array_equal(out[1][0], out[3][0])
+
+
+def test_retries_do_not_tune(gpu):
+    """fb_pso_retry evaluates swarms of re-randomised particles whose count
+    changes every iteration: none of them may start the autotuner (it times
+    ~40 evaluations per new size); they reuse the nearest tuned size's path
+    and give the bits of an ordinary evaluation of the same positions."""
+    from ferebus_amd.pso import DeviceSwarm
+    w = synthetic.water(n=300, seed=2)
+    prog = compile_kernel(w.kernel, w.kernels, 3)
+    gp = SwarmLikelihood(w.x, w.y, prog, mean="constant", noise=1e-6, max_swarm=16)
+    rng = np.random.default_rng(4)
+    sw = DeviceSwarm(gp, 16, 0.7, 1.4, 1.4)
+    sw.init(rng.uniform(0, 1, (16, 3)), rng.uniform(-1, 1, (16, 3)))
+    sw.step(rng.random((16, 3)), rng.random((16, 3)))
+    _, tuned = gp.profile_read("tune_evals")
+    assert tuned > 0
+    evaluated = []
+    for k in (1, 2, 3, 5, 7, 11, 16):
+        idx = rng.choice(16, k, replace=False)
+        xr = rng.uniform(0, 1, (k, 3))
+        c, info = sw.retry(idx, xr, rng.uniform(-1, 1, (k, 3)))
+        evaluated.append((xr, c, info))
+        _, n = gp.profile_read("tune_evals")
+        assert n == tuned, f"retry of {k} particles started the autotuner ({n - tuned} timed evaluations)"
+    sw.close()
+    for xr, c, info in evaluated:
+        ll, i2 = gp.swarm_loglik(xr)
+        assert np.array_equal(info, i2)
+        assert np.array_equal(c[info == 0], -ll[i2 == 0])
