@@ -163,8 +163,25 @@ __global__ void __launch_bounds__(kThreads) update_trsm_kernel(double *__restric
 // with device-scope atomics (they go to memory), then drops its CU's L1 and
 // the clean non-coherent L2 lines before reading.  Role work-groups come
 // first in the grid; at most two per particle wait, fewer than the launch's
-// resident slots (kChainMaxSwarm), and D never waits, so every wait ends; it
-// is still bounded at 1 s of the 100 MHz clock (INFO kInfoWaitTimeout).
+// resident slots (kChainMaxSwarm), and D never waits, so every wait ends.
+// In the fused variant every row tile waits too (for F2): there the
+// argument is the dispatch order -- work-groups are dispatched in
+// increasing id on each XCD, every role has a lower id than every row tile
+// of its XCD, so each role holds a slot (or has finished) before any row
+// tile can occupy one, and the roles' own waits end as above.  Every wait is
+// still bounded at 1 s of the 100 MHz clock (INFO kInfoWaitTimeout).  The
+// fused roles write INFO past their L2 (set_info<true>) as they write their
+// results, since the row tiles reading it after F2 may run on other XCDs.
+
+// INFO of a particle; WT: a device-scope store that goes past the XCD's L2
+// (read by work-groups of other XCDs after a flag published without an L2
+// write-back).
+template <bool WT>
+__device__ inline void set_info(int *p, int v)
+{
+    if constexpr (WT) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else *p = v;
+}
 
 __device__ inline int chain_xcc()
 {
@@ -311,7 +328,7 @@ __attribute__((amdgpu_waves_per_eu(KC == 8 ? 3 : 2, KC == 8 ? 3 : 2))) panel_cha
                 double lg = 0.0;
                 const int fail = potrf64_block<true, FUSED>(A, np, n, c0, &lg, smem, dinv0);
                 if (threadIdx.x == 0) {
-                    if (fail) info[s] = fail;
+                    if (fail) set_info<FUSED>(info + s, fail);
                     else logdet_part[(size_t)s * nbk + c0 / kNBI] = lg;
                 }
             }
@@ -321,7 +338,7 @@ __attribute__((amdgpu_waves_per_eu(KC == 8 ? 3 : 2, KC == 8 ? 3 : 2))) panel_cha
             chain_announce(cw + kChainXccB, tag);
             gemm_tile_update<kMB, kNBO, KC>(A, np, c1, c0, k0, c0, smem);
             if (!chain_wait(cw + kChainF0, tag, &ok, wait_limit)) {
-                if (threadIdx.x == 0) info[s] = kInfoWaitTimeout;
+                if (threadIdx.x == 0) set_info<FUSED>(info + s, kInfoWaitTimeout);
                 if constexpr (FUSED) chain_publish_mem(cw + kChainF2, tag);
                 return;
             }
@@ -346,7 +363,7 @@ __attribute__((amdgpu_waves_per_eu(KC == 8 ? 3 : 2, KC == 8 ? 3 : 2))) panel_cha
             const int fail = potrf64_block<true, FUSED>(A, np, n, c1, &lg, Ls,
                                                         dinv + ((size_t)s * nbk + c1 / kNBI) * kDinv);
             if (threadIdx.x == 0) {
-                if (fail) info[s] = fail;
+                if (fail) set_info<FUSED>(info + s, fail);
                 else logdet_part[(size_t)s * nbk + c1 / kNBI] = lg;
             }
             if constexpr (FUSED) chain_publish_mem(cw + kChainF2, tag);
@@ -354,7 +371,7 @@ __attribute__((amdgpu_waves_per_eu(KC == 8 ? 3 : 2, KC == 8 ? 3 : 2))) panel_cha
             chain_announce(cw + kChainXccB, tag);
             gemm_tile_update<kMB, kNBI, KC, 2, true>(A, np, c1, c0, k0, c0, smem);
             if (!chain_wait(cw + kChainF0, tag, &ok, wait_limit)) {
-                if (threadIdx.x == 0) info[s] = kInfoWaitTimeout;
+                if (threadIdx.x == 0) set_info<FUSED>(info + s, kInfoWaitTimeout);
             } else if (info[s] == 0) {
                 trsm64_solve<false, false, FUSED>(smem, kPLD, A + (size_t)c0 * np + c1, np, A + (size_t)c0 * np + c0,
                                                   np, dinv0);
@@ -364,7 +381,7 @@ __attribute__((amdgpu_waves_per_eu(KC == 8 ? 3 : 2, KC == 8 ? 3 : 2))) panel_cha
             chain_announce(cw + kChainXccC, tag);
             gemm_tile_update<kMB, kNBI, KC, 2, true>(A, np, c1, c1, k0, c0, smem);
             if (!chain_wait(cw + kChainF1, tag, &ok, wait_limit)) {
-                if (threadIdx.x == 0) info[s] = kInfoWaitTimeout;
+                if (threadIdx.x == 0) set_info<FUSED>(info + s, kInfoWaitTimeout);
                 if constexpr (FUSED) chain_publish_mem(cw + kChainF2, tag);
                 return;
             }
@@ -379,7 +396,7 @@ __attribute__((amdgpu_waves_per_eu(KC == 8 ? 3 : 2, KC == 8 ? 3 : 2))) panel_cha
             const int fail = potrf64_block<true, FUSED>(A, np, n, c1, &lg, smem,
                                                         dinv + ((size_t)s * nbk + c1 / kNBI) * kDinv);
             if (threadIdx.x == 0) {
-                if (fail) info[s] = fail;
+                if (fail) set_info<FUSED>(info + s, fail);
                 else logdet_part[(size_t)s * nbk + c1 / kNBI] = lg;
             }
             if constexpr (FUSED) chain_publish_mem(cw + kChainF2, tag);
@@ -401,7 +418,7 @@ __attribute__((amdgpu_waves_per_eu(KC == 8 ? 3 : 2, KC == 8 ? 3 : 2))) panel_cha
             static_assert(MB == kNBI && NB == kNBO, "the fused row solve needs 64 x 128 row tiles");
             const int tag = c0 / kNBO + 1;
             if (!chain_wait(chain + (size_t)s * kChainStride + kChainF2, tag, &ok, wait_limit)) {
-                if (threadIdx.x == 0) info[s] = kInfoWaitTimeout;
+                if (threadIdx.x == 0) set_info<FUSED>(info + s, kInfoWaitTimeout);
                 return;
             }
             if (info[s] != 0) return;
