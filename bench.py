@@ -179,6 +179,50 @@ def converge_runs(gp, S, cap, schedules=((10, 20), (10, 5)), **settings):
     return out
 
 
+def host_info():
+    """CPU model, logical CPUs and this process's CPU share (affinity mask,
+    cgroup quota, OMP_NUM_THREADS): the CPU baseline runs one OpenMP thread
+    per CPU of the share."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    logical = os.cpu_count() or 1
+    share = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else logical
+    try:
+        quota, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if quota != "max":
+            share = min(share, max(1, int(int(quota) / int(period))))
+    except (OSError, ValueError):
+        pass
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp and omp.isdigit():
+        share = min(share, int(omp))
+    return {"cpu_model": model, "logical_cpus": logical, "cpu_share": share, "threads_used": share}
+
+
+def lapack_info(path):
+    """The host LAPACK the CPU baseline opened: library file and OpenBLAS
+    build configuration."""
+    import ctypes
+    desc = os.path.basename(path)
+    try:
+        lib = ctypes.CDLL(path)
+        for sym in ("scipy_openblas_get_config", "openblas_get_config"):
+            if hasattr(lib, sym):
+                fn = getattr(lib, sym)
+                fn.restype = ctypes.c_char_p
+                desc += " (" + fn().decode().strip() + ")"
+                break
+    except OSError:
+        pass
+    return desc
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -340,22 +384,29 @@ def main():
 
     if not args.no_cpu:
         import oracle
-        cores = min(16, os.cpu_count() or 1)
+        host = host_info()
+        cores = host["threads_used"]
         spec = oracle.Spec.from_program(prog)
-        bl = oracle.Baseline(w.x, w.y, mfit.value, spec, noise=w.noise,
-                             lapack_path=oracle.scipy_openblas_path())
+        lapack_path = oracle.scipy_openblas_path()
+        bl = oracle.Baseline(w.x, w.y, mfit.value, spec, noise=w.noise, lapack_path=lapack_path)
+        pool = np.concatenate([np.atleast_2d(tt) for tt in thetas])
+        bl.swarm(pool[:cores], threads=cores)               # first touch of the tensors, untimed
         done, t = 0, time.perf_counter()
         while True:
-            bl.swarm(thetas[done // S % 4][:cores], threads=cores)
+            bl.swarm(pool[(done + np.arange(cores)) % len(pool)], threads=cores)
             done += cores
-            if time.perf_counter() - t > args.cpu_seconds or done >= 4 * S:
+            if time.perf_counter() - t > args.cpu_seconds:
                 break
         cpu_s = time.perf_counter() - t
         result["cpu_baseline"] = {
             "value": round(done / cpu_s, 3), "unit": "logL evals/s", "cores": cores,
             "kind": "port",
-            "sample": f"{done} particles of the same N={n} workload in batches of {cores} "
-                      f"(reference host algorithm, LAPACK={'scipy OpenBLAS' if bl.lapack else 'oracle'})",
+            "sample": f"{done} particles of the same N={n} workload ({cpu_s:.1f} s), batches of {cores}, one "
+                      f"LAPACK thread per particle (reference host algorithm: pair tensors + dgemv + exp, "
+                      f"dpotrf / dpotrs)",
+            "cpu_model": host["cpu_model"], "logical_cpus": host["logical_cpus"],
+            "cpu_share": host["cpu_share"],
+            "lapack": (lapack_info(lapack_path) if bl.lapack else "the oracle's own dpotf2 (no host LAPACK)"),
         }
         bl.close()
     gp.close()

@@ -70,11 +70,12 @@ active_dimensions = [3]
 """
 
 
-def _run(tmp_path, iterations=15, lbfgs_every=0, noise=1e-8, seed=7, extra=""):
+def _run(tmp_path, iterations=15, lbfgs_every=0, noise=1e-8, seed=7, extra="", model=""):
     if not EXE.exists():
         pytest.skip("bin/ferebus_amd not built")
     shutil.copy(EX / "WATER_O1_TRAINING_SET.csv", tmp_path)
     text = WATER_TOML.format(iterations=iterations, lbfgs_every=lbfgs_every, noise=noise)
+    text = text.replace('optimiser = "pso"\n', 'optimiser = "pso"\n' + model)
     text = text.replace("lbfgs_steps = 4\n", "lbfgs_steps = 4\n" + extra)
     (tmp_path / "ferebus.toml").write_text(text)
     out = subprocess.run([str(EXE), "-c", "ferebus.toml", "--trace", "trace.txt", "--seed", str(seed)],
@@ -237,3 +238,119 @@ def test_host_updaters(tmp_path, swarm, param):
     per_iter = [sum(1 for e in evals if e[0] == it and e[1] == 0) for it in range(1, 6)]
     assert per_iter == [20] * 5
     assert best_cost == pytest.approx(gpu_cost[ok].min(), rel=1e-14)
+
+
+def _model_sections(path):
+    """[(header, [lines])] of a .model file, metadata comments as the first."""
+    secs, cur = [], None
+    for line in path.read_text().splitlines():
+        if not line.strip():
+            continue
+        if line.startswith("# [metadata]"):
+            cur = ("metadata", [])
+            secs.append(cur)
+        elif line.startswith("["):
+            cur = (line.strip(), [])
+            secs.append(cur)
+        else:
+            cur[1].append(line.strip())
+    return secs
+
+
+def _kv(lines):
+    out = {}
+    for l in lines:
+        k, _, v = l.partition(" ")
+        out[k] = v.split()
+    return out
+
+
+@pytest.mark.parametrize("mean,likelihood,kernel", [("constant", "marginal", "k1*k2"),
+                                                    ("constant", "concentrated", "k1+k2"),
+                                                    ("linear", "marginal", "k1*k2")])
+def test_model_file_whole(tmp_path, mean, likelihood, kernel):
+    """Every section of the .model file against write_gpr (gpr.f90:351-398)
+    and the writers it calls: the order of the sections; the metadata's
+    nugget and likelihood (the oracle's at gbest); [system] with the ALF line
+    (alf.f90:75); [dimensions]; [mean] -- the fitted constant or the linear
+    mean's beta / xmin / ymin (mean/linear.f90:53-64; the oracle's least
+    squares), or for the concentrated likelihood its mu_hat (gpr.f90:374-377);
+    [kernels] and one [kernel.k*] block each with theta = l / 2, i.e.
+    exp(theta)/2 for rbf and exp(-2 theta)/2 for periodic
+    (rbf_kernel.f90:127-149, periodic_kernel.f90:126-148); [training_data]
+    units (units.f90:55-78) and the x / y blocks (data_input.f90:62-65,
+    data_output.f90:59-62) equal to the CSV; [weights] solving
+    K alpha = y - m (likelihood.f90:255-264)."""
+    noise = 1e-8
+    model = f'mean = "{mean}"\nlikelihood = "{likelihood}"\n'
+    toml_kernel = f'kernel = "{kernel}"'
+    best_cost, best_params, evals, _ = _run(tmp_path, iterations=4, noise=noise,
+                                            model="")
+    # re-run with this case's model keys (the template fixes mean and kernel)
+    text = (tmp_path / "ferebus.toml").read_text()
+    text = text.replace('mean = "constant"\n', "").replace('kernel = "k1*k2"', toml_kernel)
+    text = text.replace('optimiser = "pso"\n', 'optimiser = "pso"\n' + model)
+    (tmp_path / "ferebus.toml").write_text(text)
+    out = subprocess.run([str(EXE), "-c", "ferebus.toml", "--seed", "3"], cwd=tmp_path, capture_output=True,
+                         text=True, timeout=240)
+    assert out.returncode == 0, out.stderr[-2000:]
+    best_params = None
+    for line in out.stdout.splitlines():
+        if line.strip().startswith("Best Params:"):
+            best_params = np.array([float(v) for v in line.split(":")[1].split()])
+    raw = np.loadtxt(EX / "WATER_O1_TRAINING_SET.csv", delimiter=",", skiprows=1)
+    head = (EX / "WATER_O1_TRAINING_SET.csv").read_text().splitlines()[0].split(",")
+    x = raw[:, [head.index("f1"), head.index("f2"), head.index("f3")]]
+    y = raw[:, head.index("iqa")]
+    ks = {"k1": KernelConfig("k1", "rbf"), "k2": KernelConfig("k2", "periodic", [3])}
+    prog = compile_kernel(kernel, ks, 3)
+    spec = oracle.Spec.from_program(prog)
+    m, beta, xmin, ymin, _ = oracle.mean(mean, x, y)
+    ll, info, alpha_o, o4 = oracle.loglik(x, y, m, spec, best_params, noise=noise, likelihood=likelihood)
+    assert info == 0
+    secs = _model_sections(tmp_path / "WATER_iqa_O1.model")
+    names = [h for h, _ in secs]
+    assert names == ["metadata", "[system]", "[dimensions]", "[mean]", "[kernels]", "[kernel.k1]", "[kernel.k2]",
+                     "[training_data]", "[training_data.x]", "[training_data.y]", "[weights]"], names
+    sec = dict(secs)
+    md = sec["metadata"]
+    assert md[:2] == ["# program ferebus", "# version 7.2.0"]
+    assert md[2].startswith("# nugget ") and float(md[2].split()[2]) == noise
+    ll_file = float(md[3].split()[2])
+    assert md[3].startswith("# likelihood ") and abs(ll_file - ll) <= 1e-6 * abs(ll)
+    assert _kv(sec["[system]"]) == {"name": ["WATER"], "atom": ["O1"], "property": ["iqa"], "ALF": ["1", "2", "3"]}
+    assert _kv(sec["[dimensions]"]) == {"number_of_atoms": ["3"], "number_of_features": ["3"],
+                                        "number_of_training_points": [str(len(y))]}
+    mk = _kv(sec["[mean]"])
+    if likelihood == "concentrated":
+        assert mk["type"] == ["constant"]
+        np.testing.assert_allclose(float(mk["value"][0]), o4[2], rtol=1e-6)
+    elif mean == "constant":
+        assert mk["type"] == ["constant"]
+        np.testing.assert_allclose(float(mk["value"][0]), y.mean(), rtol=1e-14)
+    else:
+        assert mk["type"] == ["linear"]
+        np.testing.assert_allclose([float(v) for v in mk["beta"]], beta, rtol=1e-10)
+        np.testing.assert_allclose([float(v) for v in mk["xmin"]], xmin, rtol=1e-15)
+        np.testing.assert_allclose(float(mk["ymin"][0]), ymin, rtol=1e-15)
+    kk = _kv(sec["[kernels]"])
+    assert kk["number_of_kernels"] == ["2"] and kk["composition"] == [f"({kernel})"]
+    k1, k2 = _kv(sec["[kernel.k1]"]), _kv(sec["[kernel.k2]"])
+    assert k1["type"] == ["rbf"] and k1["number_of_dimensions"] == ["3"] and k1["active_dimensions"] == ["1", "2", "3"]
+    assert k2["type"] == ["periodic"] and k2["number_of_dimensions"] == ["1"] and k2["active_dimensions"] == ["3"]
+    np.testing.assert_allclose([float(v) for v in k1["theta"]], 0.5 * np.exp(best_params[:3]), rtol=1e-14)
+    np.testing.assert_allclose([float(v) for v in k2["theta"]], 0.5 * np.exp(-2.0 * best_params[3:4]), rtol=1e-14)
+    td = _kv(sec["[training_data]"])
+    assert td["units.x"] == ["bohr", "bohr", "radians"] and td["units.y"] == ["Ha"]
+    assert td["scaling.x"] == ["none"] and td["scaling.y"] == ["none"]
+    xs = np.array([[float(v) for v in l.split()] for l in sec["[training_data.x]"]])
+    np.testing.assert_array_equal(xs, x)
+    np.testing.assert_array_equal(np.array([float(v) for v in sec["[training_data.y]"]]), y)
+    wts = np.array([float(v) for v in sec["[weights]"]])
+    assert len(wts) == len(y)
+    k = oracle.kernel_matrix(x, spec, best_params)
+    k = k + np.tril(k, -1).T + noise * np.eye(len(y))
+    r = y - (o4[2] if likelihood == "concentrated" else m)
+    berr = np.abs(k @ wts - r).max() / (np.abs(k).sum(1).max() * np.abs(wts).max() + np.abs(r).max())
+    print(f"{mean}/{likelihood}/{kernel}: ll file {ll_file:.10e} oracle {ll:.10e}, weights backward error {berr:.1e}")
+    assert berr <= 1e-12
