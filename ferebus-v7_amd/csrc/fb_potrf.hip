@@ -297,6 +297,134 @@ __host__ __device__ inline int tail_tiles(int np, int c0, int tail)
 // an earlier launch applied the first k0).  FUSED (64 x 128 row tiles): the
 // roles write their results to memory and the last one publishes F2; each
 // row tile then solves its rows (solve_rows_tile) -- the panel in one launch.
+// Role work-group b (< R ceil(S / 8)) of the chained update launch of panel
+// c0 for the S matrices at K (see panel_chain_kernel).
+template <int KC, bool TWO, bool FUSED>
+__device__ __forceinline__ void chain_role(int b, double *__restrict__ K, int np, int n, int c0, int k0, int S,
+                                           int *__restrict__ info, double *__restrict__ logdet_part,
+                                           double *__restrict__ dinv, int *__restrict__ chain,
+                                           unsigned long long wait_limit, double *smem, int &ok)
+{
+    constexpr int R = chain_roles<TWO>();
+    const int role = (b % R) / 8;
+    const int s = 8 * (b / R) + b % 8;
+    if (s >= S) return;
+    const int tag = c0 / kNBO + 1, c1 = c0 + kNBI;
+    const size_t nbk = (size_t)np / kNBI;
+    double *A = K + (size_t)s * np * np;
+    int *cw = chain + (size_t)s * kChainStride;
+    double *dinv0 = dinv + ((size_t)s * nbk + c0 / kNBI) * kDinv;
+    if (role == 0) {
+        if (info[s] == 0) {
+            gemm_tile_update<kMB, kNBI, KC, 2, true>(A, np, c0, c0, k0, c0, smem);
+            __syncthreads();
+            double lg = 0.0;
+            const int fail = potrf64_block<true, FUSED>(A, np, n, c0, &lg, smem, dinv0);
+            if (threadIdx.x == 0) {
+                if (fail) set_info<FUSED>(info + s, fail);
+                else logdet_part[(size_t)s * nbk + c0 / kNBI] = lg;
+            }
+        }
+        chain_publish(cw + kChainF0, cw + kChainXccB, tag);
+    } else if (TWO) {
+        // BC: the b1 row tile (b1, b0:b1), then the rest of the chain
+        chain_announce(cw + kChainXccB, tag);
+        gemm_tile_update<kMB, kNBO, KC>(A, np, c1, c0, k0, c0, smem);
+        if (!chain_wait(cw + kChainF0, tag, &ok, wait_limit)) {
+            if (threadIdx.x == 0) set_info<FUSED>(info + s, kInfoWaitTimeout);
+            if constexpr (FUSED) chain_publish_mem(cw + kChainF2, tag);
+            return;
+        }
+        if (info[s] != 0) {
+            if constexpr (FUSED) chain_publish_mem(cw + kChainF2, tag);
+            return;
+        }
+        // L(b1, b0) solved in place, then re-read by every wave (global
+        // stores of a work-group are visible to it after the barrier)
+        double *Ls = smem;                       // block (b1, b1), then its factorisation
+        double *tile = A + (size_t)c0 * np + c1;
+        trsm64_solve<false, false, FUSED>(tile, np, tile, np, A + (size_t)c0 * np + c0, np, dinv0);
+        const double *Cg = A + (size_t)c1 * np + c1;
+        for (int e = threadIdx.x; e < kNBI * kNBI / 2; e += kThreads) {
+            const int col = e / (kNBI / 2), r2 = (e % (kNBI / 2)) * 2;
+            *reinterpret_cast<double2 *>(Ls + col * kPLD + r2) = *reinterpret_cast<const double2 *>(Cg + (size_t)col * np + r2);
+        }
+        __syncthreads();
+        syrk64(Ls, tile, np);
+        __syncthreads();
+        double lg = 0.0;
+        const int fail = potrf64_block<true, FUSED>(A, np, n, c1, &lg, Ls,
+                                                    dinv + ((size_t)s * nbk + c1 / kNBI) * kDinv);
+        if (threadIdx.x == 0) {
+            if (fail) set_info<FUSED>(info + s, fail);
+            else logdet_part[(size_t)s * nbk + c1 / kNBI] = lg;
+        }
+        if constexpr (FUSED) chain_publish_mem(cw + kChainF2, tag);
+    } else if (role == 1) {
+        chain_announce(cw + kChainXccB, tag);
+        gemm_tile_update<kMB, kNBI, KC, 2, true>(A, np, c1, c0, k0, c0, smem);
+        if (!chain_wait(cw + kChainF0, tag, &ok, wait_limit)) {
+            if (threadIdx.x == 0) set_info<FUSED>(info + s, kInfoWaitTimeout);
+        } else if (info[s] == 0) {
+            trsm64_solve<false, false, FUSED>(smem, kPLD, A + (size_t)c0 * np + c1, np, A + (size_t)c0 * np + c0,
+                                              np, dinv0);
+        }
+        chain_publish(cw + kChainF1, cw + kChainXccC, tag);
+    } else {
+        chain_announce(cw + kChainXccC, tag);
+        gemm_tile_update<kMB, kNBI, KC, 2, true>(A, np, c1, c1, k0, c0, smem);
+        if (!chain_wait(cw + kChainF1, tag, &ok, wait_limit)) {
+            if (threadIdx.x == 0) set_info<FUSED>(info + s, kInfoWaitTimeout);
+            if constexpr (FUSED) chain_publish_mem(cw + kChainF2, tag);
+            return;
+        }
+        if (info[s] != 0) {
+            if constexpr (FUSED) chain_publish_mem(cw + kChainF2, tag);
+            return;
+        }
+        // L(b1, b0), written by role B, read from memory
+        syrk64(smem, A + (size_t)c0 * np + c1, np);
+        __syncthreads();
+        double lg = 0.0;
+        const int fail = potrf64_block<true, FUSED>(A, np, n, c1, &lg, smem,
+                                                    dinv + ((size_t)s * nbk + c1 / kNBI) * kDinv);
+        if (threadIdx.x == 0) {
+            if (fail) set_info<FUSED>(info + s, fail);
+            else logdet_part[(size_t)s * nbk + c1 / kNBI] = lg;
+        }
+        if constexpr (FUSED) chain_publish_mem(cw + kChainF2, tag);
+    }}
+
+// Row tile lt (< 8 ceil(S rt CT / 8), XCD-aware order) of the chained update
+// launch of panel c0: the MB x NB update of the rows below the panel (and,
+// FUSED, their row solve once the panel's chain is published).
+template <int KC, int MB, int NB, bool FUSED>
+__device__ __forceinline__ void chain_rowtile(int lt, double *__restrict__ K, int np, int c0, int k0, int S,
+                                              int *__restrict__ info, double *__restrict__ dinv,
+                                              int *__restrict__ chain, unsigned long long wait_limit,
+                                              double *smem, int &ok)
+{
+    constexpr int CT = kNBO / NB;
+    const int rt = (np - c0 - kNBO) / MB;
+    const int t = xcd_tile(lt, S * rt * CT);
+    if (t < 0) return;
+    const int s = t / (rt * CT), u = t % (rt * CT);
+    if (info[s] != 0) return;
+    double *A = K + (size_t)s * np * np;
+    const int row0 = c0 + kNBO + (u / CT) * MB;
+    gemm_tile_update<MB, NB, KC>(A, np, row0, c0 + (u % CT) * NB, k0, c0, smem);
+    if constexpr (FUSED) {
+        static_assert(MB == kNBI && NB == kNBO, "the fused row solve needs 64 x 128 row tiles");
+        const int tag = c0 / kNBO + 1;
+        if (!chain_wait(chain + (size_t)s * kChainStride + kChainF2, tag, &ok, wait_limit)) {
+            if (threadIdx.x == 0) set_info<FUSED>(info + s, kInfoWaitTimeout);
+            return;
+        }
+        if (info[s] != 0) return;
+        solve_rows_tile(A, np, c0, row0, dinv + (size_t)s * (np / kNBI) * kDinv);
+    }
+}
+
 template <int KC, int MB, int NB, bool TWO, bool FUSED>
 __global__ void __launch_bounds__(kThreads)
 __attribute__((amdgpu_waves_per_eu(KC == 8 ? 3 : 2, KC == 8 ? 3 : 2))) panel_chain_kernel(double *__restrict__ K, int np, int n,
@@ -313,117 +441,14 @@ __attribute__((amdgpu_waves_per_eu(KC == 8 ? 3 : 2, KC == 8 ? 3 : 2))) panel_cha
     constexpr int R = chain_roles<TWO>();
     const int nrole = R * ceil_div(S, 8);
     if (b < nrole) {
-        const int role = (b % R) / 8;
-        const int s = 8 * (b / R) + b % 8;
-        if (s >= S) return;
-        const int tag = c0 / kNBO + 1, c1 = c0 + kNBI;
-        const size_t nbk = (size_t)np / kNBI;
-        double *A = K + (size_t)s * np * np;
-        int *cw = chain + (size_t)s * kChainStride;
-        double *dinv0 = dinv + ((size_t)s * nbk + c0 / kNBI) * kDinv;
-        if (role == 0) {
-            if (info[s] == 0) {
-                gemm_tile_update<kMB, kNBI, KC, 2, true>(A, np, c0, c0, k0, c0, smem);
-                __syncthreads();
-                double lg = 0.0;
-                const int fail = potrf64_block<true, FUSED>(A, np, n, c0, &lg, smem, dinv0);
-                if (threadIdx.x == 0) {
-                    if (fail) set_info<FUSED>(info + s, fail);
-                    else logdet_part[(size_t)s * nbk + c0 / kNBI] = lg;
-                }
-            }
-            chain_publish(cw + kChainF0, cw + kChainXccB, tag);
-        } else if (TWO) {
-            // BC: the b1 row tile (b1, b0:b1), then the rest of the chain
-            chain_announce(cw + kChainXccB, tag);
-            gemm_tile_update<kMB, kNBO, KC>(A, np, c1, c0, k0, c0, smem);
-            if (!chain_wait(cw + kChainF0, tag, &ok, wait_limit)) {
-                if (threadIdx.x == 0) set_info<FUSED>(info + s, kInfoWaitTimeout);
-                if constexpr (FUSED) chain_publish_mem(cw + kChainF2, tag);
-                return;
-            }
-            if (info[s] != 0) {
-                if constexpr (FUSED) chain_publish_mem(cw + kChainF2, tag);
-                return;
-            }
-            // L(b1, b0) solved in place, then re-read by every wave (global
-            // stores of a work-group are visible to it after the barrier)
-            double *Ls = smem;                       // block (b1, b1), then its factorisation
-            double *tile = A + (size_t)c0 * np + c1;
-            trsm64_solve<false, false, FUSED>(tile, np, tile, np, A + (size_t)c0 * np + c0, np, dinv0);
-            const double *Cg = A + (size_t)c1 * np + c1;
-            for (int e = threadIdx.x; e < kNBI * kNBI / 2; e += kThreads) {
-                const int col = e / (kNBI / 2), r2 = (e % (kNBI / 2)) * 2;
-                *reinterpret_cast<double2 *>(Ls + col * kPLD + r2) = *reinterpret_cast<const double2 *>(Cg + (size_t)col * np + r2);
-            }
-            __syncthreads();
-            syrk64(Ls, tile, np);
-            __syncthreads();
-            double lg = 0.0;
-            const int fail = potrf64_block<true, FUSED>(A, np, n, c1, &lg, Ls,
-                                                        dinv + ((size_t)s * nbk + c1 / kNBI) * kDinv);
-            if (threadIdx.x == 0) {
-                if (fail) set_info<FUSED>(info + s, fail);
-                else logdet_part[(size_t)s * nbk + c1 / kNBI] = lg;
-            }
-            if constexpr (FUSED) chain_publish_mem(cw + kChainF2, tag);
-        } else if (role == 1) {
-            chain_announce(cw + kChainXccB, tag);
-            gemm_tile_update<kMB, kNBI, KC, 2, true>(A, np, c1, c0, k0, c0, smem);
-            if (!chain_wait(cw + kChainF0, tag, &ok, wait_limit)) {
-                if (threadIdx.x == 0) set_info<FUSED>(info + s, kInfoWaitTimeout);
-            } else if (info[s] == 0) {
-                trsm64_solve<false, false, FUSED>(smem, kPLD, A + (size_t)c0 * np + c1, np, A + (size_t)c0 * np + c0,
-                                                  np, dinv0);
-            }
-            chain_publish(cw + kChainF1, cw + kChainXccC, tag);
-        } else {
-            chain_announce(cw + kChainXccC, tag);
-            gemm_tile_update<kMB, kNBI, KC, 2, true>(A, np, c1, c1, k0, c0, smem);
-            if (!chain_wait(cw + kChainF1, tag, &ok, wait_limit)) {
-                if (threadIdx.x == 0) set_info<FUSED>(info + s, kInfoWaitTimeout);
-                if constexpr (FUSED) chain_publish_mem(cw + kChainF2, tag);
-                return;
-            }
-            if (info[s] != 0) {
-                if constexpr (FUSED) chain_publish_mem(cw + kChainF2, tag);
-                return;
-            }
-            // L(b1, b0), written by role B, read from memory
-            syrk64(smem, A + (size_t)c0 * np + c1, np);
-            __syncthreads();
-            double lg = 0.0;
-            const int fail = potrf64_block<true, FUSED>(A, np, n, c1, &lg, smem,
-                                                        dinv + ((size_t)s * nbk + c1 / kNBI) * kDinv);
-            if (threadIdx.x == 0) {
-                if (fail) set_info<FUSED>(info + s, fail);
-                else logdet_part[(size_t)s * nbk + c1 / kNBI] = lg;
-            }
-            if constexpr (FUSED) chain_publish_mem(cw + kChainF2, tag);
-        }
+        chain_role<KC, TWO, FUSED>(b, K, np, n, c0, k0, S, info, logdet_part, dinv, chain, wait_limit, smem, ok);
         return;
     }
     constexpr int CT = kNBO / NB;            // column tiles of the panel
     const int rt = (np - c0 - kNBO) / MB;    // row tiles below the panel
     const int nreg = 8 * ceil_div(S * rt * CT, 8);
     if (b - nrole < nreg) {
-        const int t = xcd_tile(b - nrole, S * rt * CT);
-        if (t < 0) return;
-        const int s = t / (rt * CT), u = t % (rt * CT);
-        if (info[s] != 0) return;
-        double *A = K + (size_t)s * np * np;
-        const int row0 = c0 + kNBO + (u / CT) * MB;
-        gemm_tile_update<MB, NB, KC>(A, np, row0, c0 + (u % CT) * NB, k0, c0, smem);
-        if constexpr (FUSED) {
-            static_assert(MB == kNBI && NB == kNBO, "the fused row solve needs 64 x 128 row tiles");
-            const int tag = c0 / kNBO + 1;
-            if (!chain_wait(chain + (size_t)s * kChainStride + kChainF2, tag, &ok, wait_limit)) {
-                if (threadIdx.x == 0) set_info<FUSED>(info + s, kInfoWaitTimeout);
-                return;
-            }
-            if (info[s] != 0) return;
-            solve_rows_tile(A, np, c0, row0, dinv + (size_t)s * (np / kNBI) * kDinv);
-        }
+        chain_rowtile<KC, MB, NB, FUSED>(b - nrole, K, np, c0, k0, S, info, dinv, chain, wait_limit, smem, ok);
         return;
     }
     // tail pre-update tiles
