@@ -71,12 +71,13 @@ def potrf_flops(n: int) -> float:
 def uneven_run(steps: int = 3):
     """Batched uneven training sets: 10 glycine-like GPs (d=24, N ~ U[1000,
     4000]), 40 particles each -- 400 factorisations of mixed size per PSO
-    iteration, evaluated by one fb_multi_swarm_loglik call.  Timed three
-    ways: the call as autotuned (it times the persistent single launch
-    against the handles' own batched paths one after another on its first
-    call and keeps the faster), the persistent single launch pinned
-    (FB_MULTI_PERSISTENT=1), and the ten handles' own evaluations called one
-    after another; the results are bit-identical."""
+    iteration, evaluated by one fb_multi_swarm_loglik call.  Timed four
+    ways: the call as autotuned (its first call times the schedules below
+    and keeps the fastest), the merged chained schedule pinned
+    (FB_MULTI_SCHEDULE=2: every GP's matrices in each lock-step launch, one
+    hipGraph per iteration), the persistent single launch pinned (=1), and
+    the ten handles' own evaluations called one after another; the results
+    are bit-identical."""
     import torch
     from ferebus_amd import MultiSwarm, SwarmLikelihood, compile_kernel, fit_mean, synthetic
     ws = synthetic.glycine_gps(seed=2)
@@ -88,13 +89,16 @@ def uneven_run(steps: int = 3):
                                    likelihood=w.likelihood, max_swarm=40))
         thetas.append(rng.uniform(0.0, 1.0, (40, gps[-1].nparams)))
     multi = MultiSwarm(gps)
-    os.environ["FB_MULTI_PERSISTENT"] = "1"
-    try:
-        one = MultiSwarm(gps)
-    finally:
-        del os.environ["FB_MULTI_PERSISTENT"]
+    pinned = {}
+    for sched in ("2", "1"):
+        os.environ["FB_MULTI_SCHEDULE"] = sched
+        try:
+            pinned[sched] = MultiSwarm(gps)
+        finally:
+            del os.environ["FB_MULTI_SCHEDULE"]
     multi.swarm_loglik(thetas)               # tunes every handle and the schedule
-    one.swarm_loglik(thetas)
+    for mm in pinned.values():
+        mm.swarm_loglik(thetas)
 
     def timed(fn):
         torch.cuda.synchronize()
@@ -104,17 +108,21 @@ def uneven_run(steps: int = 3):
         return (time.perf_counter() - t) / steps, out
 
     t_multi, (lls, infos) = timed(lambda: multi.swarm_loglik(thetas))
-    t_one, (lls1, _) = timed(lambda: one.swarm_loglik(thetas))
+    t_merged, (llm, _) = timed(lambda: pinned["2"].swarm_loglik(thetas))
+    t_one, (lls1, _) = timed(lambda: pinned["1"].swarm_loglik(thetas))
     t_seq, _ = timed(lambda: [gp.swarm_loglik(th) for gp, th in zip(gps, thetas)])
-    same = all(np.array_equal(gp.swarm_loglik(th)[0], ll, equal_nan=True) and np.array_equal(ll, l1, equal_nan=True)
-               for gp, th, ll, l1 in zip(gps, thetas, lls, lls1))
+    same = all(np.array_equal(gp.swarm_loglik(th)[0], ll, equal_nan=True) and
+               np.array_equal(ll, l1, equal_nan=True) and np.array_equal(ll, l2, equal_nan=True)
+               for gp, th, ll, l1, l2 in zip(gps, thetas, lls, lls1, llm))
     out = {"gps": len(gps), "N": [gp.n for gp in gps], "particles": 40 * len(gps),
-           "ms_batched_call": round(1e3 * t_multi, 3), "ms_one_launch": round(1e3 * t_one, 3),
+           "ms_batched_call": round(1e3 * t_multi, 3), "schedule": multi.schedule(),
+           "ms_merged_lockstep": round(1e3 * t_merged, 3), "ms_one_launch": round(1e3 * t_one, 3),
            "ms_per_gp_evaluations": round(1e3 * t_seq, 3),
            "evals_per_s_batched_call": round(40 * len(gps) / t_multi, 1),
            "per_gp_paths": sorted({gp.factor_path(40) for gp in gps}), "bit_identical": bool(same)}
+    for mm in pinned.values():
+        mm.close()
     multi.close()
-    one.close()
     for gp in gps:
         gp.close()
     return out

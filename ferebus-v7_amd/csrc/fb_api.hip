@@ -1255,17 +1255,25 @@ int fb_swarm_loglik_static(fb_handle *h, int s, const double *d_theta, double *d
 
 // ---- several handles in one persistent factorisation ------------------------
 
+// Schedules of fb_multi_swarm_loglik.
+enum { kMultiSequential = 0, kMultiPersistent = 1, kMultiMerged = 2, kMultiSchedules = 3 };
+
 struct fb_multi {
     std::vector<fb_handle *> gps;
     int dag_grid = 0, n_xcd = 1;
     fb::DagWork w;
     int *h_err = nullptr;
     std::map<std::vector<int>, std::pair<fb::DagPlan, fb::DagMatrix *>> plans;   // per particle counts
-    // per particle counts: all factorisations in one persistent launch (true)
-    // or each handle's own tuned batched path one after another (timed on the
-    // first call; FB_MULTI_PERSISTENT=0/1 pins it)
-    std::map<std::vector<int>, bool> persistent;
+    // per particle counts: the schedule -- each handle's own tuned batched
+    // path one after another (kMultiSequential), all factorisations in one
+    // persistent launch (kMultiPersistent), or the chained lock-step schedule
+    // with every handle's matrices in each launch (kMultiMerged) -- timed on
+    // the first call; FB_MULTI_SCHEDULE=0/1/2 (or FB_MULTI_PERSISTENT=0/1)
+    // pins it
+    std::map<std::vector<int>, int> schedule;
+    std::map<std::vector<int>, std::pair<hipGraph_t, hipGraphExec_t>> graphs;   // merged schedule, per counts
     int forced = -1;
+    int merged_tile = 0;   // panel tile code of the merged launches (FB_MULTI_TILE)
 };
 
 namespace {
@@ -1275,7 +1283,74 @@ void free_multi(fb_multi *m)
     for (void *p : {(void *)m->w.ctl, (void *)m->w.dep, (void *)m->w.slots, (void *)m->w.dead})
         if (p) (void)hipFree(p);
     for (auto &pl : m->plans) (void)hipFree(pl.second.second);
+    for (auto &g : m->graphs) {
+        (void)hipGraphExecDestroy(g.second.second);
+        (void)hipGraphDestroy(g.second.first);
+    }
     if (m->h_err) (void)hipHostFree(m->h_err);
+}
+
+// Buffers of a handle's swarm in the merged schedule: the chained lock-step
+// variant (16-deep chunks, the chain words reset by the build), no
+// pre-updates of later panels.
+fb::SwarmBuffers merged_buffers(fb_handle *h, int s)
+{
+    fb::SwarmBuffers b = buffers(h, 0, s, h->d_theta, h->d_ll, h->d_info);
+    b.gemm_kc = 16;
+    b.tile = 0;
+    b.panel_tile = nullptr;
+    b.chain = h->d_chain;
+    b.tail = 0;
+    b.lookahead = 0;
+    return b;
+}
+
+// The merged schedule: every handle's build; then launch p of the chained
+// schedule holds panel p - (P - P_g) of handle g (P_g panels of 128
+// columns, P the most), so that the last, smallest panels of all the
+// handles share the last launches, where one swarm alone fills the chip
+// least; then every handle's likelihood.
+void enqueue_merged(fb_multi *m, const std::vector<int> &sv, hipStream_t st)
+{
+    const int G = (int)m->gps.size();
+    std::vector<fb::SwarmBuffers> bs(G);
+    int P = 0;
+    for (int g = 0; g < G; ++g) {
+        if (sv[g] == 0) continue;
+        bs[g] = merged_buffers(m->gps[g], sv[g]);
+        fb::launch_build(bs[g], st);
+        P = std::max(P, bs[g].np / fb::kNBO);
+    }
+    for (int p = 0; p < P; ++p) {
+        fb::ChainMulti cm;
+        cm.tile = m->merged_tile;
+        cm.wait_limit = ~0ull;
+        for (int g = 0; g < G; ++g) {
+            if (sv[g] == 0) continue;
+            const fb::SwarmBuffers &b = bs[g];
+            const int q = p - (P - b.np / fb::kNBO);
+            if (q < 0) continue;
+            fb::ChainSegment &sg = cm.seg[cm.G++];
+            sg.K = b.K; sg.info = b.info; sg.logdet_part = b.logdet_part; sg.dinv = b.dinv; sg.chain = b.chain;
+            sg.np = b.np; sg.n = b.n; sg.S = b.S; sg.c0 = q * fb::kNBO;
+            cm.wait_limit = std::min(cm.wait_limit, b.wait_limit);
+            if (cm.G == fb::kMultiMaxSegments) {
+                fb::launch_chain_multi(cm, st);
+                cm.G = 0;
+            }
+        }
+        if (cm.G) fb::launch_chain_multi(cm, st);
+    }
+    for (int g = 0; g < G; ++g)
+        if (sv[g] > 0) fb::launch_likelihood(bs[g], st);
+}
+
+// The merged schedule needs the chained schedule's words in every handle.
+bool merged_possible(const fb_multi *m, const std::vector<int> &sv)
+{
+    for (size_t g = 0; g < m->gps.size(); ++g)
+        if (sv[g] > 0 && !m->gps[g]->chain) return false;
+    return true;
 }
 
 int multi_plan(fb_multi *m, const std::vector<int> &s)
@@ -1339,9 +1414,22 @@ int fb_multi_create(fb_multi **out, int n_gps, fb_handle *const *gps)
         (void)hipGetLastError();
         return fail(FB_E_NOMEM, "fb_multi_create: scheduler workspace");
     }
-    if (const char *env = std::getenv("FB_MULTI_PERSISTENT")) m->forced = std::atoi(env) != 0 ? 1 : 0;
+    if (const char *env = std::getenv("FB_MULTI_PERSISTENT"))
+        m->forced = std::atoi(env) != 0 ? kMultiPersistent : kMultiSequential;
+    if (const char *env = std::getenv("FB_MULTI_SCHEDULE")) {
+        const int v = std::atoi(env);
+        if (v >= 0 && v < kMultiSchedules) m->forced = v;
+    }
+    if (const char *env = std::getenv("FB_MULTI_TILE")) m->merged_tile = std::atoi(env);
     *out = m;
     return FB_OK;
+}
+
+int fb_multi_schedule(const fb_multi *m, const int *s)
+{
+    if (!m || !s) return -1;
+    auto it = m->schedule.find(std::vector<int>(s, s + m->gps.size()));
+    return it == m->schedule.end() ? -1 : it->second;
 }
 
 int fb_multi_destroy(fb_multi *m)
@@ -1365,8 +1453,7 @@ int fb_multi_swarm_loglik(fb_multi *m, const int *s, const double *const *theta,
         if (sv[g] > 0 && (!theta[g] || !ll[g] || !info[g]))
             return fail(FB_E_ARG, "fb_multi_swarm_loglik: null argument");
     }
-    int rc = multi_plan(m, sv);
-    if (rc) return rc;
+    if (std::all_of(sv.begin(), sv.end(), [](int v) { return v == 0; })) return FB_OK;
     hipStream_t st = m->gps[0]->stream;
     for (int g = 0; g < G; ++g) {
         if (sv[g] == 0) continue;
@@ -1375,74 +1462,125 @@ int fb_multi_swarm_loglik(fb_multi *m, const int *s, const double *const *theta,
         std::memcpy(h->h_theta, theta[g], sizeof(double) * P * sv[g]);
         FB_HIP(hipMemcpyAsync(h->d_theta, h->h_theta, sizeof(double) * P * sv[g], hipMemcpyHostToDevice, st));
     }
-    auto run = [&](bool persistent) -> int {
-        for (int g = 0; g < G; ++g) {
-            if (sv[g] == 0) continue;
-            fb_handle *h = m->gps[g];
-            if (!persistent) {
-                // the handle's own tuned lock-step variant for this swarm size
-                auto it = h->tuned.find(sv[g]);
-                set_mode(h, it != h->tuned.end() && it->second != 0 ? it->second : 2, sv[g]);
-            }
-            const fb::SwarmBuffers b = buffers(h, 0, sv[g], h->d_theta, h->d_ll, h->d_info);
-            if (persistent) fb::launch_build(b, st);
-            else enqueue_group(h, b, st);
+    int rc = FB_OK;
+    // schedules this call may use: the persistent one only where its plan
+    // fits one launch, the merged one only with every handle chained
+    bool can[kMultiSchedules] = {true, false, merged_possible(m, sv)};
+    auto plan = [&]() -> int {
+        if (!m->plans.count(sv)) {
+            const int r = multi_plan(m, sv);
+            if (r) return r;
         }
-        if (persistent && m->plans.at(sv).second)
-            fb::launch_dag(m->plans.at(sv).first, m->plans.at(sv).second, m->w, m->dag_grid, st);
+        can[kMultiPersistent] = true;
+        return FB_OK;
+    };
+    auto run = [&](int mode) -> int {
+        if (mode == kMultiMerged) {
+            auto it = m->graphs.find(sv);
+            if (it == m->graphs.end()) {
+                hipGraph_t gr = nullptr;
+                hipGraphExec_t ge = nullptr;
+                FB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                enqueue_merged(m, sv, st);
+                FB_HIP(hipStreamEndCapture(st, &gr));
+                if (hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0) != hipSuccess) {
+                    (void)hipGraphDestroy(gr);
+                    return fail(FB_E_HIP, "fb_multi_swarm_loglik: graph instantiation failed");
+                }
+                it = m->graphs.emplace(sv, std::make_pair(gr, ge)).first;
+            }
+            FB_HIP(hipGraphLaunch(it->second.second, st));
+        } else {
+            const bool persistent = mode == kMultiPersistent;
+            for (int g = 0; g < G; ++g) {
+                if (sv[g] == 0) continue;
+                fb_handle *h = m->gps[g];
+                if (!persistent) {
+                    // the handle's own tuned lock-step variant for this swarm size
+                    auto it = h->tuned.find(sv[g]);
+                    set_mode(h, it != h->tuned.end() && it->second != 0 ? it->second : 2, sv[g]);
+                }
+                const fb::SwarmBuffers b = buffers(h, 0, sv[g], h->d_theta, h->d_ll, h->d_info);
+                if (persistent) fb::launch_build(b, st);
+                else enqueue_group(h, b, st);
+            }
+            if (persistent && m->plans.at(sv).second)
+                fb::launch_dag(m->plans.at(sv).first, m->plans.at(sv).second, m->w, m->dag_grid, st);
+            if (persistent)
+                for (int g = 0; g < G; ++g)
+                    if (sv[g] > 0)
+                        fb::launch_likelihood(buffers(m->gps[g], 0, sv[g], m->gps[g]->d_theta, m->gps[g]->d_ll,
+                                                      m->gps[g]->d_info), st);
+        }
         for (int g = 0; g < G; ++g) {
             if (sv[g] == 0) continue;
             fb_handle *h = m->gps[g];
-            if (persistent) fb::launch_likelihood(buffers(h, 0, sv[g], h->d_theta, h->d_ll, h->d_info), st);
             FB_HIP(hipMemcpyAsync(h->h_ll, h->d_ll, sizeof(double) * sv[g], hipMemcpyDeviceToHost, st));
             FB_HIP(hipMemcpyAsync(h->h_info, h->d_info, sizeof(int) * sv[g], hipMemcpyDeviceToHost, st));
         }
         *m->h_err = 0;
-        if (persistent) FB_HIP(hipMemcpyAsync(m->h_err, m->w.ctl + fb::kDagErrorWord, sizeof(int), hipMemcpyDeviceToHost, st));
+        if (mode == kMultiPersistent)
+            FB_HIP(hipMemcpyAsync(m->h_err, m->w.ctl + fb::kDagErrorWord, sizeof(int), hipMemcpyDeviceToHost, st));
         FB_HIP(hipStreamSynchronize(st));
         FB_HIP(hipGetLastError());
         return FB_OK;
     };
-    if (!m->persistent.count(sv)) {
+    if (!m->schedule.count(sv)) {
         if (m->forced >= 0) {
-            m->persistent[sv] = m->forced != 0;
+            if (m->forced == kMultiPersistent) {
+                rc = plan();
+                if (rc) return rc;
+            }
+            if (m->forced == kMultiMerged && !can[kMultiMerged])
+                return fail(FB_E_ARG, "fb_multi_swarm_loglik: the merged schedule needs the chained schedule "
+                                      "(FB_CHAIN=0 is set)");
+            m->schedule[sv] = m->forced;
         } else {
-            // tune every handle for its swarm size, then time both schedules
+            // tune every handle for its swarm size, then time the schedules
             FB_HIP(hipStreamSynchronize(st));
             for (int g = 0; g < G; ++g)
                 if (sv[g] > 0) {
                     rc = choose_path(m->gps[g], sv[g]);
                     if (rc) return rc;
                 }
+            (void)plan();   // no persistent candidate when its plan does not fit
             hipEvent_t e0, e1;
             FB_HIP(hipEventCreate(&e0));
             FB_HIP(hipEventCreate(&e1));
-            float best[2] = {1e30f, 1e30f};
-            for (int rep = 0; rep < 2; ++rep)
-                for (int p = 0; p < 2; ++p) {
+            float best[kMultiSchedules] = {1e30f, 1e30f, 1e30f};
+            for (int rep = 0; rep < 2 && rc == FB_OK; ++rep)
+                for (int p = 0; p < kMultiSchedules; ++p) {
+                    if (!can[p]) continue;
                     (void)hipEventRecord(e0, st);
-                    rc = run(p == 1);
+                    rc = run(p);
                     (void)hipEventRecord(e1, st);
                     if (rc || hipEventSynchronize(e1) != hipSuccess) break;
                     float ms = 0.f;
                     (void)hipEventElapsedTime(&ms, e0, e1);
-                    if (p == 1 && *m->h_err != 0) ms = 1e30f;   // a persistent run that could not be served
+                    if (p == kMultiPersistent && *m->h_err != 0) ms = 1e30f;   // could not be served
                     best[p] = std::min(best[p], ms);
                 }
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
             if (rc) return rc;
-            m->persistent[sv] = best[1] <= best[0];
+            int pick = kMultiSequential;
+            for (int p = 1; p < kMultiSchedules; ++p)
+                if (best[p] < best[pick]) pick = p;
+            m->schedule[sv] = pick;
         }
     }
-    const bool persistent = m->persistent.at(sv);
-    rc = run(persistent);
+    const int mode = m->schedule.at(sv);
+    if (mode == kMultiPersistent) {
+        rc = plan();
+        if (rc) return rc;
+    }
+    rc = run(mode);
     if (rc) return rc;
-    if (persistent && *m->h_err == 1)
+    if (mode == kMultiPersistent && *m->h_err == 1)
         return fail(FB_E_HIP, "fb_multi_swarm_loglik: the persistent factorisation found no ready task for 2 s");
-    if (persistent && *m->h_err == 2) {
-        m->persistent[sv] = false;
-        rc = run(false);   // some XCD queue was not served: lock-step path
+    if (mode == kMultiPersistent && *m->h_err == 2) {
+        m->schedule[sv] = kMultiSequential;
+        rc = run(kMultiSequential);   // some XCD queue was not served: lock-step path
         if (rc) return rc;
     }
     for (int g = 0; g < G; ++g)

@@ -232,6 +232,29 @@ void launch_solve_rows(const SwarmBuffers &b, int c0, hipStream_t st);
 bool panel_fused(const SwarmBuffers &b, int c0);
 void launch_likelihood(const SwarmBuffers &b, hipStream_t st);
 
+// Several swarms (training sets of different N) in one launch of the
+// chained schedule: one segment per swarm with the panel column c0 it
+// factors in this launch.  The launcher fills the block-id ends.
+constexpr int kMultiMaxSegments = 16;
+struct ChainSegment {
+    double *K;             // S x np x np
+    int *info;             // S
+    double *logdet_part;   // S x (np / kNBI)
+    double *dinv;          // S x (np / kNBI) x kDinv
+    int *chain;            // S x kChainStride, reset by the swarm's build
+    int np, n, S, c0;
+    int role_end, tile_end, solve_end;   // exclusive ends of the segment's block ids
+};
+struct ChainMulti {
+    int G = 0;             // segments
+    int tile = 0;          // panel tile code (0, 1, 2, | kTileTwoRoles, kTileFused, kTileThreePerCU)
+    unsigned long long wait_limit = 100000000ull;
+    ChainSegment seg[kMultiMaxSegments];
+};
+// The update launch (and, unless fused, the row-solve launch) of one panel
+// of every segment.
+void launch_chain_multi(ChainMulti &m, hipStream_t st);
+
 // Adjoint gradient (fb_grad.hip).
 struct GradWork {
     double *U = nullptr;        // S x np x np: L^{-T}

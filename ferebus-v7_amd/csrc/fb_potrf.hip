@@ -469,6 +469,19 @@ __attribute__((amdgpu_waves_per_eu(KC == 8 ? 3 : 2, KC == 8 ? 3 : 2))) panel_cha
     else gemm_tile_update<kMB, kNBO, KC>(A, np, cp + kNBI * v, cp, kt, c0, smem);
 }
 
+// Row-solve tile lt (< 8 ceil(S rt / 8), XCD-aware order) of panel c0.
+__device__ __forceinline__ void solve_rows_block(int lt, double *__restrict__ K, int np, int c0, int S,
+                                                 const int *__restrict__ info, const double *__restrict__ dinv)
+{
+    const int rt = (np - c0 - kNBO) / kNBI;
+    const int t = xcd_tile(lt, S * rt);
+    if (t < 0) return;
+    const int s = t / rt;
+    const int row0 = c0 + kNBO + (t % rt) * kNBI;
+    if (info[s] != 0) return;
+    solve_rows_tile(K + (size_t)s * np * np, np, c0, row0, dinv + (size_t)s * (np / kNBI) * kDinv);
+}
+
 // Launch 2 of panel c0, chained: every row tile below the panel is solved
 // (solve_rows_tile: launches 2 and 3 of the three-launch schedule for one
 // tile, each wave on its own 16 rows; no LDS).
@@ -478,13 +491,57 @@ solve_rows_kernel(double *__restrict__ K, int np, int c0, int S,
                                                               const int *__restrict__ info,
                                                               const double *__restrict__ dinv)
 {
-    const int rt = (np - c0 - kNBO) / kNBI;
-    const int t = xcd_tile(blockIdx.x, S * rt);
-    if (t < 0) return;
-    const int s = t / rt;
-    const int row0 = c0 + kNBO + (t % rt) * kNBI;
-    if (info[s] != 0) return;
-    solve_rows_tile(K + (size_t)s * np * np, np, c0, row0, dinv + (size_t)s * (np / kNBI) * kDinv);
+    solve_rows_block(blockIdx.x, K, np, c0, S, info, dinv);
+}
+
+// ---- several swarms in one chained launch ----------------------------------
+//
+// The swarms of several training sets (different N) share each launch of the
+// chained schedule: segment g of a launch is one panel (column c0_g) of
+// swarm g.  Block ids: the role work-groups of every segment first (segment
+// g from role_end[g-1], a multiple of 8, so a particle's three roles keep one
+// XCD and the order D < B < C), then the row tiles of every segment, each
+// segment padded to a multiple of 8 in XCD-aware order.  Per matrix the
+// tiles, their order and the hand-over are those of panel_chain_kernel for
+// the same tile code, so each matrix gets the bits of its own evaluation.
+//
+// Progress: the waits no longer rest on a bound of the swarm size (all the
+// segments' particles together exceed kChainMaxSwarm) but on the dispatch
+// order, as in the fused variant: on each XCD work-groups are dispatched in
+// increasing id; B and C of a particle are dispatched after its D (lower id,
+// same XCD), which never waits, and C after its B, which waits for D only,
+// so each producer a role waits for holds a slot or has finished; the fused
+// row tiles come after every role.  Every wait is still bounded (INFO
+// kInfoWaitTimeout).
+template <int KC, int MB, int NB, bool TWO, bool FUSED>
+__global__ void __launch_bounds__(kThreads)
+__attribute__((amdgpu_waves_per_eu(KC == 8 ? 3 : 2, KC == 8 ? 3 : 2))) panel_chain_multi_kernel(ChainMulti m)
+{
+    __shared__ __attribute__((aligned(16))) double smem[chain_smem<KC, MB, NB>()];
+    __shared__ int ok;
+    const int b = blockIdx.x;
+    int g = 0;
+    if (b < m.seg[m.G - 1].role_end) {
+        while (b >= m.seg[g].role_end) ++g;
+        const ChainSegment &q = m.seg[g];
+        chain_role<KC, TWO, FUSED>(b - (g ? m.seg[g - 1].role_end : 0), q.K, q.np, q.n, q.c0, 0, q.S, q.info,
+                                   q.logdet_part, q.dinv, q.chain, m.wait_limit, smem, ok);
+        return;
+    }
+    while (b >= m.seg[g].tile_end) ++g;
+    const ChainSegment &q = m.seg[g];
+    chain_rowtile<KC, MB, NB, FUSED>(b - (g ? m.seg[g - 1].tile_end : m.seg[m.G - 1].role_end), q.K, q.np, q.c0, 0,
+                                     q.S, q.info, q.dinv, q.chain, m.wait_limit, smem, ok);
+}
+
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3)))
+solve_rows_multi_kernel(ChainMulti m)
+{
+    const int b = blockIdx.x;
+    int g = 0;
+    while (b >= m.seg[g].solve_end) ++g;
+    const ChainSegment &q = m.seg[g];
+    solve_rows_block(b - (g ? m.seg[g - 1].solve_end : 0), q.K, q.np, q.c0, q.S, q.info, q.dinv);
 }
 
 template <int KC, int MB, int NB, bool TWO, bool FUSED = false>
@@ -560,6 +617,39 @@ void launch_solve_rows(const SwarmBuffers &b, int c0, hipStream_t st)
     if (rt <= 0) return;
     hipLaunchKernelGGL(solve_rows_kernel, dim3(8 * ceil_div(b.S * rt, 8)), dim3(kThreads), 0, st, b.K, b.np,
                        c0, b.S, b.info, b.dinv);
+}
+
+template <int KC, int MB, int NB, bool TWO, bool FUSED>
+static void chain_multi_launch(ChainMulti &m, hipStream_t st)
+{
+    int r = 0;
+    for (int g = 0; g < m.G; ++g) m.seg[g].role_end = r += chain_roles<TWO>() * ceil_div(m.seg[g].S, 8);
+    int t = r, s = 0;
+    for (int g = 0; g < m.G; ++g) {
+        ChainSegment &q = m.seg[g];
+        const int rows = q.np - q.c0 - kNBO;
+        q.tile_end = t += 8 * ceil_div(q.S * (rows / MB) * (kNBO / NB), 8);
+        q.solve_end = s += FUSED ? 0 : 8 * ceil_div(q.S * (rows / kNBI), 8);
+    }
+    hipLaunchKernelGGL((panel_chain_multi_kernel<KC, MB, NB, TWO, FUSED>), dim3(t), dim3(kThreads), 0, st, m);
+    if (s > 0) hipLaunchKernelGGL(solve_rows_multi_kernel, dim3(s), dim3(kThreads), 0, st, m);
+}
+
+template <bool TWO>
+static void chain_multi_tiles(ChainMulti &m, hipStream_t st)
+{
+    if (m.tile & kTileFused) chain_multi_launch<16, 64, 128, TWO, true>(m, st);
+    else if (m.tile & kTileThreePerCU) chain_multi_launch<8, 64, 128, TWO, false>(m, st);
+    else if ((m.tile & 3) == 1) chain_multi_launch<16, 32, 128, TWO, false>(m, st);
+    else if ((m.tile & 3) == 2) chain_multi_launch<16, 32, 64, TWO, false>(m, st);
+    else chain_multi_launch<16, 64, 128, TWO, false>(m, st);
+}
+
+void launch_chain_multi(ChainMulti &m, hipStream_t st)
+{
+    if (m.G < 1 || m.G > kMultiMaxSegments) return;
+    if (m.tile & kTileTwoRoles) chain_multi_tiles<true>(m, st);
+    else chain_multi_tiles<false>(m, st);
 }
 
 template <int KC, int MB, int NB>

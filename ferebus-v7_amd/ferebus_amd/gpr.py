@@ -200,9 +200,11 @@ class SwarmLikelihood:
 
 class MultiSwarm:
     """The swarms of several training sets (one SwarmLikelihood each, any N)
-    evaluated together: builds and likelihoods per training set, the
-    factorisations of all of them in one persistent launch
-    (fb_multi_swarm_loglik).  Results are bit-identical to each
+    evaluated together (fb_multi_swarm_loglik): builds and likelihoods per
+    training set, the factorisations of all of them on the schedule the
+    first call timed fastest -- the sets one after another, one persistent
+    launch, or the chained lock-step launches shared by every set and
+    replayed as one hipGraph.  Results are bit-identical to each
     SwarmLikelihood.swarm_loglik."""
 
     def __init__(self, gps: Sequence[SwarmLikelihood]):
@@ -210,6 +212,7 @@ class MultiSwarm:
         self.gps = list(gps)
         arr = (ctypes.c_void_p * len(self.gps))(*[g._h.value for g in self.gps])
         self._h = ctypes.c_void_p()
+        self._last = np.zeros(len(self.gps), dtype=np.int32)
         _abi.check(self._lib.fb_multi_create(ctypes.byref(self._h), len(self.gps), arr), "fb_multi_create")
 
     def swarm_loglik(self, thetas):
@@ -219,6 +222,7 @@ class MultiSwarm:
         ts = [np.zeros((0, g.nparams)) if t is None else g._theta(t) if len(np.atleast_2d(t)) else
               np.zeros((0, g.nparams)) for g, t in zip(self.gps, thetas)]
         s = np.array([len(t) for t in ts], dtype=np.int32)
+        self._last = s
         lls = [np.empty(len(t)) for t in ts]
         infos = [np.empty(len(t), dtype=np.int32) for t in ts]
         P_DBL, P_INT = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
@@ -228,6 +232,16 @@ class MultiSwarm:
         _abi.check(self._lib.fb_multi_swarm_loglik(self._h, _abi.iptr(s), th_p, ll_p, in_p),
                    "fb_multi_swarm_loglik")
         return lls, infos
+
+    SCHEDULES = ("sequential", "persistent", "merged")
+
+    def schedule(self, sizes=None):
+        """The schedule chosen for the swarm sizes of the last call (or
+        `sizes`): "sequential", "persistent", "merged", or None before a call
+        with those sizes."""
+        s = np.array(self._last if sizes is None else sizes, dtype=np.int32)
+        k = self._lib.fb_multi_schedule(self._h, _abi.iptr(s))
+        return None if k < 0 else self.SCHEDULES[k]
 
     def close(self) -> None:
         if getattr(self, "_h", None) is not None and self._h.value:

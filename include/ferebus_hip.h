@@ -181,13 +181,17 @@ int fb_swarm_loglik_grad(fb_handle *h, int s, const double *theta, double *ll,
 
 /* ---- several training sets at once (uneven N) ------------------------------
  * The swarms of several handles -- e.g. one GP per atom or property, each
- * with its own N, kernel and likelihood -- are evaluated by one call: their
- * factorisations together in ONE persistent launch, or each handle's own
- * batched path one after another, whichever the first call for a set of
- * swarm sizes timed faster (FB_MULTI_PERSISTENT=1/0, read at
- * fb_multi_create, pins it); covariance builds and likelihood reductions
- * stay per handle.  Results are bit-identical to fb_swarm_loglik on each
- * handle.
+ * with its own N, kernel and likelihood -- are evaluated by one call, on
+ * one of three schedules: each handle's own batched path one after another
+ * (0), all factorisations in ONE persistent launch (1), or the chained
+ * lock-step schedule with every handle's matrices in each of its launches
+ * (panel p - (P - P_g) of handle g in launch p, so the handles' last panels
+ * share the last launches), the whole evaluation replayed as one hipGraph
+ * (2).  The first call for a set of swarm sizes times the schedules and
+ * keeps the fastest (FB_MULTI_SCHEDULE=0/1/2, or FB_MULTI_PERSISTENT=1/0,
+ * read at fb_multi_create, pins one).  Covariance builds and likelihood
+ * reductions stay per handle.  Results are bit-identical to
+ * fb_swarm_loglik on each handle.
  * No reference counterpart: the reference trains its GPs one after another
  * (main.f90 loops over atoms and properties; its kernel for one GP is the
  * path of fb_swarm_loglik).  The handles are borrowed (not owned, not to be
@@ -201,6 +205,10 @@ int fb_multi_create(fb_multi **m, int n_gps, fb_handle *const *gps);
 int fb_multi_swarm_loglik(fb_multi *m, const int *s, const double *const *theta, double *const *ll,
                           int *const *info);
 int fb_multi_destroy(fb_multi *m);
+/* The schedule (0, 1, 2 as above) fb_multi_swarm_loglik uses for the swarm
+ * sizes s[0..n_gps), or -1 before a call with those sizes has chosen one.
+ * No reference counterpart (introspection of the schedule choice). */
+int fb_multi_schedule(const fb_multi *m, const int *s);
 
 /* Particle swarm on the device: positions, velocities, personal and global
  * bests stay in device memory, and one PSO iteration -- selection of the

@@ -1,11 +1,13 @@
 """Several training sets of different N evaluated by one call
-(fb_multi_swarm_loglik, the batched uneven case), all factorised in one
-persistent launch (FB_MULTI_PERSISTENT=1), by each handle's own batched path
-one after another (=0), or by whichever of the two the call's first
-evaluation timed faster (unset): every particle's ll and INFO must be
-bit-identical to its own handle's fb_swarm_loglik (same tile arithmetic per
-matrix), with handles of different kernels, likelihoods and swarm sizes
-mixed, and a skipped (s = 0) handle left untouched."""
+(fb_multi_swarm_loglik, the batched uneven case), by each handle's own
+batched path one after another (FB_MULTI_SCHEDULE=0), all factorised in one
+persistent launch (=1), by the chained lock-step schedule with every
+handle's matrices in each launch, replayed as one hipGraph (=2), or by
+whichever of them the call's first evaluation timed fastest (unset): every
+particle's ll and INFO must be bit-identical to its own handle's
+fb_swarm_loglik (same tile arithmetic per matrix), with handles of different
+kernels, likelihoods and swarm sizes mixed, and a skipped (s = 0) handle
+left untouched."""
 import time
 
 import numpy as np
@@ -13,6 +15,7 @@ import pytest
 
 import oracle
 from ferebus_amd import KernelConfig, MultiSwarm, SwarmLikelihood, compile_kernel, fit_mean, synthetic
+from test_gpu_parity import RTOL, _check, _lapack, _oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -26,45 +29,118 @@ def _gp(n, d, seed, kernel="k1+k2", kernels=None, likelihood="marginal", mean="c
     return gp, x, y, prog, rng
 
 
-@pytest.mark.parametrize("schedule", ["1", "0", None])
-def test_multi_matches_each_handle(gpu, monkeypatch, schedule):
+def _schedule(monkeypatch, schedule, tile=None):
+    monkeypatch.delenv("FB_MULTI_PERSISTENT", raising=False)
     if schedule is None:
-        monkeypatch.delenv("FB_MULTI_PERSISTENT", raising=False)
+        monkeypatch.delenv("FB_MULTI_SCHEDULE", raising=False)
     else:
-        monkeypatch.setenv("FB_MULTI_PERSISTENT", schedule)
-    specs = [dict(n=150, d=6, seed=81), dict(n=333, d=6, seed=82, likelihood="concentrated"),
-             dict(n=520, d=9, seed=83, kernel="k1*k2"), dict(n=64, d=6, seed=84, mean="linear")]
+        monkeypatch.setenv("FB_MULTI_SCHEDULE", schedule)
+    if tile is None:
+        monkeypatch.delenv("FB_MULTI_TILE", raising=False)
+    else:
+        monkeypatch.setenv("FB_MULTI_TILE", str(tile))
+
+
+SPECS = [dict(n=150, d=6, seed=81), dict(n=333, d=6, seed=82, likelihood="concentrated"),
+         dict(n=520, d=9, seed=83, kernel="k1*k2"), dict(n=64, d=6, seed=84, mean="linear")]
+
+
+def _mixed(sizes=(5, 3, 7, 2)):
     gps, thetas = [], []
-    for sp, S in zip(specs, (5, 3, 7, 2)):
+    for sp, S in zip(SPECS, sizes):
         gp, x, y, prog, rng = _gp(**sp)
         gps.append(gp)
         thetas.append(rng.uniform(0.0, 1.0, (S, gp.nparams)))
-    multi = MultiSwarm(gps)
-    lls, infos = multi.swarm_loglik(thetas)
+    return gps, thetas
+
+
+def _same_as_handles(gps, thetas, lls, infos):
     for g, (gp, th) in enumerate(zip(gps, thetas)):
+        if th is None:
+            assert len(lls[g]) == 0
+            continue
         ll, info = gp.swarm_loglik(th)
         print(f"gp {g}: n={gp.n} info {infos[g].tolist()}")
-        assert np.array_equal(info, infos[g]) and np.array_equal(ll, lls[g])
-    # one training set skipped, the others re-evaluated with new particles
+        assert np.array_equal(info, infos[g]) and np.array_equal(ll, lls[g]), f"gp {g}"
+
+
+@pytest.mark.parametrize("schedule", ["1", "0", "2", None])
+def test_multi_matches_each_handle(gpu, monkeypatch, schedule):
+    _schedule(monkeypatch, schedule)
+    gps, thetas = _mixed()
+    multi = MultiSwarm(gps)
+    lls, infos = multi.swarm_loglik(thetas)
+    chosen = multi.schedule()
+    print("schedule:", chosen)
+    assert chosen == (MultiSwarm.SCHEDULES[int(schedule)] if schedule is not None else chosen) and chosen
+    _same_as_handles(gps, thetas, lls, infos)
+    # the same counts with new particles (the merged schedule replays its graph)
     thetas2 = [t[::-1].copy() for t in thetas]
-    thetas2[1] = None
     lls2, infos2 = multi.swarm_loglik(thetas2)
-    assert len(lls2[1]) == 0
-    for g in (0, 2, 3):
-        ll, info = gps[g].swarm_loglik(thetas2[g])
-        assert np.array_equal(info, infos2[g]) and np.array_equal(ll, lls2[g])
+    _same_as_handles(gps, thetas2, lls2, infos2)
+    # one training set skipped, the others re-evaluated
+    thetas3 = [t[1:].copy() for t in thetas]
+    thetas3[1] = None
+    lls3, infos3 = multi.swarm_loglik(thetas3)
+    _same_as_handles(gps, thetas3, lls3, infos3)
     # and against the oracle for the largest one
-    gp, x, y, prog, _ = _gp(**specs[2])
+    gp, x, y, prog, _ = _gp(**SPECS[2])
     spec = oracle.Spec.from_program(prog)
     oll, oinfo = oracle.swarm_loglik(x, y, fit_mean("constant", x, y).value, spec, thetas[2][:2], noise=1e-6)
     np.testing.assert_allclose(lls[2][:2], oll, rtol=1e-9)
     multi.close()
 
 
-def test_multi_glycine_uneven_full_size(gpu, monkeypatch):
+@pytest.mark.parametrize("tile", [1, 2, 4, 8, 12, 16, 20])
+def test_merged_tile_codes(gpu, monkeypatch, tile):
+    """The merged schedule with each panel tile code (row-tile shapes, two
+    role work-groups, fused row solves, 8-deep three-per-CU): the bits of
+    each handle's own evaluation."""
+    _schedule(monkeypatch, "2", tile)
+    gps, thetas = _mixed()
+    multi = MultiSwarm(gps)
+    lls, infos = multi.swarm_loglik(thetas)
+    _same_as_handles(gps, thetas, lls, infos)
+    multi.close()
+
+
+def test_merged_more_segments_than_one_launch(gpu, monkeypatch):
+    """19 training sets: more than a launch's 16 segments, so each panel
+    step takes two launches; the bits of each handle's own evaluation."""
+    _schedule(monkeypatch, "2")
+    gps, thetas = [], []
+    for g in range(19):
+        gp, x, y, prog, rng = _gp(n=60 + 23 * g, d=6, seed=200 + g, S=3)
+        gps.append(gp)
+        thetas.append(rng.uniform(0.0, 1.0, (1 + g % 3, gp.nparams)))
+    multi = MultiSwarm(gps)
+    lls, infos = multi.swarm_loglik(thetas)
+    _same_as_handles(gps, thetas, lls, infos)
+    multi.close()
+
+
+def test_merged_wait_bound_reported(gpu, monkeypatch):
+    """A hand-over wait of the merged launches that reaches its bound ends
+    the kernel normally and the call returns an error (forced from the host
+    with a bound of 0 ticks)."""
+    _schedule(monkeypatch, "2")
+    monkeypatch.setenv("FB_CHAIN_WAIT_TICKS", "0")
+    gps, thetas = _mixed()
+    multi = MultiSwarm(gps)
+    with pytest.raises(RuntimeError, match="bound"):
+        multi.swarm_loglik(thetas)
+    multi.close()
+    for gp in gps:
+        gp.close()
+
+
+@pytest.mark.parametrize("schedule", ["2", None])
+def test_multi_glycine_uneven_full_size(gpu, monkeypatch, schedule):
     """Glycine-like: 10 GPs, d=24, N ~ U[1000, 4000], 40 particles each --
-    400 Cholesky factorisations of mixed size per PSO iteration in one launch."""
-    monkeypatch.setenv("FB_MULTI_PERSISTENT", "1")
+    400 Cholesky factorisations of mixed size per PSO iteration in one call;
+    every GP bit-identical to its own evaluation, the smallest checked
+    against the oracle."""
+    _schedule(monkeypatch, schedule)
     ws = synthetic.glycine_gps(seed=2)
     gps, thetas = [], []
     rng = np.random.default_rng(9)
@@ -87,6 +163,16 @@ def test_multi_glycine_uneven_full_size(gpu, monkeypatch):
         ll, info = gp.swarm_loglik(th)
         t_seq += time.perf_counter() - t0
         assert np.array_equal(info, infos[g]) and np.array_equal(ll, lls[g]), f"gp {g}"
-    print(f"400 factorisations: one launch {t_multi * 1e3:.1f} ms, one launch per GP {t_seq * 1e3:.1f} ms, "
+    print(f"400 factorisations: one call {t_multi * 1e3:.1f} ms, one call per GP {t_seq * 1e3:.1f} ms, "
           f"failed {sum(int((i != 0).sum()) for i in infos)}")
+    # the smallest GP against the oracle (1e-10 nugget: the tolerance takes
+    # in the oracle's own gap to LAPACK on the same K)
+    g = int(np.argmin([gp.n for gp in gps]))
+    w = ws[g]
+    prog = compile_kernel(w.kernel, w.kernels, w.x.shape[1])
+    m = fit_mean(w.mean, w.x, w.y).value
+    th = thetas[g][:2]
+    oll, oinfo = _oracle(w.x, w.y, m, prog, th, w.noise, False, w.likelihood)
+    _check(lls[g][:2], infos[g][:2], oll, oinfo, RTOL, f"glycine gp {g} n={w.x.shape[0]}",
+           _lapack(w.x, w.y, m, prog, th, w.noise, False, w.likelihood))
     multi.close()
