@@ -10,6 +10,7 @@
 // swarm size.  The batched path replays a captured hipGraph of that pipeline;
 // the plain path launches it eagerly, one particle at a time.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
 #include <cstdio>
@@ -1272,8 +1273,12 @@ struct fb_multi {
     // pins it
     std::map<std::vector<int>, int> schedule;
     std::map<std::vector<int>, std::pair<hipGraph_t, hipGraphExec_t>> graphs;   // merged schedule, per counts
+    // fb_multi_pso_step: one captured iteration per schedule, set of swarms
+    // (their ids) and the handles' graph epochs
+    std::map<std::vector<long long>, std::pair<hipGraph_t, hipGraphExec_t>> pso_graphs;
     int forced = -1;
-    int merged_tile = 0;   // panel tile code of the merged launches (FB_MULTI_TILE)
+    int merged_tile = fb::kTileFused;   // panel tile code of the merged launches (FB_MULTI_TILE)
+    int merged_tail = 3;            // panels pre-updated at the end of each matrix (FB_MULTI_TAIL)
 };
 
 namespace {
@@ -1287,15 +1292,19 @@ void free_multi(fb_multi *m)
         (void)hipGraphExecDestroy(g.second.second);
         (void)hipGraphDestroy(g.second.first);
     }
+    for (auto &g : m->pso_graphs) {
+        (void)hipGraphExecDestroy(g.second.second);
+        (void)hipGraphDestroy(g.second.first);
+    }
     if (m->h_err) (void)hipHostFree(m->h_err);
 }
 
-// Buffers of a handle's swarm in the merged schedule: the chained lock-step
-// variant (16-deep chunks, the chain words reset by the build), no
-// pre-updates of later panels.
-fb::SwarmBuffers merged_buffers(fb_handle *h, int s)
+// Buffers of a handle's swarm in the merged schedule at theta (device): the
+// chained lock-step variant (16-deep chunks, the chain words reset by the
+// build; the pre-updates of the last panels are set per launch segment).
+fb::SwarmBuffers merged_buffers(fb_handle *h, int s, const double *theta)
 {
-    fb::SwarmBuffers b = buffers(h, 0, s, h->d_theta, h->d_ll, h->d_info);
+    fb::SwarmBuffers b = buffers(h, 0, s, theta, h->d_ll, h->d_info);
     b.gemm_kc = 16;
     b.tile = 0;
     b.panel_tile = nullptr;
@@ -1310,14 +1319,14 @@ fb::SwarmBuffers merged_buffers(fb_handle *h, int s)
 // columns, P the most), so that the last, smallest panels of all the
 // handles share the last launches, where one swarm alone fills the chip
 // least; then every handle's likelihood.
-void enqueue_merged(fb_multi *m, const std::vector<int> &sv, hipStream_t st)
+void enqueue_merged(fb_multi *m, const std::vector<int> &sv, const double *const *theta, hipStream_t st)
 {
     const int G = (int)m->gps.size();
     std::vector<fb::SwarmBuffers> bs(G);
     int P = 0;
     for (int g = 0; g < G; ++g) {
         if (sv[g] == 0) continue;
-        bs[g] = merged_buffers(m->gps[g], sv[g]);
+        bs[g] = merged_buffers(m->gps[g], sv[g], theta[g]);
         fb::launch_build(bs[g], st);
         P = std::max(P, bs[g].np / fb::kNBO);
     }
@@ -1333,6 +1342,7 @@ void enqueue_merged(fb_multi *m, const std::vector<int> &sv, hipStream_t st)
             fb::ChainSegment &sg = cm.seg[cm.G++];
             sg.K = b.K; sg.info = b.info; sg.logdet_part = b.logdet_part; sg.dinv = b.dinv; sg.chain = b.chain;
             sg.np = b.np; sg.n = b.n; sg.S = b.S; sg.c0 = q * fb::kNBO;
+            sg.tail_set = m->merged_tail;
             cm.wait_limit = std::min(cm.wait_limit, b.wait_limit);
             if (cm.G == fb::kMultiMaxSegments) {
                 fb::launch_chain_multi(cm, st);
@@ -1421,6 +1431,7 @@ int fb_multi_create(fb_multi **out, int n_gps, fb_handle *const *gps)
         if (v >= 0 && v < kMultiSchedules) m->forced = v;
     }
     if (const char *env = std::getenv("FB_MULTI_TILE")) m->merged_tile = std::atoi(env);
+    if (const char *env = std::getenv("FB_MULTI_TAIL")) m->merged_tail = std::max(0, std::atoi(env));
     *out = m;
     return FB_OK;
 }
@@ -1440,6 +1451,141 @@ int fb_multi_destroy(fb_multi *m)
     delete m;
     return FB_OK;
 }
+
+namespace {
+
+// One evaluation of every swarm (s[g] particles at theta[g], device) on
+// schedule `mode`, enqueued on st; ll / INFO land in each handle's buffers.
+// The sequential schedule runs each handle's tuned batched path for its
+// swarm size, the persistent one needs the plan of multi_plan.
+void multi_enqueue(fb_multi *m, const std::vector<int> &sv, const double *const *theta, int mode, hipStream_t st)
+{
+    const int G = (int)m->gps.size();
+    if (mode == kMultiMerged) {
+        enqueue_merged(m, sv, theta, st);
+        return;
+    }
+    const bool persistent = mode == kMultiPersistent;
+    for (int g = 0; g < G; ++g) {
+        if (sv[g] == 0) continue;
+        fb_handle *h = m->gps[g];
+        if (!persistent) {
+            auto it = h->tuned.find(sv[g]);
+            set_mode(h, it != h->tuned.end() && it->second != 0 ? it->second : 2, sv[g]);
+        }
+        const fb::SwarmBuffers b = buffers(h, 0, sv[g], theta[g], h->d_ll, h->d_info);
+        if (persistent) fb::launch_build(b, st);
+        else enqueue_group(h, b, st);
+    }
+    if (!persistent) return;
+    if (m->plans.at(sv).second)
+        fb::launch_dag(m->plans.at(sv).first, m->plans.at(sv).second, m->w, m->dag_grid, st);
+    for (int g = 0; g < G; ++g)
+        if (sv[g] > 0) fb::launch_likelihood(buffers(m->gps[g], 0, sv[g], theta[g], m->gps[g]->d_ll,
+                                                     m->gps[g]->d_info), st);
+}
+
+// Evaluates the swarms staged in the handles' theta buffers on schedule
+// `mode` (the merged one as a captured graph, one per set of swarm sizes)
+// and reads ll / INFO back into the handles' pinned buffers; *m->h_err: the
+// persistent launch's error word.
+int multi_run(fb_multi *m, const std::vector<int> &sv, int mode)
+{
+    const int G = (int)m->gps.size();
+    hipStream_t st = m->gps[0]->stream;
+    std::vector<const double *> th(G);
+    for (int g = 0; g < G; ++g) th[g] = m->gps[g]->d_theta;
+    if (mode == kMultiMerged) {
+        auto it = m->graphs.find(sv);
+        if (it == m->graphs.end()) {
+            hipGraph_t gr = nullptr;
+            hipGraphExec_t ge = nullptr;
+            FB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            enqueue_merged(m, sv, th.data(), st);
+            FB_HIP(hipStreamEndCapture(st, &gr));
+            if (hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0) != hipSuccess) {
+                (void)hipGraphDestroy(gr);
+                return fail(FB_E_HIP, "fb_multi: graph instantiation failed");
+            }
+            it = m->graphs.emplace(sv, std::make_pair(gr, ge)).first;
+        }
+        FB_HIP(hipGraphLaunch(it->second.second, st));
+    } else {
+        multi_enqueue(m, sv, th.data(), mode, st);
+    }
+    for (int g = 0; g < G; ++g) {
+        if (sv[g] == 0) continue;
+        fb_handle *h = m->gps[g];
+        FB_HIP(hipMemcpyAsync(h->h_ll, h->d_ll, sizeof(double) * sv[g], hipMemcpyDeviceToHost, st));
+        FB_HIP(hipMemcpyAsync(h->h_info, h->d_info, sizeof(int) * sv[g], hipMemcpyDeviceToHost, st));
+    }
+    *m->h_err = 0;
+    if (mode == kMultiPersistent)
+        FB_HIP(hipMemcpyAsync(m->h_err, m->w.ctl + fb::kDagErrorWord, sizeof(int), hipMemcpyDeviceToHost, st));
+    FB_HIP(hipStreamSynchronize(st));
+    FB_HIP(hipGetLastError());
+    return FB_OK;
+}
+
+// The plan of the persistent schedule for these swarm sizes (FB_E_CAPACITY
+// when it does not fit one launch).
+int multi_plan_for(fb_multi *m, const std::vector<int> &sv)
+{
+    return m->plans.count(sv) ? FB_OK : multi_plan(m, sv);
+}
+
+// Chooses the schedule for the swarm sizes sv (the pinned one, or the
+// fastest of two timed evaluations each on the swarms staged in the
+// handles' theta buffers, after tuning every handle for its size).
+int multi_choose(fb_multi *m, const std::vector<int> &sv)
+{
+    const int G = (int)m->gps.size();
+    const bool merged = merged_possible(m, sv);
+    if (m->forced >= 0) {
+        if (m->forced == kMultiPersistent) {
+            if (int rc = multi_plan_for(m, sv)) return rc;
+        }
+        if (m->forced == kMultiMerged && !merged)
+            return fail(FB_E_ARG, "fb_multi: the merged schedule needs the chained schedule (FB_CHAIN=0 is set)");
+        m->schedule[sv] = m->forced;
+        return FB_OK;
+    }
+    hipStream_t st = m->gps[0]->stream;
+    FB_HIP(hipStreamSynchronize(st));
+    for (int g = 0; g < G; ++g)
+        if (sv[g] > 0) {
+            if (int rc = choose_path(m->gps[g], sv[g])) return rc;
+        }
+    // the persistent schedule is a candidate only where its plan fits
+    const bool can[kMultiSchedules] = {true, multi_plan_for(m, sv) == FB_OK, merged};
+    hipEvent_t e0, e1;
+    FB_HIP(hipEventCreate(&e0));
+    FB_HIP(hipEventCreate(&e1));
+    float best[kMultiSchedules] = {1e30f, 1e30f, 1e30f};
+    int rc = FB_OK;
+    for (int rep = 0; rep < 2 && rc == FB_OK; ++rep)
+        for (int p = 0; p < kMultiSchedules; ++p) {
+            if (!can[p]) continue;
+            (void)hipEventRecord(e0, st);
+            rc = multi_run(m, sv, p);
+            (void)hipEventRecord(e1, st);
+            if (rc || hipEventSynchronize(e1) != hipSuccess) break;
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (p == kMultiPersistent && *m->h_err != 0) ms = 1e30f;   // could not be served
+            best[p] = std::min(best[p], ms);
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    int pick = kMultiSequential;
+    for (int p = 1; p < kMultiSchedules; ++p)
+        if (best[p] < best[pick]) pick = p;
+    m->schedule[sv] = pick;
+    return FB_OK;
+}
+
+}  // namespace
 
 int fb_multi_swarm_loglik(fb_multi *m, const int *s, const double *const *theta, double *const *ll,
                           int *const *info)
@@ -1463,124 +1609,22 @@ int fb_multi_swarm_loglik(fb_multi *m, const int *s, const double *const *theta,
         FB_HIP(hipMemcpyAsync(h->d_theta, h->h_theta, sizeof(double) * P * sv[g], hipMemcpyHostToDevice, st));
     }
     int rc = FB_OK;
-    // schedules this call may use: the persistent one only where its plan
-    // fits one launch, the merged one only with every handle chained
-    bool can[kMultiSchedules] = {true, false, merged_possible(m, sv)};
-    auto plan = [&]() -> int {
-        if (!m->plans.count(sv)) {
-            const int r = multi_plan(m, sv);
-            if (r) return r;
-        }
-        can[kMultiPersistent] = true;
-        return FB_OK;
-    };
-    auto run = [&](int mode) -> int {
-        if (mode == kMultiMerged) {
-            auto it = m->graphs.find(sv);
-            if (it == m->graphs.end()) {
-                hipGraph_t gr = nullptr;
-                hipGraphExec_t ge = nullptr;
-                FB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-                enqueue_merged(m, sv, st);
-                FB_HIP(hipStreamEndCapture(st, &gr));
-                if (hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0) != hipSuccess) {
-                    (void)hipGraphDestroy(gr);
-                    return fail(FB_E_HIP, "fb_multi_swarm_loglik: graph instantiation failed");
-                }
-                it = m->graphs.emplace(sv, std::make_pair(gr, ge)).first;
-            }
-            FB_HIP(hipGraphLaunch(it->second.second, st));
-        } else {
-            const bool persistent = mode == kMultiPersistent;
-            for (int g = 0; g < G; ++g) {
-                if (sv[g] == 0) continue;
-                fb_handle *h = m->gps[g];
-                if (!persistent) {
-                    // the handle's own tuned lock-step variant for this swarm size
-                    auto it = h->tuned.find(sv[g]);
-                    set_mode(h, it != h->tuned.end() && it->second != 0 ? it->second : 2, sv[g]);
-                }
-                const fb::SwarmBuffers b = buffers(h, 0, sv[g], h->d_theta, h->d_ll, h->d_info);
-                if (persistent) fb::launch_build(b, st);
-                else enqueue_group(h, b, st);
-            }
-            if (persistent && m->plans.at(sv).second)
-                fb::launch_dag(m->plans.at(sv).first, m->plans.at(sv).second, m->w, m->dag_grid, st);
-            if (persistent)
-                for (int g = 0; g < G; ++g)
-                    if (sv[g] > 0)
-                        fb::launch_likelihood(buffers(m->gps[g], 0, sv[g], m->gps[g]->d_theta, m->gps[g]->d_ll,
-                                                      m->gps[g]->d_info), st);
-        }
-        for (int g = 0; g < G; ++g) {
-            if (sv[g] == 0) continue;
-            fb_handle *h = m->gps[g];
-            FB_HIP(hipMemcpyAsync(h->h_ll, h->d_ll, sizeof(double) * sv[g], hipMemcpyDeviceToHost, st));
-            FB_HIP(hipMemcpyAsync(h->h_info, h->d_info, sizeof(int) * sv[g], hipMemcpyDeviceToHost, st));
-        }
-        *m->h_err = 0;
-        if (mode == kMultiPersistent)
-            FB_HIP(hipMemcpyAsync(m->h_err, m->w.ctl + fb::kDagErrorWord, sizeof(int), hipMemcpyDeviceToHost, st));
-        FB_HIP(hipStreamSynchronize(st));
-        FB_HIP(hipGetLastError());
-        return FB_OK;
-    };
     if (!m->schedule.count(sv)) {
-        if (m->forced >= 0) {
-            if (m->forced == kMultiPersistent) {
-                rc = plan();
-                if (rc) return rc;
-            }
-            if (m->forced == kMultiMerged && !can[kMultiMerged])
-                return fail(FB_E_ARG, "fb_multi_swarm_loglik: the merged schedule needs the chained schedule "
-                                      "(FB_CHAIN=0 is set)");
-            m->schedule[sv] = m->forced;
-        } else {
-            // tune every handle for its swarm size, then time the schedules
-            FB_HIP(hipStreamSynchronize(st));
-            for (int g = 0; g < G; ++g)
-                if (sv[g] > 0) {
-                    rc = choose_path(m->gps[g], sv[g]);
-                    if (rc) return rc;
-                }
-            (void)plan();   // no persistent candidate when its plan does not fit
-            hipEvent_t e0, e1;
-            FB_HIP(hipEventCreate(&e0));
-            FB_HIP(hipEventCreate(&e1));
-            float best[kMultiSchedules] = {1e30f, 1e30f, 1e30f};
-            for (int rep = 0; rep < 2 && rc == FB_OK; ++rep)
-                for (int p = 0; p < kMultiSchedules; ++p) {
-                    if (!can[p]) continue;
-                    (void)hipEventRecord(e0, st);
-                    rc = run(p);
-                    (void)hipEventRecord(e1, st);
-                    if (rc || hipEventSynchronize(e1) != hipSuccess) break;
-                    float ms = 0.f;
-                    (void)hipEventElapsedTime(&ms, e0, e1);
-                    if (p == kMultiPersistent && *m->h_err != 0) ms = 1e30f;   // could not be served
-                    best[p] = std::min(best[p], ms);
-                }
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
-            if (rc) return rc;
-            int pick = kMultiSequential;
-            for (int p = 1; p < kMultiSchedules; ++p)
-                if (best[p] < best[pick]) pick = p;
-            m->schedule[sv] = pick;
-        }
+        rc = multi_choose(m, sv);
+        if (rc) return rc;
     }
     const int mode = m->schedule.at(sv);
     if (mode == kMultiPersistent) {
-        rc = plan();
+        rc = multi_plan_for(m, sv);
         if (rc) return rc;
     }
-    rc = run(mode);
+    rc = multi_run(m, sv, mode);
     if (rc) return rc;
     if (mode == kMultiPersistent && *m->h_err == 1)
         return fail(FB_E_HIP, "fb_multi_swarm_loglik: the persistent factorisation found no ready task for 2 s");
     if (mode == kMultiPersistent && *m->h_err == 2) {
         m->schedule[sv] = kMultiSequential;
-        rc = run(kMultiSequential);   // some XCD queue was not served: lock-step path
+        rc = multi_run(m, sv, kMultiSequential);   // some XCD queue was not served: lock-step path
         if (rc) return rc;
     }
     for (int g = 0; g < G; ++g)
@@ -1601,6 +1645,7 @@ int fb_multi_swarm_loglik(fb_multi *m, const int *s, const double *const *theta,
 
 struct fb_pso {
     fb_handle *h = nullptr;
+    long long id = 0;                                 // creation number (keys of captured multi steps)
     fb::PsoState st;
     double coef[3] = {0.0, 0.0, 0.0};                 // w, c1, c2 of the next step
     double *d_r = nullptr, *h_r = nullptr;            // r1 | r2 | w c1 c2, device and pinned staging
@@ -1705,8 +1750,10 @@ int fb_pso_create(fb_pso **out, fb_handle *h, int s, double w, double c1, double
     if (s < 1 || s > h->max_swarm) return fail(FB_E_CAPACITY, "fb_pso_create: swarm size out of range");
     const int P = h->prog.n_par;
     if (P < 1) return fail(FB_E_ARG, "fb_pso_create: the model has no parameters to optimise");
+    static std::atomic<long long> next_id{1};
     fb_pso *p = new fb_pso;
     p->h = h;
+    p->id = next_id++;
     p->st.S = s; p->st.P = P;
     p->coef[0] = w; p->coef[1] = c1; p->coef[2] = c2;
     const size_t SP = (size_t)s * P;
@@ -1914,6 +1961,95 @@ int fb_pso_positions(fb_pso *p, double *x, double *v)
     FB_HIP(hipStreamSynchronize(st));
     if (x) std::memcpy(x, p->h_xs, sizeof(double) * SP);
     if (v) std::memcpy(v, p->h_xs + SP, sizeof(double) * SP);
+    return FB_OK;
+}
+
+int fb_multi_pso_step(fb_multi *m, fb_pso *const *ps, const double *const *r1, const double *const *r2,
+                      double *const *cost, int *const *info)
+{
+    if (!m || !ps || !r1 || !r2) return fail(FB_E_ARG, "fb_multi_pso_step: null argument");
+    const int G = (int)m->gps.size();
+    std::vector<int> sv(G, 0);
+    for (int g = 0; g < G; ++g) {
+        if (!ps[g]) continue;
+        if (ps[g]->h != m->gps[g])
+            return fail(FB_E_ARG, "fb_multi_pso_step: swarm " + std::to_string(g) + " is not on handle " +
+                                      std::to_string(g) + " of the set");
+        if (!r1[g] || !r2[g]) return fail(FB_E_ARG, "fb_multi_pso_step: null argument");
+        sv[g] = ps[g]->st.S;
+    }
+    if (std::all_of(sv.begin(), sv.end(), [](int v) { return v == 0; })) return FB_OK;
+    hipStream_t st = m->gps[0]->stream;
+    int rc = FB_OK;
+    if (!m->schedule.count(sv)) {
+        // chosen on the swarms' current positions, staged where the
+        // evaluation call stages its particles
+        for (int g = 0; g < G; ++g)
+            if (sv[g] > 0)
+                FB_HIP(hipMemcpyAsync(m->gps[g]->d_theta, ps[g]->st.x, sizeof(double) * sv[g] * ps[g]->st.P,
+                                      hipMemcpyDeviceToDevice, st));
+        rc = multi_choose(m, sv);
+        if (rc) return rc;
+    }
+    // the persistent schedule reports its errors after the fact: steps use
+    // the merged (or sequential) schedule instead
+    int mode = m->schedule.at(sv);
+    if (mode == kMultiPersistent) mode = merged_possible(m, sv) ? kMultiMerged : kMultiSequential;
+    std::vector<long long> key{mode};
+    std::vector<const double *> th(G, nullptr);
+    for (int g = 0; g < G; ++g) {
+        key.push_back(ps[g] ? ps[g]->id : 0);
+        key.push_back(m->gps[g]->graph_epoch);
+        if (!ps[g]) continue;
+        fb_pso *p = ps[g];
+        const size_t SP = (size_t)p->st.S * p->st.P;
+        std::memcpy(p->h_r, r1[g], sizeof(double) * SP);
+        std::memcpy(p->h_r + SP, r2[g], sizeof(double) * SP);
+        std::memcpy(p->h_r + 2 * SP, p->coef, sizeof(double) * 3);
+        th[g] = p->st.x;
+    }
+    auto enqueue = [&](hipStream_t s) {
+        for (int g = 0; g < G; ++g) {
+            if (!ps[g]) continue;
+            fb_pso *p = ps[g];
+            const size_t SP = (size_t)p->st.S * p->st.P;
+            (void)hipMemcpyAsync(p->d_r, p->h_r, sizeof(double) * (2 * SP + 3), hipMemcpyHostToDevice, s);
+            fb::launch_pso_select(p->st, s);
+            fb::launch_pso_update(p->st, p->d_r, p->d_r + SP, p->d_r + 2 * SP, s);
+        }
+        multi_enqueue(m, sv, th.data(), mode, s);
+        for (int g = 0; g < G; ++g)
+            if (ps[g]) fb::launch_pso_cost(ps[g]->st, m->gps[g]->d_ll, m->gps[g]->d_info, s);
+    };
+    auto it = m->pso_graphs.find(key);
+    if (it == m->pso_graphs.end()) {
+        hipGraph_t gr = nullptr;
+        hipGraphExec_t ge = nullptr;
+        FB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        enqueue(st);
+        FB_HIP(hipStreamEndCapture(st, &gr));
+        if (hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0) != hipSuccess) {
+            (void)hipGraphDestroy(gr);
+            return fail(FB_E_HIP, "fb_multi_pso_step: graph instantiation failed");
+        }
+        it = m->pso_graphs.emplace(key, std::make_pair(gr, ge)).first;
+    }
+    FB_HIP(hipGraphLaunch(it->second.second, st));
+    for (int g = 0; g < G; ++g) {
+        if (!ps[g]) continue;
+        fb_pso *p = ps[g];
+        FB_HIP(hipMemcpyAsync(p->h_cost, p->st.cost, sizeof(double) * sv[g], hipMemcpyDeviceToHost, st));
+        FB_HIP(hipMemcpyAsync(p->h_info, p->st.info, sizeof(int) * sv[g], hipMemcpyDeviceToHost, st));
+    }
+    FB_HIP(hipStreamSynchronize(st));
+    FB_HIP(hipGetLastError());
+    for (int g = 0; g < G; ++g) {
+        if (!ps[g]) continue;
+        rc = check_waits(ps[g]->h_info, sv[g], "fb_multi_pso_step");
+        if (rc) return rc;
+        if (cost && cost[g]) std::memcpy(cost[g], ps[g]->h_cost, sizeof(double) * sv[g]);
+        if (info && info[g]) std::memcpy(info[g], ps[g]->h_info, sizeof(int) * sv[g]);
+    }
     return FB_OK;
 }
 

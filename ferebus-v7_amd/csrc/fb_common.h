@@ -243,7 +243,9 @@ struct ChainSegment {
     double *dinv;          // S x (np / kNBI) x kDinv
     int *chain;            // S x kChainStride, reset by the swarm's build
     int np, n, S, c0;
-    int role_end, tile_end, solve_end;   // exclusive ends of the segment's block ids
+    int tail_set;          // the last tail_set panels are pre-updated in the launch of the one before
+    int k0, kt, tail;      // pre-update schedule of this launch (set by the launcher)
+    int role_end, tile_end, tail_end, solve_end;   // exclusive ends of the segment's block ids
 };
 struct ChainMulti {
     int G = 0;             // segments

@@ -425,6 +425,31 @@ __device__ __forceinline__ void chain_rowtile(int lt, double *__restrict__ K, in
     }
 }
 
+// Tail pre-update tile lt (< 8 ceil(S tail_tiles / 8), XCD-aware order) of
+// the chained update launch of panel c0: columns kt .. c0 applied to the
+// diagonal 64 x 64 block or a 64 x 128 row tile of one of the next `tail`
+// panels.
+template <int KC>
+__device__ __forceinline__ void chain_tailtile(int lt, double *__restrict__ K, int np, int c0, int kt, int tail,
+                                               int S, const int *__restrict__ info, double *smem)
+{
+    const int ct = tail_tiles(np, c0, tail);
+    const int t = xcd_tile(lt, S * ct);
+    if (t < 0) return;
+    const int s = t / ct;
+    int v = t % ct, cp = c0;
+    for (int j = 1; j <= tail; ++j) {
+        cp = c0 + kNBO * j;
+        const int cnt = 1 + (np - cp - kNBI) / kNBI;
+        if (v < cnt) break;
+        v -= cnt;
+    }
+    if (info[s] != 0) return;
+    double *A = K + (size_t)s * np * np;
+    if (v == 0) gemm_tile_update<kMB, kNBI, KC>(A, np, cp, cp, kt, c0, smem);
+    else gemm_tile_update<kMB, kNBO, KC>(A, np, cp + kNBI * v, cp, kt, c0, smem);
+}
+
 template <int KC, int MB, int NB, bool TWO, bool FUSED>
 __global__ void __launch_bounds__(kThreads)
 __attribute__((amdgpu_waves_per_eu(KC == 8 ? 3 : 2, KC == 8 ? 3 : 2))) panel_chain_kernel(double *__restrict__ K, int np, int n,
@@ -451,22 +476,7 @@ __attribute__((amdgpu_waves_per_eu(KC == 8 ? 3 : 2, KC == 8 ? 3 : 2))) panel_cha
         chain_rowtile<KC, MB, NB, FUSED>(b - nrole, K, np, c0, k0, S, info, dinv, chain, wait_limit, smem, ok);
         return;
     }
-    // tail pre-update tiles
-    const int ct = tail_tiles(np, c0, tail);
-    const int t = xcd_tile(b - nrole - nreg, S * ct);
-    if (t < 0) return;
-    const int s = t / ct;
-    int v = t % ct, cp = c0;
-    for (int j = 1; j <= tail; ++j) {
-        cp = c0 + kNBO * j;
-        const int cnt = 1 + (np - cp - kNBI) / kNBI;
-        if (v < cnt) break;
-        v -= cnt;
-    }
-    if (info[s] != 0) return;
-    double *A = K + (size_t)s * np * np;
-    if (v == 0) gemm_tile_update<kMB, kNBI, KC>(A, np, cp, cp, kt, c0, smem);
-    else gemm_tile_update<kMB, kNBO, KC>(A, np, cp + kNBI * v, cp, kt, c0, smem);
+    chain_tailtile<KC>(b - nrole - nreg, K, np, c0, kt, tail, S, info, smem);
 }
 
 // Row-solve tile lt (< 8 ceil(S rt / 8), XCD-aware order) of panel c0.
@@ -524,14 +534,21 @@ __attribute__((amdgpu_waves_per_eu(KC == 8 ? 3 : 2, KC == 8 ? 3 : 2))) panel_cha
     if (b < m.seg[m.G - 1].role_end) {
         while (b >= m.seg[g].role_end) ++g;
         const ChainSegment &q = m.seg[g];
-        chain_role<KC, TWO, FUSED>(b - (g ? m.seg[g - 1].role_end : 0), q.K, q.np, q.n, q.c0, 0, q.S, q.info,
+        chain_role<KC, TWO, FUSED>(b - (g ? m.seg[g - 1].role_end : 0), q.K, q.np, q.n, q.c0, q.k0, q.S, q.info,
                                    q.logdet_part, q.dinv, q.chain, m.wait_limit, smem, ok);
         return;
     }
-    while (b >= m.seg[g].tile_end) ++g;
+    if (b < m.seg[m.G - 1].tile_end) {
+        while (b >= m.seg[g].tile_end) ++g;
+        const ChainSegment &q = m.seg[g];
+        chain_rowtile<KC, MB, NB, FUSED>(b - (g ? m.seg[g - 1].tile_end : m.seg[m.G - 1].role_end), q.K, q.np,
+                                         q.c0, q.k0, q.S, q.info, q.dinv, q.chain, m.wait_limit, smem, ok);
+        return;
+    }
+    while (b >= m.seg[g].tail_end) ++g;
     const ChainSegment &q = m.seg[g];
-    chain_rowtile<KC, MB, NB, FUSED>(b - (g ? m.seg[g - 1].tile_end : m.seg[m.G - 1].role_end), q.K, q.np, q.c0, 0,
-                                     q.S, q.info, q.dinv, q.chain, m.wait_limit, smem, ok);
+    chain_tailtile<KC>(b - (g ? m.seg[g - 1].tail_end : m.seg[m.G - 1].tile_end), q.K, q.np, q.c0, q.kt, q.tail,
+                       q.S, q.info, smem);
 }
 
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3)))
@@ -567,19 +584,19 @@ static void chain_tiles(const SwarmBuffers &b, int tile, int c0, int k0, int kt,
 // Pre-update schedule of the chained update launches: launch p applies
 // columns k0 .. c0p to its own panel (the first k0 were applied by earlier
 // launches) and columns kt .. c0p to the next `tail` panels.  Launch
-// npan - 1 - b.tail pre-updates the last b.tail panels; with b.lookahead
+// npan - 1 - tail_set pre-updates the last tail_set panels; with lookahead
 // every other launch pre-updates the next panel.  Each panel's columns are
 // applied in increasing k across launches, so the bits do not change.
-static void chain_schedule(const SwarmBuffers &b, int p, int &k0, int &kt, int &tail)
+static void chain_schedule(int np, int tail_set, int lookahead, int p, int &k0, int &kt, int &tail)
 {
-    const int npan = b.np / kNBO, q = npan - 1 - b.tail;
+    const int npan = np / kNBO, q = npan - 1 - tail_set;
     int k[1024];
     const int np_ = npan < 1024 ? npan : 1024;
     for (int r = 0; r < np_; ++r) k[r] = 0;
     for (int r = 0; r <= p && r < np_; ++r) {
         int t = 0;
-        if (b.tail > 0 && q >= 1 && r == q) t = b.tail;
-        else if (b.lookahead && r >= 1 && r + 1 < npan) t = 1;
+        if (tail_set > 0 && q >= 1 && r == q) t = tail_set;
+        else if (lookahead && r >= 1 && r + 1 < npan) t = 1;
         if (r == p) {
             k0 = k[r];
             kt = r + 1 < np_ ? k[r + 1] : 0;
@@ -597,7 +614,7 @@ void launch_panel_chain(const SwarmBuffers &b, int c0, hipStream_t st)
 {
     const int code = b.panel_tile ? b.panel_tile[c0 / kNBO] : b.tile;
     int k0 = 0, kt = 0, tail = 0;
-    chain_schedule(b, c0 / kNBO, k0, kt, tail);
+    chain_schedule(b.np, b.tail, b.lookahead, c0 / kNBO, k0, kt, tail);
     const int tile = (code & kTileFused) ? kTileFused : (code & kTileThreePerCU) ? kTileThreePerCU : (code & 3);
     if (code & kTileTwoRoles) chain_tiles<true>(b, tile, c0, k0, kt, tail, st);
     else chain_tiles<false>(b, tile, c0, k0, kt, tail, st);
@@ -627,9 +644,14 @@ static void chain_multi_launch(ChainMulti &m, hipStream_t st)
     int t = r, s = 0;
     for (int g = 0; g < m.G; ++g) {
         ChainSegment &q = m.seg[g];
+        chain_schedule(q.np, q.tail_set, 0, q.c0 / kNBO, q.k0, q.kt, q.tail);
         const int rows = q.np - q.c0 - kNBO;
         q.tile_end = t += 8 * ceil_div(q.S * (rows / MB) * (kNBO / NB), 8);
         q.solve_end = s += FUSED ? 0 : 8 * ceil_div(q.S * (rows / kNBI), 8);
+    }
+    for (int g = 0; g < m.G; ++g) {
+        ChainSegment &q = m.seg[g];
+        q.tail_end = t += 8 * ceil_div(q.S * tail_tiles(q.np, q.c0, q.tail), 8);
     }
     hipLaunchKernelGGL((panel_chain_multi_kernel<KC, MB, NB, TWO, FUSED>), dim3(t), dim3(kThreads), 0, st, m);
     if (s > 0) hipLaunchKernelGGL(solve_rows_multi_kernel, dim3(s), dim3(kThreads), 0, st, m);

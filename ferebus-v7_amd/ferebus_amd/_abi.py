@@ -24,6 +24,7 @@ EXPORTS = (
     "fb_swarm_loglik_async", "fb_particle_loglik", "fb_get_alpha", "fb_swarm_loglik_grad",
     "fb_profile_enable", "fb_profile_read", "fb_factor_path", "fb_device_name", "fb_last_error",
     "fb_multi_create", "fb_multi_swarm_loglik", "fb_multi_destroy", "fb_multi_schedule",
+    "fb_multi_pso_step",
     "fb_pso_create", "fb_pso_destroy", "fb_pso_init", "fb_pso_step", "fb_pso_retry", "fb_pso_select",
     "fb_pso_set_gbest", "fb_pso_positions", "fb_debug_matrix", "fb_swarm_loglik_static",
     "fb_pso_set_coefficients",
@@ -98,6 +99,8 @@ def load() -> ctypes.CDLL:
                                                  ctypes.POINTER(_P_INT)]),
         "fb_multi_destroy": (ctypes.c_int, [vp]),
         "fb_multi_schedule": (ctypes.c_int, [vp, _P_INT]),
+        "fb_multi_pso_step": (ctypes.c_int, [vp, ctypes.POINTER(vp), ctypes.POINTER(_P_DBL), ctypes.POINTER(_P_DBL),
+                                             ctypes.POINTER(_P_DBL), ctypes.POINTER(_P_INT)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

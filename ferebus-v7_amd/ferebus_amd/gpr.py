@@ -235,6 +235,30 @@ class MultiSwarm:
 
     SCHEDULES = ("sequential", "persistent", "merged")
 
+    def pso_step(self, steps):
+        """One iteration of several device swarms (fb_multi_pso_step).
+        steps: {g: (swarm, r1, r2)} with `swarm` a DeviceSwarm on training
+        set g and r1, r2 of shape (S_g, P_g).  Returns ({g: cost},
+        {g: INFO})."""
+        G = len(self.gps)
+        P_DBL, P_INT = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+        ps = (ctypes.c_void_p * G)()
+        r1p, r2p, cp = (P_DBL * G)(), (P_DBL * G)(), (P_DBL * G)()
+        ip = (P_INT * G)()
+        keep, costs, infos = [], {}, {}
+        for g, (sw, r1, r2) in steps.items():
+            if sw.gp is not self.gps[g]:
+                raise ValueError(f"swarm {g} is not on training set {g}")
+            r1, r2 = sw._rows(r1, sw.S), sw._rows(r2, sw.S)
+            costs[g], infos[g] = np.empty(sw.S), np.empty(sw.S, dtype=np.int32)
+            keep += [r1, r2]
+            ps[g] = sw._p.value
+            r1p[g], r2p[g] = _abi.dptr(r1), _abi.dptr(r2)
+            cp[g], ip[g] = _abi.dptr(costs[g]), _abi.iptr(infos[g])
+        self._last = np.array([steps[g][0].S if g in steps else 0 for g in range(G)], dtype=np.int32)
+        _abi.check(self._lib.fb_multi_pso_step(self._h, ps, r1p, r2p, cp, ip), "fb_multi_pso_step")
+        return costs, infos
+
     def schedule(self, sizes=None):
         """The schedule chosen for the swarm sizes of the last call (or
         `sizes`): "sequential", "persistent", "merged", or None before a call

@@ -29,6 +29,9 @@ src/optimiser/pso/particle.f90, src/optimiser/iter_state.f90):
     b = 24 (1 - w^2) / (7 - 5 w)); unified variants update after every swarm
     update, individual ones after every particle of the forced updater only
     (the global updater never calls them, pso.f90:445-447).
+Several training sets (run_pso_multi): their swarms advance in lock step,
+one fb_multi_pso_step per iteration evaluating all of them together, each
+with its own RNG and host rules, so each gets the run run_pso gives it.
 Optional (lbfgs_every > 0): every lbfgs_every iterations the global best is
 polished by a few L-BFGS steps on the adjoint gradient and replaces gbest
 when its cost is lower.  The RNG is numpy's PCG64 (the reference's Fortran
@@ -41,7 +44,7 @@ import ctypes
 import math
 import time
 from dataclasses import dataclass, field
-from typing import Callable, Dict, List, Optional, Tuple
+from typing import Callable, Dict, List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -339,6 +342,132 @@ def _record(record, **kw):
         record.append({k: (np.array(v, copy=True) if isinstance(v, np.ndarray) else v) for k, v in kw.items()})
 
 
+class _SwarmRun:
+    """The host side of one PSO run around its device swarm (ParticleSwarm,
+    pso.f90:382-472): the initial swarm, the draws of each iteration, the
+    re-randomisation of failed particles (particle.f90:117-128), the
+    running gbest, the parameter updater, polishing and the stopping rules.
+    The iteration's device step itself is left to the caller -- the swarm's
+    own fb_pso_step (run_pso) or its part of fb_multi_pso_step
+    (run_pso_multi) -- so both drivers apply the same rules in the same
+    order and give the same results."""
+
+    def __init__(self, gp, cfg: PsoSettings, trace, target_cost, record):
+        self.gp, self.cfg = gp, cfg
+        self.trace, self.target_cost, self.record = trace, target_cost, record
+        self.S, self.P = cfg.swarm_size, gp.nparams
+        self.rng = np.random.default_rng(cfg.seed)
+        self.lo, self.hi = search_bounds(gp, cfg)
+        self.sw = DeviceSwarm(gp, self.S, cfg.inertia_weight, cfg.cognitive_learning_rate,
+                              cfg.social_learning_rate)
+        self.res = PsoResult(best_position=np.empty(self.P), best_cost=math.inf)
+        self.done = False
+
+    def start(self) -> None:
+        S, P, rng, lo, hi, res = self.S, self.P, self.rng, self.lo, self.hi, self.res
+        self.t0 = time.perf_counter()
+        x = lo + rng.random((S, P)) * (hi - lo)
+        v = (lo - hi) + rng.random((S, P)) * (2.0 * (hi - lo))
+        cost, info = self.sw.init(x, v)
+        res.evaluations += S
+        res.rejections += int(np.sum(info != 0))
+        if self.trace:
+            for k in range(S):
+                self.trace(0, 0, k, int(info[k]), float(cost[k]), x[k])
+        _record(self.record, it=0, kind="init", x=x, v=v, cost=cost, info=info)
+        self.gcost = math.inf
+        for c in cost:                        # state%update in particle order
+            if c <= self.gcost:
+                self.gcost = float(c)
+        res.history.append((0, time.perf_counter() - self.t0, self.gcost))
+        self.upd = ParameterUpdater(self.cfg.parameter_updater, self.cfg, rng, cost)
+        _record(self.record, it=0, kind="coef", coef=list(self.upd.coef), draws=list(self.upd.draws))
+        self.stall_cost, self.stall = math.inf, 0
+        self.it = 1
+
+    def draw(self) -> Tuple[np.ndarray, np.ndarray]:
+        """The coefficients and uniforms of the next iteration's update."""
+        _abi.check(self.sw._lib.fb_pso_set_coefficients(self.sw._p, *self.upd.coef), "fb_pso_set_coefficients")
+        self.r1 = self.rng.random((self.S, self.P))
+        self.r2 = self.rng.random((self.S, self.P))
+        return self.r1, self.r2
+
+    def after_step(self, cost, info) -> None:
+        """Everything after the iteration's evaluation; sets `done`."""
+        cfg, res, sw, it, S, P = self.cfg, self.res, self.sw, self.it, self.S, self.P
+        res.evaluations += S
+        if self.trace or self.record is not None:
+            xs, vs = sw.positions()
+            if self.trace:
+                for k in range(S):
+                    self.trace(it, 0, k, int(info[k]), float(cost[k]), xs[k])
+            _record(self.record, it=it, kind="step", coef=list(self.upd.coef), r1=self.r1, r2=self.r2, x=xs,
+                    v=vs, cost=cost, info=info)
+        todo = np.flatnonzero(info != 0)
+        rnd = 0
+        while len(todo):
+            res.rejections += len(todo)
+            rnd += 1
+            if rnd > cfg.max_retries:
+                raise RuntimeError("PSO: particles stayed non-positive-definite after max_retries "
+                                   "re-randomisations")
+            # particle.f90:117-128
+            xr = self.lo + self.rng.random((len(todo), P)) * (self.hi - self.lo)
+            vr = (self.lo - self.hi) + self.rng.random((len(todo), P)) * (2.0 * (self.hi - self.lo))
+            c, inf = sw.retry(todo, xr, vr)
+            res.evaluations += len(todo)
+            cost[todo] = c
+            if self.trace:
+                for r, k in enumerate(todo):
+                    self.trace(it, rnd, int(k), int(inf[r]), float(c[r]), xr[r])
+            _record(self.record, it=it, kind="retry", idx=todo.copy(), x=xr, v=vr, cost=c, info=inf)
+            todo = todo[inf != 0]
+        for c in cost:
+            if c <= self.gcost:
+                self.gcost = float(c)
+                res.best_iter = it
+        if self.upd.per_swarm:                # next_iter_pso, pso.f90:563-565
+            self.upd.draws = []
+            self.upd.update(cost)
+            _record(self.record, it=it, kind="coef", coef=list(self.upd.coef), draws=list(self.upd.draws))
+        if cfg.lbfgs_every > 0 and it % cfg.lbfgs_every == 0:
+            gdev, gbest, _ = sw.select()
+            xp, fp, ne, na = polish(self.gp, gbest, cfg.lbfgs)
+            res.polish_calls += 1
+            res.polish_evals += ne
+            if fp < gdev:
+                sw.set_gbest(xp, fp)
+                self.gcost = float(fp)
+                res.best_iter = it
+                res.polish_improvements += 1
+            if self.trace:
+                self.trace(it, -1, -1, 0, float(fp), xp)
+        res.iterations = it
+        res.history.append((it, time.perf_counter() - self.t0, self.gcost))
+        if cfg.relative_change:
+            if abs((self.gcost - self.stall_cost) / self.stall_cost) <= cfg.tolerance:
+                self.stall += 1
+            else:
+                self.stall, self.stall_cost = 0, self.gcost
+            if self.stall >= cfg.stall_iterations:
+                self.done = True
+        if self.target_cost is not None and self.gcost <= self.target_cost:
+            self.done = True
+        if it >= cfg.iterations:
+            self.done = True
+        if not self.done:
+            self.it += 1
+
+    def finish(self) -> PsoResult:
+        gdev, gbest, _ = self.sw.select()
+        self.res.seconds = time.perf_counter() - self.t0
+        self.res.best_cost, self.res.best_position = gdev, gbest
+        return self.res
+
+    def close(self) -> None:
+        self.sw.close()
+
+
 def run_pso(gp, cfg: PsoSettings, trace: Optional[Callable] = None,
             target_cost: Optional[float] = None, record: Optional[List[Dict]] = None) -> PsoResult:
     """Runs the PSO on `gp` (a SwarmLikelihood with max_swarm >= swarm_size).
@@ -352,103 +481,67 @@ def run_pso(gp, cfg: PsoSettings, trace: Optional[Callable] = None,
     """
     if cfg.swarm_updater == "forced":
         return _run_forced(gp, cfg, trace, target_cost, record)
-    S, P = cfg.swarm_size, gp.nparams
-    rng = np.random.default_rng(cfg.seed)
-    lo, hi = search_bounds(gp, cfg)
-    sw = DeviceSwarm(gp, S, cfg.inertia_weight, cfg.cognitive_learning_rate, cfg.social_learning_rate)
-    res = PsoResult(best_position=np.empty(P), best_cost=math.inf)
+    run = _SwarmRun(gp, cfg, trace, target_cost, record)
     try:
-        t0 = time.perf_counter()
-        x = lo + rng.random((S, P)) * (hi - lo)
-        v = (lo - hi) + rng.random((S, P)) * (2.0 * (hi - lo))
-        cost, info = sw.init(x, v)
-        res.evaluations += S
-        res.rejections += int(np.sum(info != 0))
-        if trace:
-            for k in range(S):
-                trace(0, 0, k, int(info[k]), float(cost[k]), x[k])
-        _record(record, it=0, kind="init", x=x, v=v, cost=cost, info=info)
-        gcost = math.inf
-        for c in cost:                        # state%update in particle order
-            if c <= gcost:
-                gcost = float(c)
-        res.history.append((0, time.perf_counter() - t0, gcost))
-        upd = ParameterUpdater(cfg.parameter_updater, cfg, rng, cost)
-        _record(record, it=0, kind="coef", coef=list(upd.coef), draws=list(upd.draws))
-        stall_cost, stall = math.inf, 0
-        it = 1
-        while True:
-            _abi.check(sw._lib.fb_pso_set_coefficients(sw._p, *upd.coef), "fb_pso_set_coefficients")
-            r1 = rng.random((S, P))
-            r2 = rng.random((S, P))
-            cost, info = sw.step(r1, r2)
-            res.evaluations += S
-            if trace or record is not None:
-                xs, vs = sw.positions()
-                if trace:
-                    for k in range(S):
-                        trace(it, 0, k, int(info[k]), float(cost[k]), xs[k])
-                _record(record, it=it, kind="step", coef=list(upd.coef), r1=r1, r2=r2, x=xs, v=vs, cost=cost,
-                        info=info)
-            todo = np.flatnonzero(info != 0)
-            rnd = 0
-            while len(todo):
-                res.rejections += len(todo)
-                rnd += 1
-                if rnd > cfg.max_retries:
-                    raise RuntimeError("PSO: particles stayed non-positive-definite after max_retries "
-                                       "re-randomisations")
-                # particle.f90:117-128
-                xr = lo + rng.random((len(todo), P)) * (hi - lo)
-                vr = (lo - hi) + rng.random((len(todo), P)) * (2.0 * (hi - lo))
-                c, inf = sw.retry(todo, xr, vr)
-                res.evaluations += len(todo)
-                cost[todo] = c
-                if trace:
-                    for r, k in enumerate(todo):
-                        trace(it, rnd, int(k), int(inf[r]), float(c[r]), xr[r])
-                _record(record, it=it, kind="retry", idx=todo.copy(), x=xr, v=vr, cost=c, info=inf)
-                todo = todo[inf != 0]
-            for c in cost:
-                if c <= gcost:
-                    gcost = float(c)
-                    res.best_iter = it
-            if upd.per_swarm:                 # next_iter_pso, pso.f90:563-565
-                upd.draws = []
-                upd.update(cost)
-                _record(record, it=it, kind="coef", coef=list(upd.coef), draws=list(upd.draws))
-            if cfg.lbfgs_every > 0 and it % cfg.lbfgs_every == 0:
-                gdev, gbest, _ = sw.select()
-                xp, fp, ne, na = polish(gp, gbest, cfg.lbfgs)
-                res.polish_calls += 1
-                res.polish_evals += ne
-                if fp < gdev:
-                    sw.set_gbest(xp, fp)
-                    gcost = float(fp)
-                    res.best_iter = it
-                    res.polish_improvements += 1
-                if trace:
-                    trace(it, -1, -1, 0, float(fp), xp)
-            res.iterations = it
-            res.history.append((it, time.perf_counter() - t0, gcost))
-            if cfg.relative_change:
-                if abs((gcost - stall_cost) / stall_cost) <= cfg.tolerance:
-                    stall += 1
-                else:
-                    stall, stall_cost = 0, gcost
-                if stall >= cfg.stall_iterations:
-                    break
-            if target_cost is not None and gcost <= target_cost:
-                break
-            if it >= cfg.iterations:
-                break
-            it += 1
-        gdev, gbest, _ = sw.select()
-        res.seconds = time.perf_counter() - t0
-        res.best_cost, res.best_position = gdev, gbest
-        return res
+        run.start()
+        while not run.done:
+            r1, r2 = run.draw()
+            cost, info = run.sw.step(r1, r2)
+            run.after_step(cost, info)
+        return run.finish()
     finally:
-        sw.close()
+        run.close()
+
+
+def run_pso_multi(gps: Sequence, cfgs: Sequence[PsoSettings], traces: Optional[Sequence] = None,
+                  target_costs: Optional[Sequence] = None,
+                  records: Optional[Sequence] = None) -> List[PsoResult]:
+    """The PSO of several training sets (one SwarmLikelihood each, any N) in
+    lock step: every iteration advances all unfinished swarms with one
+    fb_multi_pso_step call -- one captured graph that updates every swarm
+    and evaluates all of them together (the schedules of
+    fb_multi_swarm_loglik, the merged chained launches for uneven N) -- and
+    then applies each run's host rules (retries, updater, polishing,
+    stopping) as run_pso does.  A run that stops leaves the others going.
+    Each result equals run_pso(gps[g], cfgs[g]) with the same settings, bit
+    for bit.  Runs with the forced swarm updater (sequential by definition)
+    run on their own first.  The reference trains its GPs one after another
+    (app/main.f90:78-130)."""
+    from .gpr import MultiSwarm
+    G = len(gps)
+    if len(cfgs) != G:
+        raise ValueError("one PsoSettings per training set")
+    traces = list(traces) if traces is not None else [None] * G
+    target_costs = list(target_costs) if target_costs is not None else [None] * G
+    records = list(records) if records is not None else [None] * G
+    results: List[Optional[PsoResult]] = [None] * G
+    runs: List[Optional[_SwarmRun]] = [None] * G
+    for g in range(G):
+        if cfgs[g].swarm_updater == "forced":
+            results[g] = run_pso(gps[g], cfgs[g], traces[g], target_costs[g], records[g])
+    multi = MultiSwarm(gps)
+    try:
+        for g in range(G):
+            if results[g] is None:
+                runs[g] = _SwarmRun(gps[g], cfgs[g], traces[g], target_costs[g], records[g])
+                runs[g].start()
+        while True:
+            active = [g for g in range(G) if runs[g] is not None and not runs[g].done]
+            if not active:
+                break
+            steps = {g: (runs[g].sw,) + runs[g].draw() for g in active}
+            costs, infos = multi.pso_step(steps)
+            for g in active:
+                runs[g].after_step(costs[g], infos[g])
+        for g in range(G):
+            if runs[g] is not None:
+                results[g] = runs[g].finish()
+        return results
+    finally:
+        for r in runs:
+            if r is not None:
+                r.close()
+        multi.close()
 
 
 def _run_forced(gp, cfg: PsoSettings, trace, target_cost, record) -> PsoResult:

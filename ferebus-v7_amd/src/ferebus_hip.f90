@@ -26,7 +26,7 @@ module ferebus_hip
 
   public :: fb_create, fb_destroy, fb_nparams, fb_set_stream, fb_swarm_loglik, &
             fb_swarm_loglik_async, fb_particle_loglik, fb_get_alpha, fb_swarm_loglik_grad, &
-            fb_multi_create, fb_multi_swarm_loglik, fb_multi_destroy, &
+            fb_multi_create, fb_multi_swarm_loglik, fb_multi_destroy, fb_multi_schedule, fb_multi_pso_step, &
             fb_pso_create, fb_pso_destroy, fb_pso_init, fb_pso_step, fb_pso_retry, fb_pso_select, &
             fb_pso_set_gbest, fb_pso_positions, fb_pso_set_coefficients, &
             fb_profile_enable, fb_profile_read, fb_factor_path, fb_device_name, fb_last_error, fb_error_message
@@ -104,7 +104,8 @@ module ferebus_hip
       integer(c_int), intent(out) :: info(*)
     end function fb_swarm_loglik_grad
 
-    !> Several handles factorised in one persistent launch (uneven N).
+    !> Several handles evaluated in one call (uneven N; schedules as in
+    !> include/ferebus_hip.h).
     !> gps, theta, ll, info: arrays of c_ptr (c_loc of each handle's arrays).
     integer(c_int) function fb_multi_create(m, n_gps, gps) bind(c, name="fb_multi_create")
       import :: c_int, c_ptr
@@ -124,6 +125,22 @@ module ferebus_hip
       import :: c_int, c_ptr
       type(c_ptr), value :: m
     end function fb_multi_destroy
+
+    !> Schedule chosen for the swarm sizes s (0 sequential, 1 persistent,
+    !> 2 merged lock-step), -1 before a call with those sizes.
+    integer(c_int) function fb_multi_schedule(m, s) bind(c, name="fb_multi_schedule")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+      integer(c_int), intent(in) :: s(*)
+    end function fb_multi_schedule
+
+    !> One PSO iteration of the swarms ps (c_null_ptr skips a handle), all
+    !> evaluated in one captured graph; r1, r2, cost, info: arrays of c_ptr.
+    integer(c_int) function fb_multi_pso_step(m, ps, r1, r2, cost, info) bind(c, name="fb_multi_pso_step")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+      type(c_ptr), intent(in) :: ps(*), r1(*), r2(*), cost(*), info(*)
+    end function fb_multi_pso_step
 
     ! device-resident particle swarm: one graph replay per PSO iteration
     integer(c_int) function fb_pso_create(p, h, s, w, c1, c2) bind(c, name="fb_pso_create")

@@ -255,6 +255,22 @@ int fb_pso_select(fb_pso *p, double *gcost, double *gbest, int *gidx);
 int fb_pso_set_gbest(fb_pso *p, const double *gbest, double gcost);
 int fb_pso_positions(fb_pso *p, double *x, double *v);
 
+/* One PSO iteration of several swarms at once: ps[g] (or NULL to skip) is a
+ * swarm created on handle g of m.  Each swarm's pending selection and its
+ * velocity / position update (r1[g], r2[g]: P_g * s_g each; the swarm's
+ * coefficients) as fb_pso_step, then one evaluation of all of them on the
+ * schedule of fb_multi_swarm_loglik for these swarm sizes (chosen on the
+ * first call, on the swarms' current positions; where that is the
+ * persistent launch, the merged schedule is used), the whole iteration one
+ * captured graph replay.  cost[g], info[g] (either array or entry may be
+ * NULL) receive s_g entries each.  Each swarm ends in the state fb_pso_step
+ * would leave, bit for bit; init, retries, selection and coefficients stay
+ * per swarm (fb_pso_*).
+ * No reference counterpart: the reference trains its GPs one after another
+ * (app/main.f90:78-130), each with the swarm loop of pso.f90:382-472. */
+int fb_multi_pso_step(fb_multi *m, fb_pso *const *ps, const double *const *r1, const double *const *r2,
+                      double *const *cost, int *const *info);
+
 /* Kernel timing: when enabled, each launch of the handle is bracketed by HIP
  * events and the per-kernel totals accumulate until read.
  * kind: 0 covariance build; lock-step factorisation, per 128-column panel:

@@ -15,6 +15,7 @@ import pytest
 
 import oracle
 from ferebus_amd import KernelConfig, MultiSwarm, SwarmLikelihood, compile_kernel, fit_mean, synthetic
+from ferebus_amd.pso import PsoSettings, run_pso, run_pso_multi
 from test_gpu_parity import RTOL, _check, _lapack, _oracle
 
 pytestmark = pytest.mark.gpu
@@ -29,16 +30,17 @@ def _gp(n, d, seed, kernel="k1+k2", kernels=None, likelihood="marginal", mean="c
     return gp, x, y, prog, rng
 
 
-def _schedule(monkeypatch, schedule, tile=None):
+def _schedule(monkeypatch, schedule, tile=None, tail=None):
     monkeypatch.delenv("FB_MULTI_PERSISTENT", raising=False)
     if schedule is None:
         monkeypatch.delenv("FB_MULTI_SCHEDULE", raising=False)
     else:
         monkeypatch.setenv("FB_MULTI_SCHEDULE", schedule)
-    if tile is None:
-        monkeypatch.delenv("FB_MULTI_TILE", raising=False)
-    else:
-        monkeypatch.setenv("FB_MULTI_TILE", str(tile))
+    for key, v in (("FB_MULTI_TILE", tile), ("FB_MULTI_TAIL", tail)):
+        if v is None:
+            monkeypatch.delenv(key, raising=False)
+        else:
+            monkeypatch.setenv(key, str(v))
 
 
 SPECS = [dict(n=150, d=6, seed=81), dict(n=333, d=6, seed=82, likelihood="concentrated"),
@@ -91,12 +93,13 @@ def test_multi_matches_each_handle(gpu, monkeypatch, schedule):
     multi.close()
 
 
-@pytest.mark.parametrize("tile", [1, 2, 4, 8, 12, 16, 20])
-def test_merged_tile_codes(gpu, monkeypatch, tile):
+@pytest.mark.parametrize("tile,tail", [(0, 3), (1, 3), (2, 0), (4, 1), (8, 0), (8, 5), (12, 2), (16, 3), (20, 4)])
+def test_merged_tile_codes(gpu, monkeypatch, tile, tail):
     """The merged schedule with each panel tile code (row-tile shapes, two
-    role work-groups, fused row solves, 8-deep three-per-CU): the bits of
-    each handle's own evaluation."""
-    _schedule(monkeypatch, "2", tile)
+    role work-groups, fused row solves, 8-deep three-per-CU) and a number of
+    last panels pre-updated in the launch before them: the bits of each
+    handle's own evaluation."""
+    _schedule(monkeypatch, "2", tile, tail)
     gps, thetas = _mixed()
     multi = MultiSwarm(gps)
     lls, infos = multi.swarm_loglik(thetas)
@@ -132,6 +135,48 @@ def test_merged_wait_bound_reported(gpu, monkeypatch):
     multi.close()
     for gp in gps:
         gp.close()
+
+
+def _same_record(a, b, label):
+    assert len(a) == len(b), label
+    for ea, eb in zip(a, b):
+        assert ea.keys() == eb.keys(), label
+        for k in ea:
+            va, vb = ea[k], eb[k]
+            if isinstance(va, np.ndarray):
+                assert np.array_equal(va, vb), (label, ea["it"], ea["kind"], k)
+            else:
+                assert va == vb, (label, ea["it"], ea["kind"], k)
+
+
+@pytest.mark.parametrize("schedule", ["2", "0", None])
+def test_multi_pso_matches_each_run(gpu, monkeypatch, schedule):
+    """run_pso_multi over four training sets (different N, kernels,
+    likelihoods, swarm sizes, parameter updaters; one stopping early on the
+    relative change, one with the forced updater run on its own) gives each
+    the run of its own run_pso, bit for bit: every drawn coefficient and
+    uniform, position, velocity, cost and INFO of every iteration, and the
+    final gbest.  The schedules: merged (2), sequential (0), as timed."""
+    _schedule(monkeypatch, schedule)
+    gps = [_gp(**sp, S=12)[0] for sp in SPECS] + [_gp(n=90, d=6, seed=85, S=12)[0]]
+    cfgs = [PsoSettings(swarm_size=10, iterations=7, seed=40),
+            PsoSettings(swarm_size=6, iterations=7, seed=41, parameter_updater="unified_adaptive"),
+            PsoSettings(swarm_size=12, iterations=40, seed=42, relative_change=True, stall_iterations=2,
+                        tolerance=1e-2),
+            PsoSettings(swarm_size=4, iterations=9, seed=43, parameter_updater="random_unified"),
+            PsoSettings(swarm_size=5, iterations=3, seed=44, swarm_updater="forced")]
+    rec_m = [[] for _ in gps]
+    res_m = run_pso_multi(gps, cfgs, records=rec_m)
+    for g, (gp, cfg) in enumerate(zip(gps, cfgs)):
+        rec = []
+        res = run_pso(gp, cfg, record=rec)
+        print(f"gp {g}: n={gp.n} iterations {res.iterations} / {res_m[g].iterations} cost {res.best_cost}")
+        assert res.iterations == res_m[g].iterations and res.evaluations == res_m[g].evaluations
+        assert res.best_cost == res_m[g].best_cost
+        assert np.array_equal(res.best_position, res_m[g].best_position)
+        assert [h[2] for h in res.history] == [h[2] for h in res_m[g].history]
+        _same_record(rec, rec_m[g], f"gp {g}")
+    assert res_m[2].iterations < 40      # stopped on the relative change while others ran on
 
 
 @pytest.mark.parametrize("schedule", ["2", None])
