@@ -114,8 +114,26 @@ def uneven_run(steps: int = 3):
     same = all(np.array_equal(gp.swarm_loglik(th)[0], ll, equal_nan=True) and
                np.array_equal(ll, l1, equal_nan=True) and np.array_equal(ll, l2, equal_nan=True)
                for gp, th, ll, l1, l2 in zip(gps, thetas, lls, lls1, llm))
+    # one PSO iteration of the ten swarms (selection, update, evaluation):
+    # one fb_multi_pso_step against each swarm's own fb_pso_step in turn
+    from ferebus_amd.pso import DeviceSwarm
+    prng = np.random.default_rng(5)
+    sws = [DeviceSwarm(gp, 40, 0.789, 1.494, 1.494) for gp in gps]
+    for sw, th in zip(sws, thetas):
+        sw.init(th, prng.uniform(-0.05, 0.05, th.shape))
+    rs = [(prng.random(th.shape), prng.random(th.shape)) for th in thetas]
+    pso_steps = {g: (sws[g],) + rs[g] for g in range(len(gps))}
+    multi.pso_step(pso_steps)                # captures the iteration
+    for sw, r in zip(sws, rs):
+        sw.step(*r)
+    t_pso_multi, _ = timed(lambda: multi.pso_step(pso_steps))
+    t_pso_seq, _ = timed(lambda: [sw.step(*r) for sw, r in zip(sws, rs)])
+    for sw in sws:
+        sw.close()
     out = {"gps": len(gps), "N": [gp.n for gp in gps], "particles": 40 * len(gps),
            "ms_batched_call": round(1e3 * t_multi, 3), "schedule": multi.schedule(),
+           "ms_pso_iteration_batched": round(1e3 * t_pso_multi, 3),
+           "ms_pso_iteration_per_gp": round(1e3 * t_pso_seq, 3),
            "ms_merged_lockstep": round(1e3 * t_merged, 3), "ms_one_launch": round(1e3 * t_one, 3),
            "ms_per_gp_evaluations": round(1e3 * t_seq, 3),
            "evals_per_s_batched_call": round(40 * len(gps) / t_multi, 1),
