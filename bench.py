@@ -320,37 +320,49 @@ def main():
     ms_per_step = gpu_ms / args.steps
     value = S * args.steps / (gpu_ms / 1e3)
 
-    # per-kernel times from HIP events on the launch stream (a separate pass:
-    # the events split the pipeline, so this pass is not the timed one)
-    gp.profile(True)
+    # times from HIP events on the launch stream, in separate passes (the
+    # events split the pipeline, so these passes are not the timed one): the
+    # phases (build, factorisation span, likelihood) with no events between
+    # the factorisation's launches, then every launch bracketed
     prof_steps = 3
+    kinds = ("build", "factor", "update", "solve_factor", "update_solve", "likelihood", "swarm", "factor_span")
+    gp.profile(2)
     for k in range(prof_steps):
         gp.swarm_loglik(thetas[k % 4])
-    prof = {k: gp.profile_read(k) for k in ("build", "factor", "update", "solve_factor", "update_solve",
-                                            "likelihood", "swarm")}
+    span = {k: gp.profile_read(k) for k in kinds}
+    gp.profile(1)
+    for k in range(prof_steps):
+        gp.swarm_loglik(thetas[k % 4])
+    prof = {k: gp.profile_read(k) for k in kinds}
     gp.profile(False)
     n = args.n
+    groups = gp.factor_groups(S)
     persistent = prof["factor"][1] > 0
     if persistent:
         # one persistent launch (dag_factor_kernel, after a counter reset) per evaluation
-        fact_ms = prof["factor"][0] / prof_steps
+        fact_ms = span["factor"][0] / prof_steps
         fact_kernel = "dag_factor_kernel (whole batched Cholesky of the swarm, one persistent launch)"
     else:
-        fact_ms = (prof["update"][0] + prof["solve_factor"][0] + prof["update_solve"][0]) / prof_steps
+        fact_ms = span["factor_span"][0] / prof_steps
     # chained lock-step schedule: no third launch per panel
     chained = not persistent and prof["update_solve"][1] == 0
+    launches = (prof["update"][1] + prof["solve_factor"][1] + prof["update_solve"][1]) // prof_steps
+    grouped = (f", the swarm in {groups} groups of particles on concurrent streams" if groups > 1 else "")
     if chained:
         fact_kernel = ("chained lock-step batched Cholesky: per 128-column panel panel_chain_kernel (update, "
-                       "diagonal chain handed over between work-groups) and solve_rows_kernel "
-                       "(31 launches per evaluation, durations summed)")
+                       "diagonal chain handed over between work-groups) and solve_rows_kernel where the next "
+                       f"update launch does not solve the rows ({launches} launches per evaluation{grouped}); "
+                       "duration = the factorisation's span from the end of the build to the start of the "
+                       "likelihood")
     elif not persistent:
         fact_kernel = ("lock-step batched Cholesky: per 128-column panel panel_update_factor_kernel, "
-                       "trsm_factor_kernel, update_trsm_kernel (47 launches per evaluation, durations summed)")
+                       f"trsm_factor_kernel, update_trsm_kernel ({launches} launches per evaluation{grouped}); "
+                       "duration = the factorisation's span")
     fact_flops = S * potrf_flops(n)
     factor_path = gp.factor_path(S)
     traffic, traffic_src = (profiled_traffic(factor_path, chained) if (n == 2000 and S == 50)
                             else (None, None))
-    build_ms = prof["build"][0] / prof_steps
+    build_ms = span["build"][0] / prof_steps
     build_bytes = S * 8.0 * n * (n + 1) / 2.0 + 8.0 * n * d
     achieved = fact_flops / (fact_ms * 1e-3) / 1e12
 
@@ -372,6 +384,7 @@ def main():
                    "path": "batched (hipGraph replay per swarm evaluation, "
                            + ("persistent factorisation)" if persistent else "lock-step factorisation)"),
                    "factor_path": factor_path,
+                   "stream_groups": groups,
                    "schedule": "persistent" if persistent else ("chained (2 launches per panel)" if chained
                                                                else "3 launches per panel"),
                    "failed_particles_last_step": fails},
@@ -387,6 +400,8 @@ def main():
             "traffic_source": None if traffic is None else "rocprofv3 FETCH_SIZE + WRITE_SIZE, " + traffic_src,
             "avg_duration_ms": round(fact_ms, 4),
         },
+        "phases_ms_per_step": {k: round(span[k][0] / prof_steps, 4) for k in ("build", "factor_span", "likelihood",
+                                                                               "swarm")},
         "kernels_ms_per_step": {k: round(v[0] / prof_steps, 4) for k, v in prof.items()},
         "kernel_launches_per_step": {k: v[1] // prof_steps for k, v in prof.items()},
         "build_hbm_gbs": round(build_bytes / (build_ms * 1e-3) / 1e9, 1) if build_ms > 0 else None,

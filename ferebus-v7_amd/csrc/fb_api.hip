@@ -172,7 +172,10 @@ struct fb_handle {
     // concurrently, so one group's latency-bound potrf / trsm steps overlap the
     // other groups' trailing updates
     static constexpr int kMaxGroups = 8;
-    int n_groups = 1;
+    static constexpr int kTuneGroups = 4;   // the autotuner tries 1 .. kTuneGroups stream groups
+    int n_groups = 1;                // streams created for the groups
+    int forced_groups = -1;          // FB_STREAMS pins the group count
+    std::map<int, int> groups;       // tuned stream-group count per swarm size
     bool use_graph = true;
     bool use_dag = true;             // persistent factorisation for the batched path
     int gemm_kc = 32;                // lock-step trailing-update chunk depth
@@ -219,11 +222,11 @@ struct fb_handle {
     std::map<int, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
     int graph_epoch = 0;             // bumped whenever captured work goes stale
     // profiling
-    bool profile = false;
+    int profile = 0;                 // 1: an event pair per launch, 2: per phase only (kinds 0, 4, 5, 6, 8)
     std::vector<EventPair> pending;
     std::vector<hipEvent_t> event_pool;
-    double prof_ms[7] = {0, 0, 0, 0, 0, 0, 0};
-    long long prof_n[7] = {0, 0, 0, 0, 0, 0, 0};
+    double prof_ms[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    long long prof_n[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     long long tune_evals = 0;        // evaluations the autotuner has timed (read as profile kind 7)
 };
 
@@ -272,7 +275,7 @@ hipEvent_t take_event(fb_handle *h)
 template <class F>
 void timed(fb_handle *h, int kind, hipStream_t st, F fn)
 {
-    if (!h->profile) { fn(); return; }
+    if (!h->profile || (h->profile == 2 && kind >= 1 && kind <= 3)) { fn(); return; }
     EventPair p{kind, take_event(h), take_event(h)};
     if (p.a) (void)hipEventRecord(p.a, st);
     fn();
@@ -280,10 +283,9 @@ void timed(fb_handle *h, int kind, hipStream_t st, F fn)
     if (p.a && p.b) h->pending.push_back(p);
 }
 
-// Lock-step pipeline: one launch per factorisation step (the plain path).
-void enqueue_group(fb_handle *h, const fb::SwarmBuffers &b, hipStream_t st)
+// The lock-step factorisation of the particles of b: one launch per step.
+void enqueue_factor(fb_handle *h, const fb::SwarmBuffers &b, hipStream_t st)
 {
-    timed(h, 0, st, [&] { fb::launch_build(b, st); });
     // per 128-column panel: update + first block's factorisation (kind 1),
     // panel solve + second block's factorisation (kind 2), second block's
     // update + panel solve (kind 3); chained: the update launch also solves
@@ -291,7 +293,7 @@ void enqueue_group(fb_handle *h, const fb::SwarmBuffers &b, hipStream_t st)
     // one launch (kind 2)
     for (int c0 = 0; c0 < h->np; c0 += fb::kNBO) {
         if (b.chain) {
-            const bool rows = c0 + fb::kNBO < h->np && !fb::panel_fused(b, c0);
+            const bool rows = fb::panel_rows_launch(b, c0);
             if (h->k1_events) {
                 // tuning: the panel's whole time, its row-solve launch included
                 std::pair<hipEvent_t, hipEvent_t> ev{take_event(h), take_event(h)};
@@ -319,6 +321,13 @@ void enqueue_group(fb_handle *h, const fb::SwarmBuffers &b, hipStream_t st)
         timed(h, 2, st, [&] { fb::launch_trsm_factor(b, c0, st); });
         if (c1 + fb::kNBI < h->np) timed(h, 3, st, [&] { fb::launch_update_trsm(b, c1, st); });
     }
+}
+
+// Lock-step pipeline: build, factorisation (its span: kind 8), likelihood.
+void enqueue_group(fb_handle *h, const fb::SwarmBuffers &b, hipStream_t st)
+{
+    timed(h, 0, st, [&] { fb::launch_build(b, st); });
+    timed(h, 8, st, [&] { enqueue_factor(h, b, st); });
     timed(h, 4, st, [&] { fb::launch_likelihood(b, st); });
 }
 
@@ -362,34 +371,55 @@ int dag_plan_for(fb_handle *h, int S)
     return FB_OK;
 }
 
-// The whole swarm [0, S): the persistent path, or groups of particles on
-// concurrent streams forked from and joined back into `st` (also inside a
-// graph capture).
+// Stream groups of a swarm of S on the lock-step path: FB_STREAMS, else the
+// autotuner's choice for S, else one.
+int groups_for(const fb_handle *h, int S)
+{
+    int G = h->forced_groups;
+    if (G < 1) {
+        auto it = h->groups.find(S);
+        G = it != h->groups.end() ? it->second : 1;
+    }
+    return std::max(1, std::min({G, S, h->n_groups}));
+}
+
+// The whole swarm [0, S): the persistent path, or the lock-step one -- with
+// G > 1 stream groups, one build of the swarm, the factorisations of G
+// groups of particles on concurrent streams forked from and joined back into
+// `st` (also inside a graph capture: a launch of one group that leaves the
+// chip partly idle in its last round overlaps with the others'), then one
+// likelihood launch.
 void enqueue_pipeline(fb_handle *h, int S, const double *theta, double *ll, int *info, hipStream_t st)
 {
     if (h->use_dag) {
         enqueue_dag(h, buffers(h, 0, S, theta, ll, info), st);
         return;
     }
-    const int G = std::max(1, std::min(h->n_groups, S));
+    const int G = groups_for(h, S);
     if (G == 1) {
         enqueue_group(h, buffers(h, 0, S, theta, ll, info), st);
         return;
     }
-    (void)hipEventRecord(h->fork_ev, st);
-    const int per = (S + G - 1) / G;
-    for (int g = 0; g < G; ++g) {
-        const int g0 = g * per;
-        const int cnt = std::min(per, S - g0);
-        if (cnt <= 0) break;
-        hipStream_t gs = g == 0 ? st : h->group_stream[g];
-        if (g > 0) (void)hipStreamWaitEvent(gs, h->fork_ev, 0);
-        enqueue_group(h, buffers(h, g0, cnt, theta, ll, info), gs);
-        if (g > 0) {
-            (void)hipEventRecord(h->join_ev[g], gs);
-            (void)hipStreamWaitEvent(st, h->join_ev[g], 0);
+    fb::SwarmBuffers all = buffers(h, 0, S, theta, ll, info);
+    if (h->chain) all.chain = h->d_chain;   // the build resets every group's chain words
+    timed(h, 0, st, [&] { fb::launch_build(all, st); });
+    timed(h, 8, st, [&] {
+        (void)hipEventRecord(h->fork_ev, st);
+        const int per = (S + G - 1) / G;
+        for (int g = 0; g < G; ++g) {
+            const int g0 = g * per;
+            const int cnt = std::min(per, S - g0);
+            if (cnt <= 0) break;
+            hipStream_t gs = g == 0 ? st : h->group_stream[g];
+            if (g > 0) (void)hipStreamWaitEvent(gs, h->fork_ev, 0);
+            enqueue_factor(h, buffers(h, g0, cnt, theta, ll, info), gs);
+            if (g > 0) {
+                (void)hipEventRecord(h->join_ev[g], gs);
+                (void)hipStreamWaitEvent(st, h->join_ev[g], 0);
+            }
         }
-    }
+    });
+    timed(h, 4, st, [&] { fb::launch_likelihood(all, st); });
 }
 
 int drain_profile(fb_handle *h)
@@ -496,17 +526,20 @@ struct TuneRecord {
     int np = 0, swarm = 0, path = -1;
     double ms = 0.0;
     std::string tiles;     // per-panel shapes of the per-panel path, "0,2,1,..."
+    int groups = 1;        // stream groups of the lock-step paths
 };
 
 // Per-panel tile codes the lock-step launchers accept: a shape (0 64x128,
-// 1 32x128, 2 32x64), + kTileTwoRoles; kTileFused and kTileThreePerCU are
-// 64x128 only.
+// 1 32x128, 2 32x64), + kTileTwoRoles; kTileFused, kTileThreePerCU and
+// kTileAhead (alone or with kTileThreePerCU) are 64x128 only.
 static bool valid_tile_code(int t)
 {
     const int shape = t & 3, var = t & ~(3 | fb::kTileTwoRoles);
     if (t < 0 || shape == 3) return false;
     if (var == 0) return true;
-    return shape == 0 && (var == fb::kTileFused || var == fb::kTileThreePerCU);
+    return shape == 0 && (var == fb::kTileFused || var == fb::kTileThreePerCU || var == fb::kTileFourPerCU ||
+                          var == fb::kTileAhead || var == (fb::kTileAhead | fb::kTileThreePerCU) ||
+                          var == (fb::kTileAhead | fb::kTileFourPerCU));
 }
 
 static std::vector<TuneRecord> read_tune_file(const std::string &path)
@@ -531,6 +564,7 @@ static std::vector<TuneRecord> read_tune_file(const std::string &path)
         r.swarm = std::atoi(b + 9);
         r.path = std::atoi(c + 8);
         r.ms = m ? std::atof(m + 6) : 0.0;
+        if (const char *g = std::strstr(line, "\"groups\": ")) r.groups = std::max(1, std::atoi(g + 10));
         if (const char *t = std::strstr(line, "\"tiles\": \"")) {
             t += 10;
             const char *te = std::strchr(t, '"');
@@ -545,23 +579,27 @@ static std::vector<TuneRecord> read_tune_file(const std::string &path)
 static void write_tune_file(const fb_handle *h, int s, int path, double ms)
 {
     std::vector<TuneRecord> recs = read_tune_file(h->tune_file);
-    bool found = false;
-    for (auto &r : recs)
-        if (r.device == h->device_key && r.np == h->np && r.swarm == s) { r.path = path; r.ms = ms; found = true; }
     std::string tiles;
     if (path == kModePerPanel && h->panel_tiles.count(s))
         for (signed char t : h->panel_tiles.at(s)) tiles += (tiles.empty() ? "" : ",") + std::to_string((int)t);
+    const int groups = h->groups.count(s) ? h->groups.at(s) : 1;
+    bool found = false;
     for (auto &r : recs)
-        if (r.device == h->device_key && r.np == h->np && r.swarm == s) r.tiles = tiles;
-    if (!found) recs.push_back({h->device_key, h->np, s, path, ms, tiles});
+        if (r.device == h->device_key && r.np == h->np && r.swarm == s) {
+            r.path = path; r.ms = ms; r.tiles = tiles; r.groups = groups;
+            found = true;
+        }
+    if (!found) recs.push_back({h->device_key, h->np, s, path, ms, tiles, groups});
     const std::string tmp = h->tune_file + ".tmp" + std::to_string((long long)std::rand());
     FILE *f = std::fopen(tmp.c_str(), "w");
     if (!f) return;   // a cache that cannot be written only costs re-tuning
     std::fprintf(f, "[\n");
     for (size_t k = 0; k < recs.size(); ++k)
-        std::fprintf(f, "  {\"device\": \"%s\", \"np\": %d, \"swarm\": %d, \"path\": %d, \"ms\": %.4f, \"tiles\": \"%s\"}%s\n",
+        std::fprintf(f,
+                     "  {\"device\": \"%s\", \"np\": %d, \"swarm\": %d, \"path\": %d, \"ms\": %.4f, \"tiles\": \"%s\", "
+                     "\"groups\": %d}%s\n",
                      recs[k].device.c_str(), recs[k].np, recs[k].swarm, recs[k].path, recs[k].ms,
-                     recs[k].tiles.c_str(), k + 1 < recs.size() ? "," : "");
+                     recs[k].tiles.c_str(), recs[k].groups, k + 1 < recs.size() ? "," : "");
     std::fprintf(f, "]\n");
     const bool ok = std::fclose(f) == 0;
     if (!ok || std::rename(tmp.c_str(), h->tune_file.c_str()) != 0) std::remove(tmp.c_str());
@@ -653,7 +691,7 @@ int fb_create(fb_handle **out, int n, int d, const double *x, const double *y, c
         return fail(FB_E_HIP, "fb_create: stream creation failed");
     }
     h->stream = h->own_stream;
-    if (const char *env = std::getenv("FB_STREAMS")) h->n_groups = std::atoi(env);
+    if (const char *env = std::getenv("FB_STREAMS")) h->forced_groups = std::atoi(env);
     if (const char *env = std::getenv("FB_GRAPH")) h->use_graph = std::atoi(env) != 0;
     if (const char *env = std::getenv("FB_PERSISTENT")) {
         h->use_dag = std::atoi(env) != 0;
@@ -711,6 +749,7 @@ int fb_create(fb_handle **out, int n, int d, const double *x, const double *y, c
                     h->panel_tiles[r.swarm] = v;
                 }
                 h->tuned[r.swarm] = r.path;
+                if (r.groups > 1) h->groups[r.swarm] = std::min(r.groups, (int)fb_handle::kTuneGroups);
             }
         } else {
             h->tune_file.clear();
@@ -720,7 +759,7 @@ int fb_create(fb_handle **out, int n, int d, const double *x, const double *y, c
         h->use_dag = false;
         if (h->forced_mode == 0) h->forced_mode = 1;
     }
-    h->n_groups = std::max(1, std::min(h->n_groups, fb_handle::kMaxGroups));
+    h->n_groups = std::max(1, std::min(std::max(h->forced_groups, (int)fb_handle::kTuneGroups), fb_handle::kMaxGroups));
     bool ev_ok = hipEventCreateWithFlags(&h->fork_ev, hipEventDisableTiming) == hipSuccess;
     for (int g = 1; g < h->n_groups && ev_ok; ++g)
         ev_ok = hipStreamCreateWithFlags(&h->group_stream[g], hipStreamNonBlocking) == hipSuccess &&
@@ -828,6 +867,7 @@ static int choose_path(fb_handle *h, int s)
     FB_HIP(hipEventCreate(&e1));
     double best = 1e300;
     int best_mode = 0;
+    std::vector<std::pair<float, int>> lockstep_ms;   // (time, mode) of the lock-step paths, one group
     for (int mode = 0; mode < kNumModes; ++mode) {
         set_mode(h, mode);
         if (mode == 0 && dag_plan_for(h, s) != FB_OK) continue;
@@ -847,58 +887,19 @@ static int choose_path(fb_handle *h, int s)
             (void)hipMemcpy(&err, h->d_ctl + fb::kDagErrorWord, sizeof(int), hipMemcpyDeviceToHost);
             if (err != 0) ms_best = 1e30f;
         }
+        if (mode != 0) lockstep_ms.push_back({ms_best, mode});
         if (ms_best < best) { best = ms_best; best_mode = mode; }
     }
     // the panel updates of the three 16-deep shapes (chained: each with three
-    // or two role work-groups, and the fused and three-per-CU 64x128
-    // variants), timed panel by panel: a mix of the fastest variant of each
-    // panel is one more candidate
+    // or two role work-groups, and the fused, ahead, three- and four-per-CU
+    // 64x128 variants), timed panel by panel: a mix of the fastest variant
+    // of each panel is one more candidate
     const int npan = h->np / fb::kNBO;
-    std::vector<int> codes = {0, 1, 2};
-    if (h->chain && s <= fb::kChainMaxSwarm) {
-        for (int t = 0; t < 3; ++t) codes.push_back(t | fb::kTileTwoRoles);
-        codes.push_back(fb::kTileFused);
-        codes.push_back(fb::kTileFused | fb::kTileTwoRoles);
-        codes.push_back(fb::kTileThreePerCU);
-        codes.push_back(fb::kTileThreePerCU | fb::kTileTwoRoles);
-    }
-    const int nc = (int)codes.size();
-    std::vector<std::vector<float>> tpan(nc, std::vector<float>(npan, 1e30f));
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> evs;
-    // three passes over the candidates, the fastest time of each panel kept
-    // (with two, the choice between variants a few per cent apart changed
-    // from run to run)
-    for (int pass = 0; pass < 3; ++pass)
-    for (int t = 0; t < nc; ++t) {
-        set_mode(h, 2 + ((codes[t] & fb::kTileFused) ? 0 : (codes[t] & 3)), s);
-        h->tile = codes[t];
-        evs.clear();
-        h->k1_events = &evs;
-        enqueue_pipeline(h, s, h->d_theta, h->d_ll, h->d_info, st);
-        ++h->tune_evals;
-        h->k1_events = nullptr;
-        if (hipStreamSynchronize(st) != hipSuccess) break;
-        for (size_t k = 0; k < evs.size() && (int)k < npan; ++k) {
-            float ms = 1e30f;
-            (void)hipEventElapsedTime(&ms, evs[k].first, evs[k].second);
-            tpan[t][k] = std::min(tpan[t][k], ms);
-        }
-        for (auto &e : evs) {
-            h->event_pool.push_back(e.first);
-            h->event_pool.push_back(e.second);
-        }
-    }
-    std::vector<signed char> mix(npan, 0);
-    bool mixed = false;
-    for (int k = 0; k < npan; ++k) {
-        int bt = 0;
-        for (int t = 1; t < nc; ++t) if (tpan[t][k] < tpan[bt][k]) bt = t;
-        mix[k] = (signed char)codes[bt];
-        mixed = mixed || mix[k] != mix[0];
-    }
-    if (mixed || (mix[0] & (fb::kTileTwoRoles | fb::kTileFused | fb::kTileThreePerCU))) {
-        h->panel_tiles[s] = mix;
-        set_mode(h, kModePerPanel, s);
+    auto set_groups = [&](int G) {
+        if (G > 1) h->groups[s] = G;
+        else h->groups.erase(s);
+    };
+    auto time_eval = [&]() -> float {
         float ms_best = 1e30f;
         for (int rep = 0; rep < 2; ++rep) {
             (void)hipEventRecord(e0, st);
@@ -910,14 +911,113 @@ static int choose_path(fb_handle *h, int s)
             (void)hipEventElapsedTime(&ms, e0, e1);
             ms_best = std::min(ms_best, ms);
         }
-        if (ms_best < best) { best = ms_best; best_mode = kModePerPanel; }
-        else h->panel_tiles.erase(s);
+        return ms_best;
+    };
+    // the fastest code of each panel with the swarm in G stream groups (timed
+    // on the first group's launches, the others running beside them); true
+    // when the mix is worth timing as a whole
+    auto panel_mix = [&](int G, std::vector<signed char> &mix) -> bool {
+        std::vector<int> codes = {0, 1, 2};
+        if (h->chain && fb::ceil_div(s, G) <= fb::kChainMaxSwarm) {
+            for (int t = 0; t < 3; ++t) codes.push_back(t | fb::kTileTwoRoles);
+            codes.push_back(fb::kTileFused);
+            codes.push_back(fb::kTileFused | fb::kTileTwoRoles);
+            codes.push_back(fb::kTileThreePerCU);
+            codes.push_back(fb::kTileThreePerCU | fb::kTileTwoRoles);
+            for (int v : {0, fb::kTileTwoRoles, fb::kTileThreePerCU, fb::kTileThreePerCU | fb::kTileTwoRoles})
+                codes.push_back(fb::kTileAhead | v);
+            for (int v : {0, fb::kTileTwoRoles, fb::kTileAhead, fb::kTileAhead | fb::kTileTwoRoles})
+                codes.push_back(fb::kTileFourPerCU | v);
+        }
+        set_groups(G);
+        const int nc = (int)codes.size();
+        std::vector<std::vector<float>> tpan(nc, std::vector<float>(npan, 1e30f));
+        std::vector<std::pair<hipEvent_t, hipEvent_t>> evs;
+        // three passes over the candidates, the fastest time of each panel
+        // kept (with two, the choice between variants a few per cent apart
+        // changed from run to run)
+        for (int pass = 0; pass < 3; ++pass)
+            for (int t = 0; t < nc; ++t) {
+                set_mode(h, 2 + ((codes[t] & fb::kTileFused) ? 0 : (codes[t] & 3)), s);
+                h->tile = codes[t];
+                evs.clear();
+                h->k1_events = &evs;
+                enqueue_pipeline(h, s, h->d_theta, h->d_ll, h->d_info, st);
+                ++h->tune_evals;
+                h->k1_events = nullptr;
+                if (hipStreamSynchronize(st) != hipSuccess) break;
+                for (size_t k = 0; k < evs.size() && (int)k < npan; ++k) {
+                    float ms = 1e30f;
+                    (void)hipEventElapsedTime(&ms, evs[k].first, evs[k].second);
+                    tpan[t][k] = std::min(tpan[t][k], ms);
+                }
+                for (auto &e : evs) {
+                    h->event_pool.push_back(e.first);
+                    h->event_pool.push_back(e.second);
+                }
+            }
+        mix.assign(npan, 0);
+        bool mixed = false;
+        for (int k = 0; k < npan; ++k) {
+            int bt = 0;
+            for (int t = 1; t < nc; ++t)
+                if (tpan[t][k] < tpan[bt][k]) bt = t;
+            mix[k] = (signed char)codes[bt];
+            mixed = mixed || mix[k] != mix[0];
+        }
+        return mixed || (mix[0] & (fb::kTileTwoRoles | fb::kTileFused | fb::kTileThreePerCU | fb::kTileAhead |
+                                   fb::kTileFourPerCU));
+    };
+    std::vector<signed char> mix;
+    if (panel_mix(1, mix)) {
+        h->panel_tiles[s] = mix;
+        set_mode(h, kModePerPanel, s);
+        const float ms = time_eval();
+        lockstep_ms.push_back({ms, kModePerPanel});
+        if (ms < best) { best = ms; best_mode = kModePerPanel; }
     }
+    // stream groups: the swarm's factorisations split over 2 .. kTuneGroups
+    // concurrent streams (the groups' launches fill each other's last
+    // rounds), timed with the three fastest lock-step paths of one group,
+    // then the per-panel mix re-timed with the best group count
+    int best_g = 1;
+    if (h->forced_groups < 1 && !lockstep_ms.empty()) {
+        std::sort(lockstep_ms.begin(), lockstep_ms.end());
+        const int nc = std::min<int>(3, (int)lockstep_ms.size());
+        for (int G = 2; G <= std::min((int)fb_handle::kTuneGroups, std::min(s, h->n_groups)); ++G)
+            for (int c = 0; c < nc; ++c) {
+                const int mode = lockstep_ms[c].second;
+                set_mode(h, mode, s);
+                set_groups(G);
+                const float ms = time_eval();
+                if (ms < best) { best = ms; best_mode = mode; best_g = G; }
+            }
+        std::vector<signed char> mixg;
+        if (best_g > 1 && panel_mix(best_g, mixg)) {
+            const bool had = h->panel_tiles.count(s) != 0;
+            const std::vector<signed char> keep = had ? h->panel_tiles[s] : std::vector<signed char>();
+            h->panel_tiles[s] = mixg;
+            set_mode(h, kModePerPanel, s);
+            set_groups(best_g);
+            const float ms = time_eval();
+            if (ms < best) {
+                best = ms;
+                best_mode = kModePerPanel;
+            } else if (had) {
+                h->panel_tiles[s] = keep;
+            } else {
+                h->panel_tiles.erase(s);
+            }
+        }
+    }
+    if (best_g > 1) h->groups[s] = best_g;
+    else h->groups.erase(s);
+    if (best_mode != kModePerPanel) h->panel_tiles.erase(s);
+    h->tuned[s] = best_mode;
+    set_mode(h, best_mode, s);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     FB_HIP(hipGetLastError());
-    h->tuned[s] = best_mode;
-    set_mode(h, best_mode, s);
     if (!h->tune_file.empty()) write_tune_file(h, s, best_mode, best);
     return FB_OK;
 }
@@ -966,11 +1066,10 @@ static int run_swarm(fb_handle *h, int s, int *dag_error)
 {
     hipStream_t st = h->stream;
     if (h->profile) {
-        // one stream, so the per-kernel event pairs time each kernel alone
-        const int g = h->n_groups;
-        h->n_groups = 1;
+        // eager, with an event pair around each launch (with stream groups,
+        // launches of different groups overlap; kind 8 is the factorisation's
+        // span)
         timed(h, 5, st, [&] { enqueue_pipeline(h, s, h->d_theta, h->d_ll, h->d_info, st); });
-        h->n_groups = g;
     } else if (!h->use_graph) {
         enqueue_pipeline(h, s, h->d_theta, h->d_ll, h->d_info, st);
     } else {
@@ -1242,7 +1341,7 @@ int fb_swarm_loglik_static(fb_handle *h, int s, const double *d_theta, double *d
     for (int c0 = 0; c0 < h->np; c0 += fb::kNBO) {
         if (b.chain) {
             fb::launch_panel_chain(b, c0, st);
-            if (c0 + fb::kNBO < h->np && !fb::panel_fused(b, c0)) fb::launch_solve_rows(b, c0, st);
+            if (fb::panel_rows_launch(b, c0)) fb::launch_solve_rows(b, c0, st);
             continue;
         }
         fb::launch_panel_update_factor(b, c0, st);
@@ -2053,6 +2152,13 @@ int fb_multi_pso_step(fb_multi *m, fb_pso *const *ps, const double *const *r1, c
     return FB_OK;
 }
 
+int fb_factor_groups(const fb_handle *h, int s, int *groups)
+{
+    if (!h || !groups) return fail(FB_E_ARG, "fb_factor_groups: null argument");
+    *groups = s >= 1 ? groups_for(h, s) : 1;
+    return FB_OK;
+}
+
 int fb_factor_path(const fb_handle *h, int s, int *path)
 {
     if (!h || !path) return fail(FB_E_ARG, "fb_factor_path: null argument");
@@ -2066,13 +2172,13 @@ int fb_profile_enable(fb_handle *h, int enable)
 {
     if (!h) return fail(FB_E_ARG, "fb_profile_enable: null handle");
     int rc = drain_profile(h);
-    h->profile = enable != 0;
+    h->profile = enable == 2 ? 2 : enable != 0 ? 1 : 0;
     return rc;
 }
 
 int fb_profile_read(fb_handle *h, int kind, double *total_ms, long long *launches)
 {
-    if (!h || kind < 0 || kind > 7 || !total_ms || !launches) return fail(FB_E_ARG, "fb_profile_read: bad argument");
+    if (!h || kind < 0 || kind > 8 || !total_ms || !launches) return fail(FB_E_ARG, "fb_profile_read: bad argument");
     if (kind == 7) {   // the autotuner's timed evaluations so far (not reset)
         *total_ms = 0.0;
         *launches = h->tune_evals;

@@ -157,6 +157,7 @@ __global__ void __launch_bounds__(kThreads) build_kernel(BuildArgs a)
         if (a.chain) {
             int *w = a.chain + (size_t)(s0 + threadIdx.x) * kChainStride;
             w[kChainF0] = 0; w[kChainF1] = 0; w[kChainXccB] = 0; w[kChainXccC] = 0; w[kChainF2] = 0;
+            w[kChainS0] = 0; w[kChainS1] = 0;
         }
     }
     __syncthreads();

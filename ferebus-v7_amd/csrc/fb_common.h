@@ -169,6 +169,8 @@ constexpr int kDinv = kDinvFlags + 4;
 // announcements, each in its own 32-byte sector.
 constexpr int kChainStride = 64;
 constexpr int kChainF0 = 0, kChainF1 = 8, kChainXccB = 16, kChainXccC = 24, kChainF2 = 32;
+// (kTileAhead) rows b0 / b1 of the panel solved against the previous panel
+constexpr int kChainS0 = 40, kChainS1 = 48;
 // Swarms up to this size: the waiting work-groups of the update launch (two
 // per particle) hold fewer than its 512 resident slots (two per CU; the
 // three-per-CU variant has 768), so every producer they wait for finds a
@@ -190,6 +192,20 @@ constexpr int kTileFused = 8;
 // three work-groups per CU instead of two (more resident row tiles for
 // launches of more than 512 work-groups; the same bits).
 constexpr int kTileThreePerCU = 16;
+// Panel tile code bit (64 x 128 row tiles, not fused; ignored on the first
+// panel and after a fused one): the update launch of the panel also solves
+// the previous panel's rows -- each row tile its own rows before its update,
+// the role work-groups the panel's two diagonal row blocks first -- so the
+// previous panel needs no row-solve launch.  The update takes the previous
+// panel's 128 columns last, after waiting for the two diagonal row blocks
+// (flags kChainS0 / kChainS1), with the accumulators kept in registers: the
+// same operations in the same order, so the same bits.
+constexpr int kTileAhead = 32;
+// Panel tile code bit (64 x 128 row tiles, not fused; alone or with
+// kTileAhead): 8-deep chunks at four work-groups per CU (1024 resident
+// slots instead of 512: launches of a few hundred work-groups more than a
+// multiple of 512 no longer leave a nearly empty last round; the same bits).
+constexpr int kTileFourPerCU = 64;
 
 inline __host__ __device__ int ceil_div(int a, int b) { return (a + b - 1) / b; }
 inline __host__ __device__ int round_up(int a, int b) { return ceil_div(a, b) * b; }
@@ -230,6 +246,8 @@ void launch_update_trsm(const SwarmBuffers &b, int c1, hipStream_t st);
 void launch_panel_chain(const SwarmBuffers &b, int c0, hipStream_t st);
 void launch_solve_rows(const SwarmBuffers &b, int c0, hipStream_t st);
 bool panel_fused(const SwarmBuffers &b, int c0);
+bool panel_ahead(const SwarmBuffers &b, int c0);
+bool panel_rows_launch(const SwarmBuffers &b, int c0);
 void launch_likelihood(const SwarmBuffers &b, hipStream_t st);
 
 // Several swarms (training sets of different N) in one launch of the

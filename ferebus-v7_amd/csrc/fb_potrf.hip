@@ -242,11 +242,31 @@ __device__ inline void chain_publish_mem(int *flag, int tag)
 // The row solve of one 64-row tile at row0 below panel c0 of the matrix at
 // A: each wave solves its 16 rows alone (solve_rows16: solve with
 // L(b0, b0), the 64-deep update of the b1 part, solve with L(b1, b1);
-// dinv_s: the particle's 16x16 inverses).
+// dinv_s: the particle's 16x16 inverses).  WT: stored past the XCD's L2.
+template <bool WT = false>
 __device__ inline void solve_rows_tile(double *A, int np, int c0, int row0, const double *dinv_s)
 {
-    solve_rows16(A, np, c0, row0 + 16 * (threadIdx.x >> 6), dinv_s);
+    solve_rows16<WT>(A, np, c0, row0 + 16 * (threadIdx.x >> 6), dinv_s);
 }
+
+// Hook of an update that takes the previous panel's columns last
+// (kTileAhead): waits for the particle's flag(s) (rows that role
+// work-groups solved against the previous panel in this launch); a wait
+// that reaches its bound is recorded in INFO.
+struct FlagWait {
+    static constexpr bool kActive = true;
+    const int *f1, *f2;    // f2 may be null
+    int tag;
+    int *info_s;
+    int *ok;
+    unsigned long long limit;
+    __device__ void operator()() const
+    {
+        bool good = chain_wait(f1, tag, ok, limit);
+        if (f2) good = chain_wait(f2, tag, ok, limit) && good;
+        if (!good && threadIdx.x == 0) set_info<true>(info_s, kInfoWaitTimeout);
+    }
+};
 
 // LDS of the chained update launch: its row tiles, the 64 x 64 and 64 x 128
 // tiles of the roles and of the tail pre-update (whatever the row-tile
@@ -299,7 +319,12 @@ __host__ __device__ inline int tail_tiles(int np, int c0, int tail)
 // row tile then solves its rows (solve_rows_tile) -- the panel in one launch.
 // Role work-group b (< R ceil(S / 8)) of the chained update launch of panel
 // c0 for the S matrices at K (see panel_chain_kernel).
-template <int KC, bool TWO, bool FUSED>
+// AHEAD (kTileAhead, c0 >= 128): D and B (BC) first solve rows b0 and b1
+// against the previous panel (stored past the L2: every row tile of the
+// particle reads them) and publish S0 / S1; the updates that read rows
+// solved by another role take the previous panel's columns last, after its
+// flag.
+template <int KC, bool TWO, bool FUSED, bool AHEAD = false, int DIST = 2>
 __device__ __forceinline__ void chain_role(int b, double *__restrict__ K, int np, int n, int c0, int k0, int S,
                                            int *__restrict__ info, double *__restrict__ logdet_part,
                                            double *__restrict__ dinv, int *__restrict__ chain,
@@ -314,9 +339,18 @@ __device__ __forceinline__ void chain_role(int b, double *__restrict__ K, int np
     double *A = K + (size_t)s * np * np;
     int *cw = chain + (size_t)s * kChainStride;
     double *dinv0 = dinv + ((size_t)s * nbk + c0 / kNBI) * kDinv;
+    const int cp = c0 - kNBO;   // the previous panel (AHEAD)
+    if constexpr (AHEAD) {
+        // the role's own 64 rows against the previous panel (D: b0, B / BC:
+        // b1; C reads B's)
+        if (role <= 1) {
+            if (info[s] == 0) solve_rows_tile<true>(A, np, cp, role == 0 ? c0 : c1, dinv + (size_t)s * nbk * kDinv);
+            chain_publish_mem(cw + (role == 0 ? kChainS0 : kChainS1), tag);
+        }
+    }
     if (role == 0) {
         if (info[s] == 0) {
-            gemm_tile_update<kMB, kNBI, KC, 2, true>(A, np, c0, c0, k0, c0, smem);
+            gemm_tile_update<kMB, kNBI, KC, DIST, true>(A, np, c0, c0, k0, c0, smem);
             __syncthreads();
             double lg = 0.0;
             const int fail = potrf64_block<true, FUSED>(A, np, n, c0, &lg, smem, dinv0);
@@ -329,7 +363,11 @@ __device__ __forceinline__ void chain_role(int b, double *__restrict__ K, int np
     } else if (TWO) {
         // BC: the b1 row tile (b1, b0:b1), then the rest of the chain
         chain_announce(cw + kChainXccB, tag);
-        gemm_tile_update<kMB, kNBO, KC>(A, np, c1, c0, k0, c0, smem);
+        if constexpr (AHEAD)
+            gemm_tile_update<kMB, kNBO, KC, DIST, false, FlagWait>(
+                A, np, c1, c0, k0, c0, smem, cp, FlagWait{cw + kChainS0, nullptr, tag, info + s, &ok, wait_limit});
+        else
+            gemm_tile_update<kMB, kNBO, KC, DIST>(A, np, c1, c0, k0, c0, smem);
         if (!chain_wait(cw + kChainF0, tag, &ok, wait_limit)) {
             if (threadIdx.x == 0) set_info<FUSED>(info + s, kInfoWaitTimeout);
             if constexpr (FUSED) chain_publish_mem(cw + kChainF2, tag);
@@ -362,7 +400,11 @@ __device__ __forceinline__ void chain_role(int b, double *__restrict__ K, int np
         if constexpr (FUSED) chain_publish_mem(cw + kChainF2, tag);
     } else if (role == 1) {
         chain_announce(cw + kChainXccB, tag);
-        gemm_tile_update<kMB, kNBI, KC, 2, true>(A, np, c1, c0, k0, c0, smem);
+        if constexpr (AHEAD)
+            gemm_tile_update<kMB, kNBI, KC, DIST, true, FlagWait>(
+                A, np, c1, c0, k0, c0, smem, cp, FlagWait{cw + kChainS0, nullptr, tag, info + s, &ok, wait_limit});
+        else
+            gemm_tile_update<kMB, kNBI, KC, DIST, true>(A, np, c1, c0, k0, c0, smem);
         if (!chain_wait(cw + kChainF0, tag, &ok, wait_limit)) {
             if (threadIdx.x == 0) set_info<FUSED>(info + s, kInfoWaitTimeout);
         } else if (info[s] == 0) {
@@ -372,7 +414,11 @@ __device__ __forceinline__ void chain_role(int b, double *__restrict__ K, int np
         chain_publish(cw + kChainF1, cw + kChainXccC, tag);
     } else {
         chain_announce(cw + kChainXccC, tag);
-        gemm_tile_update<kMB, kNBI, KC, 2, true>(A, np, c1, c1, k0, c0, smem);
+        if constexpr (AHEAD)
+            gemm_tile_update<kMB, kNBI, KC, DIST, true, FlagWait>(
+                A, np, c1, c1, k0, c0, smem, cp, FlagWait{cw + kChainS1, nullptr, tag, info + s, &ok, wait_limit});
+        else
+            gemm_tile_update<kMB, kNBI, KC, DIST, true>(A, np, c1, c1, k0, c0, smem);
         if (!chain_wait(cw + kChainF1, tag, &ok, wait_limit)) {
             if (threadIdx.x == 0) set_info<FUSED>(info + s, kInfoWaitTimeout);
             if constexpr (FUSED) chain_publish_mem(cw + kChainF2, tag);
@@ -398,7 +444,10 @@ __device__ __forceinline__ void chain_role(int b, double *__restrict__ K, int np
 // Row tile lt (< 8 ceil(S rt CT / 8), XCD-aware order) of the chained update
 // launch of panel c0: the MB x NB update of the rows below the panel (and,
 // FUSED, their row solve once the panel's chain is published).
-template <int KC, int MB, int NB, bool FUSED>
+// AHEAD (64 x 128 tiles): the tile first solves its rows against the
+// previous panel, then updates with that panel's columns last, after the
+// flags of the two diagonal row blocks.
+template <int KC, int MB, int NB, bool FUSED, bool AHEAD = false, int DIST = 2>
 __device__ __forceinline__ void chain_rowtile(int lt, double *__restrict__ K, int np, int c0, int k0, int S,
                                               int *__restrict__ info, double *__restrict__ dinv,
                                               int *__restrict__ chain, unsigned long long wait_limit,
@@ -412,7 +461,17 @@ __device__ __forceinline__ void chain_rowtile(int lt, double *__restrict__ K, in
     if (info[s] != 0) return;
     double *A = K + (size_t)s * np * np;
     const int row0 = c0 + kNBO + (u / CT) * MB;
-    gemm_tile_update<MB, NB, KC>(A, np, row0, c0 + (u % CT) * NB, k0, c0, smem);
+    if constexpr (AHEAD) {
+        static_assert(MB == kNBI && NB == kNBO && !FUSED, "the ahead row solve needs 64 x 128 row tiles");
+        solve_rows_tile(A, np, c0 - kNBO, row0, dinv + (size_t)s * (np / kNBI) * kDinv);
+        __syncthreads();   // each wave solved 16 rows; the update reads all 64
+        const int *cw = chain + (size_t)s * kChainStride, tag = c0 / kNBO + 1;
+        gemm_tile_update<MB, NB, KC, DIST, false, FlagWait>(
+            A, np, row0, c0, k0, c0, smem, c0 - kNBO,
+            FlagWait{cw + kChainS0, cw + kChainS1, tag, info + s, &ok, wait_limit});
+        return;
+    }
+    gemm_tile_update<MB, NB, KC, DIST>(A, np, row0, c0 + (u % CT) * NB, k0, c0, smem);
     if constexpr (FUSED) {
         static_assert(MB == kNBI && NB == kNBO, "the fused row solve needs 64 x 128 row tiles");
         const int tag = c0 / kNBO + 1;
@@ -429,7 +488,7 @@ __device__ __forceinline__ void chain_rowtile(int lt, double *__restrict__ K, in
 // the chained update launch of panel c0: columns kt .. c0 applied to the
 // diagonal 64 x 64 block or a 64 x 128 row tile of one of the next `tail`
 // panels.
-template <int KC>
+template <int KC, int DIST = 2>
 __device__ __forceinline__ void chain_tailtile(int lt, double *__restrict__ K, int np, int c0, int kt, int tail,
                                                int S, const int *__restrict__ info, double *smem)
 {
@@ -446,13 +505,16 @@ __device__ __forceinline__ void chain_tailtile(int lt, double *__restrict__ K, i
     }
     if (info[s] != 0) return;
     double *A = K + (size_t)s * np * np;
-    if (v == 0) gemm_tile_update<kMB, kNBI, KC>(A, np, cp, cp, kt, c0, smem);
-    else gemm_tile_update<kMB, kNBO, KC>(A, np, cp + kNBI * v, cp, kt, c0, smem);
+    if (v == 0) gemm_tile_update<kMB, kNBI, KC, DIST>(A, np, cp, cp, kt, c0, smem);
+    else gemm_tile_update<kMB, kNBO, KC, DIST>(A, np, cp + kNBI * v, cp, kt, c0, smem);
 }
 
-template <int KC, int MB, int NB, bool TWO, bool FUSED>
+// WPE work-groups per CU (waves per SIMD): 2 (16-deep chunks), 3 or 4
+// (8-deep; at 4 the update loads run one chunk ahead instead of two, which
+// fits its registers, and the diagonal factorisation spills a little).
+template <int KC, int MB, int NB, bool TWO, bool FUSED, bool AHEAD, int WPE>
 __global__ void __launch_bounds__(kThreads)
-__attribute__((amdgpu_waves_per_eu(KC == 8 ? 3 : 2, KC == 8 ? 3 : 2))) panel_chain_kernel(double *__restrict__ K, int np, int n,
+__attribute__((amdgpu_waves_per_eu(WPE, WPE))) panel_chain_kernel(double *__restrict__ K, int np, int n,
                                                                        int c0, int k0, int kt, int tail, int S,
                                                                        int *__restrict__ info,
                                                                        double *__restrict__ logdet_part,
@@ -466,17 +528,19 @@ __attribute__((amdgpu_waves_per_eu(KC == 8 ? 3 : 2, KC == 8 ? 3 : 2))) panel_cha
     constexpr int R = chain_roles<TWO>();
     const int nrole = R * ceil_div(S, 8);
     if (b < nrole) {
-        chain_role<KC, TWO, FUSED>(b, K, np, n, c0, k0, S, info, logdet_part, dinv, chain, wait_limit, smem, ok);
+        chain_role<KC, TWO, FUSED, AHEAD, (WPE >= 4 ? 1 : 2)>(b, K, np, n, c0, k0, S, info, logdet_part, dinv, chain,
+                                                            wait_limit, smem, ok);
         return;
     }
     constexpr int CT = kNBO / NB;            // column tiles of the panel
     const int rt = (np - c0 - kNBO) / MB;    // row tiles below the panel
     const int nreg = 8 * ceil_div(S * rt * CT, 8);
     if (b - nrole < nreg) {
-        chain_rowtile<KC, MB, NB, FUSED>(b - nrole, K, np, c0, k0, S, info, dinv, chain, wait_limit, smem, ok);
+        chain_rowtile<KC, MB, NB, FUSED, AHEAD, (WPE >= 4 ? 1 : 2)>(b - nrole, K, np, c0, k0, S, info, dinv, chain,
+                                                                   wait_limit, smem, ok);
         return;
     }
-    chain_tailtile<KC>(b - nrole - nreg, K, np, c0, kt, tail, S, info, smem);
+    chain_tailtile<KC, (WPE >= 4 ? 1 : 2)>(b - nrole - nreg, K, np, c0, kt, tail, S, info, smem);
 }
 
 // Row-solve tile lt (< 8 ceil(S rt / 8), XCD-aware order) of panel c0.
@@ -561,20 +625,24 @@ solve_rows_multi_kernel(ChainMulti m)
     solve_rows_block(b - (g ? m.seg[g - 1].solve_end : 0), q.K, q.np, q.c0, q.S, q.info, q.dinv);
 }
 
-template <int KC, int MB, int NB, bool TWO, bool FUSED = false>
+template <int KC, int MB, int NB, bool TWO, bool FUSED = false, bool AHEAD = false, int WPE = (KC == 8 ? 3 : 2)>
 static void chain_launch(const SwarmBuffers &b, int c0, int k0, int kt, int tail, hipStream_t st)
 {
     const int tiles = ((b.np - c0 - kNBO) / MB) * (kNBO / NB);
     const int nrole = chain_roles<TWO>() * ceil_div(b.S, 8);
     const int grid = nrole + 8 * ceil_div(b.S * tiles, 8) + 8 * ceil_div(b.S * tail_tiles(b.np, c0, tail), 8);
-    hipLaunchKernelGGL((panel_chain_kernel<KC, MB, NB, TWO, FUSED>), dim3(grid), dim3(kThreads), 0, st, b.K, b.np, b.n, c0,
-                       k0, kt, tail, b.S, b.info, b.logdet_part, b.dinv, b.chain, b.wait_limit);
+    hipLaunchKernelGGL((panel_chain_kernel<KC, MB, NB, TWO, FUSED, AHEAD, WPE>), dim3(grid), dim3(kThreads), 0, st, b.K,
+                       b.np, b.n, c0, k0, kt, tail, b.S, b.info, b.logdet_part, b.dinv, b.chain, b.wait_limit);
 }
 
 template <bool TWO>
 static void chain_tiles(const SwarmBuffers &b, int tile, int c0, int k0, int kt, int tail, hipStream_t st)
 {
-    if (tile & kTileFused) chain_launch<16, 64, 128, TWO, true>(b, c0, k0, kt, tail, st);
+    if ((tile & kTileAhead) && (tile & kTileFourPerCU)) chain_launch<8, 64, 128, TWO, false, true, 4>(b, c0, k0, kt, tail, st);
+    else if ((tile & kTileAhead) && (tile & kTileThreePerCU)) chain_launch<8, 64, 128, TWO, false, true>(b, c0, k0, kt, tail, st);
+    else if (tile & kTileAhead) chain_launch<16, 64, 128, TWO, false, true>(b, c0, k0, kt, tail, st);
+    else if (tile & kTileFourPerCU) chain_launch<8, 64, 128, TWO, false, false, 4>(b, c0, k0, kt, tail, st);
+    else if (tile & kTileFused) chain_launch<16, 64, 128, TWO, true>(b, c0, k0, kt, tail, st);
     else if (tile & kTileThreePerCU) chain_launch<8, 64, 128, TWO>(b, c0, k0, kt, tail, st);
     else if (tile == 1) chain_launch<16, 32, 128, TWO>(b, c0, k0, kt, tail, st);
     else if (tile == 2) chain_launch<16, 32, 64, TWO>(b, c0, k0, kt, tail, st);
@@ -587,7 +655,22 @@ static void chain_tiles(const SwarmBuffers &b, int tile, int c0, int k0, int kt,
 // npan - 1 - tail_set pre-updates the last tail_set panels; with lookahead
 // every other launch pre-updates the next panel.  Each panel's columns are
 // applied in increasing k across launches, so the bits do not change.
-static void chain_schedule(int np, int tail_set, int lookahead, int p, int &k0, int &kt, int &tail)
+// A launch that also solves the previous panel's rows (panel_ahead) takes
+// no pre-updates: its tail tiles would read rows solved in the same launch.
+static int panel_code(const SwarmBuffers &b, int c0)
+{
+    return b.panel_tile ? b.panel_tile[c0 / kNBO] : b.tile;
+}
+
+bool panel_ahead(const SwarmBuffers &b, int c0)
+{
+    if (!b.chain || c0 < kNBO || c0 >= b.np) return false;
+    const int code = panel_code(b, c0);
+    return (code & kTileAhead) && (code & 3) == 0 && !(code & kTileFused) && !(panel_code(b, c0 - kNBO) & kTileFused);
+}
+
+static void chain_schedule(int np, int tail_set, int lookahead, int p, int &k0, int &kt, int &tail,
+                           const SwarmBuffers *b = nullptr)
 {
     const int npan = np / kNBO, q = npan - 1 - tail_set;
     int k[1024];
@@ -595,7 +678,8 @@ static void chain_schedule(int np, int tail_set, int lookahead, int p, int &k0, 
     for (int r = 0; r < np_; ++r) k[r] = 0;
     for (int r = 0; r <= p && r < np_; ++r) {
         int t = 0;
-        if (tail_set > 0 && q >= 1 && r == q) t = tail_set;
+        if (b && panel_ahead(*b, r * kNBO)) t = 0;
+        else if (tail_set > 0 && q >= 1 && r == q) t = tail_set;
         else if (lookahead && r >= 1 && r + 1 < npan) t = 1;
         if (r == p) {
             k0 = k[r];
@@ -612,10 +696,12 @@ static void chain_schedule(int np, int tail_set, int lookahead, int p, int &k0, 
 // select 64x128 variants.
 void launch_panel_chain(const SwarmBuffers &b, int c0, hipStream_t st)
 {
-    const int code = b.panel_tile ? b.panel_tile[c0 / kNBO] : b.tile;
+    const int code = panel_code(b, c0);
     int k0 = 0, kt = 0, tail = 0;
-    chain_schedule(b.np, b.tail, b.lookahead, c0 / kNBO, k0, kt, tail);
-    const int tile = (code & kTileFused) ? kTileFused : (code & kTileThreePerCU) ? kTileThreePerCU : (code & 3);
+    chain_schedule(b.np, b.tail, b.lookahead, c0 / kNBO, k0, kt, tail, &b);
+    const int cu = code & (kTileThreePerCU | kTileFourPerCU);
+    const int tile = panel_ahead(b, c0) ? kTileAhead | cu
+                     : (code & kTileFused) ? kTileFused : cu ? cu : (code & 3);
     if (code & kTileTwoRoles) chain_tiles<true>(b, tile, c0, k0, kt, tail, st);
     else chain_tiles<false>(b, tile, c0, k0, kt, tail, st);
 }
@@ -624,8 +710,14 @@ void launch_panel_chain(const SwarmBuffers &b, int c0, hipStream_t st)
 // launch for it.
 bool panel_fused(const SwarmBuffers &b, int c0)
 {
-    const int code = b.panel_tile ? b.panel_tile[c0 / kNBO] : b.tile;
-    return b.chain && (code & kTileFused);
+    return b.chain && (panel_code(b, c0) & kTileFused);
+}
+
+// The rows below panel c0 need their own solve launch: neither the panel's
+// update launch (fused) nor the next panel's (ahead) solves them.
+bool panel_rows_launch(const SwarmBuffers &b, int c0)
+{
+    return c0 + kNBO < b.np && !panel_fused(b, c0) && !panel_ahead(b, c0 + kNBO);
 }
 
 void launch_solve_rows(const SwarmBuffers &b, int c0, hipStream_t st)

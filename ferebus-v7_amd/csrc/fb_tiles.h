@@ -60,11 +60,21 @@ struct TileShape {
 // is staged, so the inner loop passes LDS reads straight to the matrix cores
 // (the products see the same operands either way).
 // ZERO_C: C starts at zero (not loaded); PLUS: C += A B^T instead of -=.
-template <int MB, int NB, int KC, int DIST = 2, bool TO_LDS = false, bool ZERO_C = false, bool PLUS = false>
+// hook (Hook::kActive): called by every thread once, right before the
+// global loads of chunk hook_chunk (0 <= hook_chunk < depth / KC) are
+// issued -- a wait for operands that another work-group produces while this
+// one accumulates the earlier chunks; the accumulators stay in registers.
+struct NoHook {
+    static constexpr bool kActive = false;
+    __device__ void operator()() const {}
+};
+
+template <int MB, int NB, int KC, int DIST = 2, bool TO_LDS = false, bool ZERO_C = false, bool PLUS = false,
+          class Hook = NoHook>
 __device__ inline void tile_gemm_sub(double *__restrict__ Cg, int ldc,
                                      const double *__restrict__ Ag, int lda,
                                      const double *__restrict__ Bg, int ldb, int depth,
-                                     double *__restrict__ smem)
+                                     double *__restrict__ smem, int hook_chunk = -1, const Hook &hook = Hook())
 {
     using T = TileShape<MB, NB, KC>;
     constexpr int NA = MB * KC / 2 / kThreads;   // double2 loads per thread (A)
@@ -110,6 +120,9 @@ __device__ inline void tile_gemm_sub(double *__restrict__ Cg, int ldc,
         double r1ax[DIST > 1 ? NA : 1], r1ay[DIST > 1 ? NA : 1], r1bx[DIST > 1 ? NBL : 1], r1by[DIST > 1 ? NBL : 1];
 #define FB_LOAD(SET, CH)                                                                        \
         do {                                                                                    \
+            if constexpr (Hook::kActive) {                                                      \
+                if ((CH) == hook_chunk) hook();                                                 \
+            }                                                                                   \
             const size_t ka = (size_t)(CH) * KC * lda, kb = (size_t)(CH) * KC * ldb;            \
             _Pragma("unroll") for (int u = 0; u < NA; ++u) {                                    \
                 const double2 t = *reinterpret_cast<const double2 *>(Ag + ka + a_off[u]);      \
@@ -209,13 +222,15 @@ __device__ inline void tile_gemm_sub(double *__restrict__ Cg, int ldc,
 }
 
 // C(row0:row0+MB, col0:col0+NB) -= L(row0.., k0:k1) L(col0.., k0:k1)^T
-// for the matrix at A (trailing update of one tile).
-template <int MB, int NB, int KC = kKC, int DIST = 2, bool TO_LDS = false>
+// for the matrix at A (trailing update of one tile); hook: called before
+// the loads of columns kh .. (k0 <= kh < k1, kh - k0 a multiple of KC).
+template <int MB, int NB, int KC = kKC, int DIST = 2, bool TO_LDS = false, class Hook = NoHook>
 __device__ inline void gemm_tile_update(double *__restrict__ A, int np, int row0, int col0, int k0, int k1,
-                                        double *__restrict__ smem)
+                                        double *__restrict__ smem, int kh = -1, const Hook &hook = Hook())
 {
-    tile_gemm_sub<MB, NB, KC, DIST, TO_LDS>(A + (size_t)col0 * np + row0, np, A + (size_t)k0 * np + row0, np,
-                              A + (size_t)k0 * np + col0, np, k1 - k0, smem);
+    tile_gemm_sub<MB, NB, KC, DIST, TO_LDS, false, false, Hook>(
+        A + (size_t)col0 * np + row0, np, A + (size_t)k0 * np + row0, np, A + (size_t)k0 * np + col0, np, k1 - k0,
+        smem, Hook::kActive ? (kh - k0) / KC : -1, hook);
 }
 
 constexpr int kPotrfSmem = kNBI * kPLD + kNBI + 8;   // doubles: block, 1 / L_ii, flag, log sum
@@ -665,6 +680,8 @@ __device__ inline void trsm64_solve(const double *Cin, int ldi, double *Ct, int 
 // A operand -X0 taken from the solve's registers: an MFMA result holds four
 // consecutive k-steps of an operand.  Each element sees the operations of
 // the 64-row tile routines in the same order, so the bits are theirs.
+// WT: the results are stored past the XCD's L2 (st_f64<true>).
+template <bool WT = false>
 __device__ inline void solve_rows16(double *A, int np, int c0, int row, const double *dinv_s)
 {
     const int lane = threadIdx.x & 63;
@@ -681,7 +698,7 @@ __device__ inline void solve_rows16(double *A, int np, int c0, int row, const do
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) C0[(size_t)(16 * j + 4 * v + lq) * np] = x0[j][v];
+        for (int v = 0; v < 4; ++v) st_f64<WT>(C0 + (size_t)(16 * j + 4 * v + lq) * np, x0[j][v]);
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
@@ -698,7 +715,7 @@ __device__ inline void solve_rows16(double *A, int np, int c0, int row, const do
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) C1[(size_t)(16 * j + 4 * v + lq) * np] = x1[j][v];
+        for (int v = 0; v < 4; ++v) st_f64<WT>(C1 + (size_t)(16 * j + 4 * v + lq) * np, x1[j][v]);
 }
 
 // Ls -= X X^T on the lower 16x16 blocks of a 64x64 block on chip

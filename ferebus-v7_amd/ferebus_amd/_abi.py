@@ -22,7 +22,8 @@ ERRORS = {-1: "FB_E_ARG", -2: "FB_E_SPEC", -3: "FB_E_HIP", -4: "FB_E_NOMEM",
 EXPORTS = (
     "fb_create", "fb_destroy", "fb_nparams", "fb_set_stream", "fb_swarm_loglik",
     "fb_swarm_loglik_async", "fb_particle_loglik", "fb_get_alpha", "fb_swarm_loglik_grad",
-    "fb_profile_enable", "fb_profile_read", "fb_factor_path", "fb_device_name", "fb_last_error",
+    "fb_profile_enable", "fb_profile_read", "fb_factor_path", "fb_factor_groups", "fb_device_name",
+    "fb_last_error",
     "fb_multi_create", "fb_multi_swarm_loglik", "fb_multi_destroy", "fb_multi_schedule",
     "fb_multi_pso_step",
     "fb_pso_create", "fb_pso_destroy", "fb_pso_init", "fb_pso_step", "fb_pso_retry", "fb_pso_select",
@@ -81,6 +82,7 @@ def load() -> ctypes.CDLL:
         "fb_debug_matrix": (ctypes.c_int, [vp, _P_DBL, ctypes.c_int, _P_DBL, _P_INT]),
         "fb_profile_enable": (ctypes.c_int, [vp, ctypes.c_int]),
         "fb_factor_path": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+        "fb_factor_groups": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
         "fb_profile_read": (ctypes.c_int, [vp, ctypes.c_int, _P_DBL, ctypes.POINTER(ctypes.c_longlong)]),
         "fb_pso_create": (ctypes.c_int, [ctypes.POINTER(vp), vp, ctypes.c_int, ctypes.c_double,
                                          ctypes.c_double, ctypes.c_double]),

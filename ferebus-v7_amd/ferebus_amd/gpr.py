@@ -21,7 +21,7 @@ LIKELIHOODS = {"marginal": 0, "concentrated": 1}
 # first diagonal block, "solve_factor" = panel solve + second diagonal block,
 # "update_solve" = second block's update + solve
 PROFILE_KINDS = {"build": 0, "update": 1, "solve_factor": 2, "update_solve": 3, "likelihood": 4, "swarm": 5,
-                 "factor": 6, "tune_evals": 7}
+                 "factor": 6, "tune_evals": 7, "factor_span": 8}
 
 
 class SwarmLikelihood:
@@ -170,7 +170,16 @@ class SwarmLikelihood:
         _abi.check(self._lib.fb_factor_path(self._h, int(s), ctypes.byref(p)), "fb_factor_path")
         return self.FACTOR_PATHS[p.value]
 
-    def profile(self, enable: bool) -> None:
+    def factor_groups(self, s: int) -> int:
+        """Stream groups of the lock-step factorisation for a swarm of s (see
+        fb_factor_groups)."""
+        g = ctypes.c_int(1)
+        _abi.check(self._lib.fb_factor_groups(self._h, int(s), ctypes.byref(g)), "fb_factor_groups")
+        return g.value
+
+    def profile(self, enable) -> None:
+        """True / 1: an event pair around every launch; 2: around the phases
+        only (build, factorisation span, likelihood); False / 0: off."""
         _abi.check(self._lib.fb_profile_enable(self._h, int(enable)), "fb_profile_enable")
 
     def profile_read(self, kind: str) -> Tuple[float, int]:

@@ -29,7 +29,8 @@ module ferebus_hip
             fb_multi_create, fb_multi_swarm_loglik, fb_multi_destroy, fb_multi_schedule, fb_multi_pso_step, &
             fb_pso_create, fb_pso_destroy, fb_pso_init, fb_pso_step, fb_pso_retry, fb_pso_select, &
             fb_pso_set_gbest, fb_pso_positions, fb_pso_set_coefficients, &
-            fb_profile_enable, fb_profile_read, fb_factor_path, fb_device_name, fb_last_error, fb_error_message
+            fb_profile_enable, fb_profile_read, fb_factor_path, fb_factor_groups, fb_device_name, fb_last_error, &
+            fb_error_message
 
   interface
     integer(c_int) function fb_create(h, n, d, x, y, mean, spec, noise, optimise_noise, &
@@ -230,6 +231,13 @@ module ferebus_hip
       integer(c_int), value :: s
       integer(c_int), intent(out) :: path
     end function fb_factor_path
+
+    integer(c_int) function fb_factor_groups(h, s, groups) bind(c, name="fb_factor_groups")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: s
+      integer(c_int), intent(out) :: groups
+    end function fb_factor_groups
 
     integer(c_int) function fb_device_name(buf, len) bind(c, name="fb_device_name")
       import :: c_int, c_char

@@ -271,8 +271,10 @@ int fb_pso_positions(fb_pso *p, double *x, double *v);
 int fb_multi_pso_step(fb_multi *m, fb_pso *const *ps, const double *const *r1, const double *const *r2,
                       double *const *cost, int *const *info);
 
-/* Kernel timing: when enabled, each launch of the handle is bracketed by HIP
- * events and the per-kernel totals accumulate until read.
+/* Kernel timing: when enabled (1), each launch of the handle is bracketed by
+ * HIP events and the per-kernel totals accumulate until read; enable = 2
+ * brackets only the phases (kinds 0, 4, 5, 6, 8: no events between the
+ * factorisation's launches).
  * kind: 0 covariance build; lock-step factorisation, per 128-column panel:
  *       1 trailing update + first diagonal block, 2 panel solve + second
  *       diagonal block, 3 second block's update + solve; 4 likelihood
@@ -281,7 +283,10 @@ int fb_multi_pso_step(fb_multi *m, fb_pso *const *ps, const double *const *r1, c
  *       has timed on this handle so far (counted whether or not profiling
  *       is enabled, not reset; *total_ms = 0).  fb_pso_retry never tunes:
  *       its swarms of re-randomised particles take the nearest tuned size's
- *       path. */
+ *       path; 8 lock-step factorisation span, from the end of the build to
+ *       the start of the likelihood (with stream groups, the launches of
+ *       different groups overlap inside it, and kinds 1-3 add up durations
+ *       on several streams). */
 int fb_profile_enable(fb_handle *h, int enable);
 int fb_profile_read(fb_handle *h, int kind, double *total_ms, long long *launches);
 
@@ -304,15 +309,26 @@ int fb_profile_read(fb_handle *h, int kind, double *total_ms, long long *launche
  * chained update launch, 8 and 12 the 64x128 shape with three or two such
  * work-groups and the row solve fused into the update launch, 16 and 20 the
  * 64x128 shape with three or two such work-groups in 8-deep chunks, three
- * work-groups per CU instead of two) pins a
+ * work-groups per CU instead of two; + 32 on a 64x128 code without 8: the
+ * update launch also solves the previous panel's rows; 64 and 68 (+ 32) the
+ * 64x128 shape in 8-deep chunks at four work-groups per CU) pins a
  * per-panel mix; FB_CHAIN=0 gives three launches per panel instead;
  * FB_TAIL=n sets how many trailing panels the chained schedule pre-updates
  * in the launch before them (default 3, 0 off); FB_PERSISTENT=0/1 pins path
  * 1/0;
  * FB_AUTOTUNE=0 keeps path 0 without timing; FB_TUNE_CACHE=<file> keeps the
  * choices across processes (JSON records keyed on device, padded order and
- * swarm size, with the per-panel shapes). */
+ * swarm size, with the per-panel shapes and the stream groups). */
 int fb_factor_path(const fb_handle *h, int s, int *path);
+
+/* Stream groups of the lock-step paths for a swarm of s: the swarm is built
+ * once, its particles' factorisations run as `groups` groups on concurrent
+ * streams (a launch of one group that leaves the chip partly idle in its
+ * last round overlaps with the others'), then one likelihood launch.  The
+ * autotuner times 1 .. 4 groups with the path it chose; FB_STREAMS=n (read
+ * at fb_create) pins n; 1 before the first call for s.  The bits do not
+ * depend on it.  No reference counterpart (introspection of the schedule). */
+int fb_factor_groups(const fb_handle *h, int s, int *groups);
 
 /* Library-wide information: device name into buf (len bytes), 0 on success. */
 int fb_device_name(char *buf, int len);

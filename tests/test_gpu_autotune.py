@@ -129,6 +129,12 @@ def _dup_case(n, dup, src, seed):
     (700, 300, 17, 9, 12),
     (700, 200, 5, 9, 16),    # 8-deep chunks, three work-groups per CU
     (700, 300, 17, 9, 20),
+    (700, 200, 5, 9, 32),    # the previous panel's rows solved in the update launch
+    (700, 300, 17, 9, 36),   # (three and two roles, 16- and 8-deep chunks)
+    (700, 200, 5, 9, 48),
+    (700, 300, 17, 9, 52),
+    (700, 200, 5, 9, 64),    # 8-deep chunks, four work-groups per CU (alone and ahead)
+    (700, 300, 17, 9, 100),
 ])
 def test_chained_schedule_bit_identical(gpu, monkeypatch, n, dup, src, S, code):
     """Two launches per panel (the diagonal chain handed over inside the
@@ -186,6 +192,39 @@ def test_chained_schedule_tile_shapes(gpu, monkeypatch, path, tiles):
     assert np.array_equal(out[0][1], out[1][1]) and np.array_equal(out[0][0], out[1][0])
 
 
+@pytest.mark.parametrize("n,tiles,tail", [(900, "32,32,32,32,32,32,32,32", 0),
+                                          (900, "36,52,48,32,36,48,52,32", 0),
+                                          (900, "8,32,0,32,16,48,2,32", 0),
+                                          (1100, "0,32,32,32,0,32,32,32,36", 3),
+                                          (1100, "0,32,32,32,32,32,32,32,32", 3),
+                                          (1100, "32,32,4,32,48,0,32,52,32", 2),
+                                          (250, "32,32,32", 0),
+                                          (900, "64,96,100,68,64,96,4,100", 0),
+                                          (1100, "64,64,96,64,64,96,96,68,96", 3)])
+def test_ahead_row_solve_bits(gpu, monkeypatch, n, tiles, tail):
+    """Update launches that also solve the previous panel's rows (tile code
+    + 32, alone, with two role work-groups (+ 4) or 8-deep chunks at three
+    or four work-groups per CU (+ 16, + 64); on the first panel and after a
+    fused one (+ 8) the bit is ignored), mixed with the other variants, after tail pre-updates (panels 5-8 of 9 start
+    from the pre-updated columns) and in the launch that would carry them
+    (then it carries none): the bits of the three-launch schedule."""
+    _, x, y, prog, mfit = _problem(n, 6, seed=n + tail)
+    out = {}
+    for chain in (0, 1):
+        monkeypatch.setenv("FB_CHAIN", str(chain))
+        monkeypatch.setenv("FB_TAIL", str(tail))
+        monkeypatch.setenv("FB_FACTOR_PATH", "5")
+        monkeypatch.setenv("FB_PANEL_TILES", tiles if chain else ",".join(str(int(t) & 3) for t in tiles.split(",")))
+        gp = _gp(x, y, prog, mfit, 9)
+        th = np.random.default_rng(n).uniform(0.0, 1.0, (9, gp.nparams))
+        out[chain] = gp.swarm_loglik(th)
+        # the graph replay of the same evaluation
+        out[chain + 2] = gp.swarm_loglik(th)
+    print("info", out[1][1].tolist())
+    for k in (1, 2, 3):
+        assert np.array_equal(out[0][1], out[k][1]) and np.array_equal(out[0][0], out[k][0], equal_nan=True)
+
+
 @pytest.mark.parametrize("tail,lookahead", [(1, 0), (5, 0), (7, 0), (0, 1), (3, 1), (7, 1)])
 def test_tail_pre_update_bits(gpu, monkeypatch, tail, lookahead):
     """The chained schedule with 1, 5 and 7 trailing panels pre-updated by the
@@ -203,6 +242,49 @@ def test_tail_pre_update_bits(gpu, monkeypatch, tail, lookahead):
         th = np.random.default_rng(6).uniform(0.0, 1.0, (5, gp.nparams))
         out[chain] = gp.swarm_loglik(th)
     assert np.array_equal(out[0][1], out[1][1]) and np.array_equal(out[0][0], out[1][0])
+
+
+def test_stream_groups_tuned_and_bits(gpu, monkeypatch, tmp_path):
+    """The autotuner also picks how many stream groups the swarm's
+    factorisations run as (1-4, after the path); the tuning cache keeps the
+    choice; every group count gives the bits of one group (graph replay
+    included); and a swarm above 160 particles split into groups of at most
+    160 runs the chained schedule in each, with the bits of single-particle
+    evaluations."""
+    import json
+    for k in ("FB_FACTOR_PATH", "FB_PERSISTENT", "FB_AUTOTUNE", "FB_STREAMS"):
+        monkeypatch.delenv(k, raising=False)
+    cache = tmp_path / "tune.json"
+    monkeypatch.setenv("FB_TUNE_CACHE", str(cache))
+    rng, x, y, prog, mfit = _problem(1200, 6, seed=31)
+    gp = _gp(x, y, prog, mfit, 24)
+    th = rng.uniform(0.0, 1.0, (24, gp.nparams))
+    assert gp.factor_groups(24) == 1
+    ll, info = gp.swarm_loglik(th)
+    G = gp.factor_groups(24)
+    print("tuned:", gp.factor_path(24), "groups", G)
+    assert 1 <= G <= 4
+    recs = json.loads(cache.read_text())
+    assert any(r["swarm"] == 24 and r["groups"] == G for r in recs)
+    gp2 = _gp(x, y, prog, mfit, 24)
+    assert gp2.factor_groups(24) == G and gp2.factor_path(24) == gp.factor_path(24)
+    monkeypatch.delenv("FB_TUNE_CACHE")
+    for streams in (1, 2, 3, 4):
+        monkeypatch.setenv("FB_STREAMS", str(streams))
+        monkeypatch.setenv("FB_FACTOR_PATH", "2")
+        g = _gp(x, y, prog, mfit, 24)
+        for _ in range(2):                     # eager, then the graph replay
+            l, i = g.swarm_loglik(th)
+            assert g.factor_groups(24) == streams
+            assert np.array_equal(i, info) and np.array_equal(l, ll, equal_nan=True), f"{streams} groups"
+        g.close()
+    monkeypatch.setenv("FB_STREAMS", "2")
+    _, x3, y3, prog3, mfit3 = _problem(300, 6, seed=11)
+    g = _gp(x3, y3, prog3, mfit3, 170)
+    th3 = np.random.default_rng(7).uniform(0.0, 1.0, (170, g.nparams))
+    l3, i3 = g.swarm_loglik(th3)
+    pl, pinfo = g.particle_loglik(th3[::17])
+    assert np.array_equal(pinfo, i3[::17]) and np.array_equal(pl[pinfo == 0], l3[::17][pinfo == 0])
 
 
 @pytest.mark.parametrize("S", [160, 170])
