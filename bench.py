@@ -31,10 +31,12 @@ sys.path.insert(0, str(REPO))
 
 # committed rocprofv3 summaries of this workload, per factorisation path
 # (lock-step paths: the three-launch schedule, FB_CHAIN=0; CHAINED_PROFILES:
-# the default two-launch schedule)
+# the default two-launch schedule, keyed on (path, stream groups))
 CHAINED_PROFILES = {
-    "lockstep-kc16-per-panel": (REPO / "profiles" / "chained_factor_per_panel_methanol_n2000_s50.md",
-                                "lockstep_factor"),
+    ("lockstep-kc16-per-panel", 1): (REPO / "profiles" / "chained_factor_per_panel_methanol_n2000_s50.md",
+                                     "lockstep_factor"),
+    ("lockstep-kc16", 2): (REPO / "profiles" / "chained_factor_groups2_methanol_n2000_s50.md",
+                           "lockstep_factor"),
 }
 PROFILES = {
     "persistent": (REPO / "profiles" / "dag_factor_kernel_methanol_n2000_s50.md", "dag_factor_kernel"),
@@ -46,14 +48,14 @@ PROFILES = {
 }
 
 
-def profiled_traffic(path: str, chained: bool = False):
+def profiled_traffic(path: str, chained: bool = False, groups: int = 1):
     """(HBM bytes per factorisation, source) from the committed rocprofv3
     counter summary of that path (FETCH_SIZE + WRITE_SIZE, copy-kernel
     calibrated), or (None, None)."""
-    table = CHAINED_PROFILES if chained else PROFILES
-    if path not in table:
+    table, key_ = (CHAINED_PROFILES, (path, groups)) if chained else (PROFILES, path)
+    if key_ not in table:
         return None, None
-    f, key = table[path]
+    f, key = table[key_]
     try:
         for line in f.read_text().splitlines():
             if line.startswith(f"traffic_bytes_per_launch {key}:"):
@@ -254,8 +256,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n", type=int, default=2000)
-    ap.add_argument("--swarm", type=int, default=50)
+    ap.add_argument("--workload", choices=("methanol", "ibuprofen"), default="methanol",
+                    help="methanol: the headline workload (N=2000, d=12, swarm 50); ibuprofen: N=6000, d=93, "
+                         "linear mean, concentrated likelihood, swarm 64")
+    ap.add_argument("--n", type=int, default=None, help="training points (default 2000 / 6000)")
+    ap.add_argument("--swarm", type=int, default=None, help="particles (default 50 / 64)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="bound on the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -287,7 +292,12 @@ def main():
         print(json.dumps({"copy_bytes": args.copy_bytes, "copies": 5}))
         return
 
-    w = synthetic.methanol(n=args.n, seed=0)
+    ibu = args.workload == "ibuprofen"
+    if args.n is None:
+        args.n = 6000 if ibu else 2000
+    if args.swarm is None:
+        args.swarm = 64 if ibu else 50
+    w = synthetic.ibuprofen(n=args.n, seed=4) if ibu else synthetic.methanol(n=args.n, seed=0)
     d = w.x.shape[1]
     prog = compile_kernel(w.kernel, w.kernels, d)
     mfit = fit_mean(w.mean, w.x, w.y)
@@ -360,14 +370,14 @@ def main():
                        "duration = the factorisation's span")
     fact_flops = S * potrf_flops(n)
     factor_path = gp.factor_path(S)
-    traffic, traffic_src = (profiled_traffic(factor_path, chained) if (n == 2000 and S == 50)
-                            else (None, None))
+    traffic, traffic_src = (profiled_traffic(factor_path, chained, groups)
+                            if (n == 2000 and S == 50 and not ibu) else (None, None))
     build_ms = span["build"][0] / prof_steps
     build_bytes = S * 8.0 * n * (n + 1) / 2.0 + 8.0 * n * d
     achieved = fact_flops / (fact_ms * 1e-3) / 1e12
 
     result = {
-        "metric": "logL evals/sec at N=2000 d=12 swarm 50 (FP64)",
+        "metric": f"logL evals/sec at N={n} d={d} swarm {S} (FP64)",
         "value": round(value, 2),
         "unit": "logL evals/s",
         "n_gpus": 1,
@@ -378,9 +388,11 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded methanol-like ALF features, Gaussian-bump target)",
-        "config": {"workload": f"methanol-like N={n} d={d} swarm={S} rbf(9)+periodic(3) "
-                               f"constant mean, marginal, nugget 1e-10, theta~U[0,1)",
+        "data": f"synthetic (seeded {w.name} ALF features, Gaussian-bump target)",
+        "config": {"workload": (f"ibuprofen-like N={n} d={d} swarm={S} rbf(63)+periodic(30) linear mean, "
+                                f"concentrated, nugget 1e-10, theta~U[0,1)") if ibu else
+                               (f"methanol-like N={n} d={d} swarm={S} rbf(9)+periodic(3) "
+                                f"constant mean, marginal, nugget 1e-10, theta~U[0,1)"),
                    "path": "batched (hipGraph replay per swarm evaluation, "
                            + ("persistent factorisation)" if persistent else "lock-step factorisation)"),
                    "factor_path": factor_path,
