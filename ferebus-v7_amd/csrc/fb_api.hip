@@ -161,6 +161,8 @@ struct EventPair {
 // chained, of their role, fused and three-per-CU variants (measured per
 // panel: reachable by tuning or FB_PANEL_TILES)
 constexpr int kNumModes = 5;
+// zeros after the slot values: np - n < 130 columns past the last slot row
+constexpr size_t kSlotPad = 256;
 constexpr int kModePerPanel = 5;
 
 struct fb_handle {
@@ -242,6 +244,9 @@ fb::SwarmBuffers buffers(fb_handle *h, int g0, int S, const double *theta, doubl
     b.n_slots = h->n_slots; b.n_leaves = h->prog.n_leaves;
     b.slots = h->d_slots; b.RHS = h->d_rhs;
     b.hyp = h->d_hyp + (size_t)g0 * (h->prog.n_par_kernel + fb::kMaxTerms + 1);
+    // the transposed hyperparameter table follows the records of max_swarm particles
+    b.hypT = h->d_hyp + (size_t)h->max_swarm * (h->prog.n_par_kernel + fb::kMaxTerms + 1);
+    b.ldT = h->max_swarm + 16;
     b.theta = theta + (size_t)g0 * h->prog.n_par; b.prog = h->d_prog; b.prog_host = &h->prog;
     b.gemm_kc = h->gemm_kc;
     b.tile = h->tile;
@@ -655,10 +660,12 @@ int fb_create(fb_handle **out, int n, int d, const double *x, const double *y, c
         }                                                                                 \
     } while (0)
     FB_ALLOC(h->d_prog, sizeof(KernelProgram));
-    FB_ALLOC(h->d_slots, sizeof(double) * slots.size());
+    // slots padded with kSlotPad zeros: the build reads columns up to np - 1
+    // of every slot row without a clamp
+    FB_ALLOC(h->d_slots, sizeof(double) * (slots.size() + kSlotPad));
     FB_ALLOC(h->d_rhs, sizeof(double) * rhs.size());
     FB_ALLOC(h->d_theta, sizeof(double) * std::max(h->prog.n_par, 1) * S);
-    FB_ALLOC(h->d_hyp, sizeof(double) * hs * S);
+    FB_ALLOC(h->d_hyp, sizeof(double) * hs * (2 * S + 16));   // records, then their transpose
     FB_ALLOC(h->d_chain, sizeof(int) * fb::kChainStride * S);
     FB_ALLOC(h->d_K, sizeof(double) * np2 * S);
     FB_ALLOC(h->d_logdet, sizeof(double) * h->nsteps * S);
@@ -771,6 +778,7 @@ int fb_create(fb_handle **out, int n, int d, const double *x, const double *y, c
     }
     hipError_t e = hipMemcpy(h->d_prog, &h->prog, sizeof(KernelProgram), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(h->d_slots, slots.data(), sizeof(double) * slots.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(h->d_slots + slots.size(), 0, sizeof(double) * kSlotPad);
     if (e == hipSuccess) e = hipMemcpy(h->d_rhs, rhs.data(), sizeof(double) * rhs.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(h->d_info, 0, sizeof(int) * S);
     if (e == hipSuccess) e = hipMemset(h->d_ctl, 0, sizeof(int) * fb::kDagCtl);
@@ -1299,7 +1307,7 @@ int fb_swarm_loglik_static(fb_handle *h, int s, const double *d_theta, double *d
     const size_t S = (size_t)s, np2 = (size_t)h->np * h->np;
     const size_t hs = (size_t)h->prog.n_par_kernel + fb::kMaxTerms + 1;
     const size_t off_hyp = 0;
-    const size_t off_chain = align256(off_hyp + sizeof(double) * hs * S);
+    const size_t off_chain = align256(off_hyp + sizeof(double) * hs * (2 * S + 16));   // records, transpose
     const size_t off_K = align256(off_chain + sizeof(int) * fb::kChainStride * S);
     const size_t off_logdet = align256(off_K + sizeof(double) * np2 * S);
     const size_t off_dinv = align256(off_logdet + sizeof(double) * h->nsteps * S);
@@ -1320,6 +1328,8 @@ int fb_swarm_loglik_static(fb_handle *h, int s, const double *d_theta, double *d
     b.slots = h->d_slots; b.RHS = h->d_rhs; b.prog = h->d_prog; b.prog_host = &h->prog;
     b.theta = d_theta; b.ll = d_ll; b.info = d_info;
     b.hyp = reinterpret_cast<double *>(w + off_hyp);
+    b.hypT = b.hyp + hs * S;
+    b.ldT = s + 16;
     b.K = reinterpret_cast<double *>(w + off_K);
     b.logdet_part = reinterpret_cast<double *>(w + off_logdet);
     b.dinv = reinterpret_cast<double *>(w + off_dinv);

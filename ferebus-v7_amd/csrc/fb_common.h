@@ -218,6 +218,8 @@ struct SwarmBuffers {
     const double *slots;   // n_slots x n
     const double *RHS;     // nrhs x n
     double *hyp;           // S x hyper record (see fb_build.hip)
+    double *hypT = nullptr;   // the same records transposed: entry k of particle s at hypT[k * ldT + s],
+    int ldT = 0;              // particles S .. S + 15 repeating particle S - 1 (ldT >= S + 16)
     const double *theta;   // n_par x S
     const KernelProgram *prog;   // device copy
     const KernelProgram *prog_host;   // host copy (passed by value to kernels whose loops read it)
