@@ -230,6 +230,7 @@ struct fb_handle {
     double prof_ms[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     long long prof_n[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     long long tune_evals = 0;        // evaluations the autotuner has timed (read as profile kind 7)
+    long long dispatch_fallbacks = 0;   // persistent evaluations re-run on lock-step (profile kind 9)
 };
 
 namespace {
@@ -1124,6 +1125,8 @@ int fb_swarm_loglik(fb_handle *h, int s, const double *theta, double *ll, int *i
     if (err == 2) {
         // some XCD queue found no work-group of its own: this device does not
         // dispatch the way the persistent schedule assumes; use lock-step
+        // (counted: profile kind 9)
+        ++h->dispatch_fallbacks;
         h->forced_mode = 1;
         set_mode(h, 1);
         drop_graphs(h);
@@ -1240,6 +1243,7 @@ int fb_swarm_loglik_grad(fb_handle *h, int s, const double *theta, double *ll, d
     if (rc) return rc;
     if (err == 1) return fail(FB_E_HIP, "fb_swarm_loglik_grad: the persistent factorisation found no ready task for 2 s");
     if (err == 2) {
+        ++h->dispatch_fallbacks;
         h->forced_mode = 1;
         set_mode(h, 1);
         drop_graphs(h);
@@ -1732,6 +1736,7 @@ int fb_multi_swarm_loglik(fb_multi *m, const int *s, const double *const *theta,
     if (mode == kMultiPersistent && *m->h_err == 1)
         return fail(FB_E_HIP, "fb_multi_swarm_loglik: the persistent factorisation found no ready task for 2 s");
     if (mode == kMultiPersistent && *m->h_err == 2) {
+        for (int g = 0; g < G; ++g) ++m->gps[g]->dispatch_fallbacks;
         m->schedule[sv] = kMultiSequential;
         rc = multi_run(m, sv, kMultiSequential);   // some XCD queue was not served: lock-step path
         if (rc) return rc;
@@ -1835,6 +1840,7 @@ int pso_finish(fb_pso *p, double *cost, int *info)
     if (int rc = check_waits(p->h_info, S, "PSO")) return rc;
     if (*h->h_err == 1) return fail(FB_E_HIP, "PSO: the persistent factorisation found no ready task for 2 s");
     if (*h->h_err == 2) {
+        ++h->dispatch_fallbacks;
         h->forced_mode = 1;
         set_mode(h, 1);
         drop_graphs(h);
@@ -2006,6 +2012,7 @@ int fb_pso_retry(fb_pso *p, int k, const int *idx, const double *x, const double
     if (rc) return rc;
     if (err == 1) return fail(FB_E_HIP, "fb_pso_retry: the persistent factorisation found no ready task for 2 s");
     if (err == 2) {
+        ++h->dispatch_fallbacks;
         h->forced_mode = 1;
         set_mode(h, 1);
         drop_graphs(h);
@@ -2188,10 +2195,10 @@ int fb_profile_enable(fb_handle *h, int enable)
 
 int fb_profile_read(fb_handle *h, int kind, double *total_ms, long long *launches)
 {
-    if (!h || kind < 0 || kind > 8 || !total_ms || !launches) return fail(FB_E_ARG, "fb_profile_read: bad argument");
-    if (kind == 7) {   // the autotuner's timed evaluations so far (not reset)
+    if (!h || kind < 0 || kind > 9 || !total_ms || !launches) return fail(FB_E_ARG, "fb_profile_read: bad argument");
+    if (kind == 7 || kind == 9) {   // counters, not reset
         *total_ms = 0.0;
-        *launches = h->tune_evals;
+        *launches = kind == 7 ? h->tune_evals : h->dispatch_fallbacks;
         return FB_OK;
     }
     int rc = drain_profile(h);

@@ -21,7 +21,7 @@ LIKELIHOODS = {"marginal": 0, "concentrated": 1}
 # first diagonal block, "solve_factor" = panel solve + second diagonal block,
 # "update_solve" = second block's update + solve
 PROFILE_KINDS = {"build": 0, "update": 1, "solve_factor": 2, "update_solve": 3, "likelihood": 4, "swarm": 5,
-                 "factor": 6, "tune_evals": 7, "factor_span": 8}
+                 "factor": 6, "tune_evals": 7, "factor_span": 8, "dispatch_fallbacks": 9}
 
 
 class SwarmLikelihood:

@@ -286,7 +286,11 @@ int fb_multi_pso_step(fb_multi *m, fb_pso *const *ps, const double *const *r1, c
  *       path; 8 lock-step factorisation span, from the end of the build to
  *       the start of the likelihood (with stream groups, the launches of
  *       different groups overlap inside it, and kinds 1-3 add up durations
- *       on several streams). */
+ *       on several streams); 9: *launches = the persistent evaluations that
+ *       found an XCD queue no work-group served (the device does not
+ *       dispatch the way the persistent schedule assumes) and were re-run on
+ *       the lock-step path, which the handle then keeps (a counter, not
+ *       reset; *total_ms = 0). */
 int fb_profile_enable(fb_handle *h, int enable);
 int fb_profile_read(fb_handle *h, int kind, double *total_ms, long long *launches);
 

@@ -45,6 +45,8 @@ def test_paths_bit_identical(gpu, monkeypatch, n, S):
         ll2, info2 = gp.swarm_loglik(th)          # graph replay
         assert np.array_equal(ll, ll2) and np.array_equal(info, info2)
         assert gp.factor_path(S) == SwarmLikelihood.FACTOR_PATHS[path]
+        # the persistent launch was served as scheduled: no lock-step re-run
+        assert gp.profile_read("dispatch_fallbacks") == (0.0, 0)
         out[path] = (ll, info)
     print({p: o[1].tolist() for p, o in out.items()})
     for path in range(1, 5):
