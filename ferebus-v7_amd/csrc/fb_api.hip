@@ -290,14 +290,15 @@ void timed(fb_handle *h, int kind, hipStream_t st, F fn)
 }
 
 // The lock-step factorisation of the particles of b: one launch per step.
-void enqueue_factor(fb_handle *h, const fb::SwarmBuffers &b, hipStream_t st)
+// The launches of panel c0 (column c0, a multiple of 128) of the lock-step
+// factorisation of the particles of b: update + first block's
+// factorisation (kind 1), panel solve + second block's factorisation (kind
+// 2), second block's update + panel solve (kind 3); chained: the update
+// launch also solves and factors the second block (kind 1) and the rows
+// below are solved in one launch (kind 2).
+void enqueue_panel(fb_handle *h, const fb::SwarmBuffers &b, int c0, hipStream_t st)
 {
-    // per 128-column panel: update + first block's factorisation (kind 1),
-    // panel solve + second block's factorisation (kind 2), second block's
-    // update + panel solve (kind 3); chained: the update launch also solves
-    // and factors the second block (kind 1) and the rows below are solved in
-    // one launch (kind 2)
-    for (int c0 = 0; c0 < h->np; c0 += fb::kNBO) {
+    {
         if (b.chain) {
             const bool rows = fb::panel_rows_launch(b, c0);
             if (h->k1_events) {
@@ -312,7 +313,7 @@ void enqueue_factor(fb_handle *h, const fb::SwarmBuffers &b, hipStream_t st)
                 timed(h, 1, st, [&] { fb::launch_panel_chain(b, c0, st); });
                 if (rows) timed(h, 2, st, [&] { fb::launch_solve_rows(b, c0, st); });
             }
-            continue;
+            return;
         }
         const int c1 = c0 + fb::kNBI;
         if (h->k1_events) {
@@ -327,6 +328,12 @@ void enqueue_factor(fb_handle *h, const fb::SwarmBuffers &b, hipStream_t st)
         timed(h, 2, st, [&] { fb::launch_trsm_factor(b, c0, st); });
         if (c1 + fb::kNBI < h->np) timed(h, 3, st, [&] { fb::launch_update_trsm(b, c1, st); });
     }
+}
+
+// The lock-step factorisation of the particles of b: one launch per step.
+void enqueue_factor(fb_handle *h, const fb::SwarmBuffers &b, hipStream_t st)
+{
+    for (int c0 = 0; c0 < h->np; c0 += fb::kNBO) enqueue_panel(h, b, c0, st);
 }
 
 // Lock-step pipeline: build, factorisation (its span: kind 8), likelihood.
@@ -412,17 +419,32 @@ void enqueue_pipeline(fb_handle *h, int S, const double *theta, double *ll, int 
     timed(h, 8, st, [&] {
         (void)hipEventRecord(h->fork_ev, st);
         const int per = (S + G - 1) / G;
+        fb::SwarmBuffers bg[fb_handle::kMaxGroups];
+        hipStream_t gs[fb_handle::kMaxGroups];
+        int ng = 0;
         for (int g = 0; g < G; ++g) {
             const int g0 = g * per;
             const int cnt = std::min(per, S - g0);
             if (cnt <= 0) break;
-            hipStream_t gs = g == 0 ? st : h->group_stream[g];
-            if (g > 0) (void)hipStreamWaitEvent(gs, h->fork_ev, 0);
-            enqueue_factor(h, buffers(h, g0, cnt, theta, ll, info), gs);
-            if (g > 0) {
-                (void)hipEventRecord(h->join_ev[g], gs);
-                (void)hipStreamWaitEvent(st, h->join_ev[g], 0);
+            gs[g] = g == 0 ? st : h->group_stream[g];
+            if (g > 0) (void)hipStreamWaitEvent(gs[g], h->fork_ev, 0);
+            bg[g] = buffers(h, g0, cnt, theta, ll, info);
+            ng = g + 1;
+        }
+        // panel by panel, group after group: every group's first launches
+        // are submitted at once (group after group, a group's launches
+        // would wait for the host to submit all of the previous group's);
+        // per-panel tuning times the first group's launches
+        std::vector<std::pair<hipEvent_t, hipEvent_t>> *k1 = h->k1_events;
+        for (int c0 = 0; c0 < h->np; c0 += fb::kNBO)
+            for (int g = 0; g < ng; ++g) {
+                h->k1_events = g == 0 ? k1 : nullptr;
+                enqueue_panel(h, bg[g], c0, gs[g]);
             }
+        h->k1_events = k1;
+        for (int g = 1; g < ng; ++g) {
+            (void)hipEventRecord(h->join_ev[g], gs[g]);
+            (void)hipStreamWaitEvent(st, h->join_ev[g], 0);
         }
     });
     timed(h, 4, st, [&] { fb::launch_likelihood(all, st); });
@@ -1017,6 +1039,30 @@ static int choose_path(fb_handle *h, int s)
             } else {
                 h->panel_tiles.erase(s);
             }
+        }
+    }
+    // one uniform candidate the per-panel mix can miss (it times each panel
+    // beside the other groups' launches): every panel before the tail
+    // pre-update solves the previous panel's rows in its update launch, so
+    // no row-solve launch runs until the tail
+    const int q_tail = npan - 1 - h->tail;
+    if (h->chain && q_tail >= 2 &&
+        fb::ceil_div(s, std::max(best_g, 1)) <= fb::kChainMaxSwarm) {
+        std::vector<signed char> ahead(npan, 0);
+        for (int k = 1; k < q_tail; ++k) ahead[k] = (signed char)fb::kTileAhead;
+        const bool had = h->panel_tiles.count(s) != 0;
+        const std::vector<signed char> keep = had ? h->panel_tiles[s] : std::vector<signed char>();
+        h->panel_tiles[s] = ahead;
+        set_mode(h, kModePerPanel, s);
+        set_groups(best_g);
+        const float ms = time_eval();
+        if (ms < best) {
+            best = ms;
+            best_mode = kModePerPanel;
+        } else if (had) {
+            h->panel_tiles[s] = keep;
+        } else {
+            h->panel_tiles.erase(s);
         }
     }
     if (best_g > 1) h->groups[s] = best_g;
