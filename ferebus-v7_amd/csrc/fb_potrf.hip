@@ -243,10 +243,37 @@ __device__ inline void chain_publish_mem(int *flag, int tag)
 // A: each wave solves its 16 rows alone (solve_rows16: solve with
 // L(b0, b0), the 64-deep update of the b1 part, solve with L(b1, b1);
 // dinv_s: the particle's 16x16 inverses).  WT: stored past the XCD's L2.
+// Rows at or beyond nreal are padding: their entries are zero left of the
+// diagonal and stay zero (every product they take has a zero factor), so a
+// wave whose 16 rows are all padding skips its solve.
 template <bool WT = false>
-__device__ inline void solve_rows_tile(double *A, int np, int c0, int row0, const double *dinv_s)
+__device__ inline void solve_rows_tile(double *A, int np, int c0, int row0, const double *dinv_s, int nreal)
 {
-    solve_rows16<WT>(A, np, c0, row0 + 16 * (threadIdx.x >> 6), dinv_s);
+    const int row = row0 + 16 * (threadIdx.x >> 6);
+    if (row < nreal) solve_rows16<WT>(A, np, c0, row, dinv_s);
+}
+
+// The update of the MB x NB tile at (row0, col0) over columns k0 .. k1 of
+// the matrix at A, with rows at or beyond nreal (padding, zero left of the
+// diagonal: their products are all zero) left out where that saves whole
+// 32-row halves of a 64-row tile; each element's products in the same
+// order, so the same bits.  Returns without work when every row is padding.
+template <int MB, int NB, int KC, int DIST, class Hook = NoHook>
+__device__ __forceinline__ void row_tile_update(double *A, int np, int row0, int col0, int k0, int k1, int nreal,
+                                                double *smem, int kh = -1, const Hook &hook = Hook())
+{
+    if constexpr (MB == 64 && (32 * KC / 2) % kThreads == 0) {   // the 32-row tile's loads fit its chunk
+        if (row0 + 32 >= nreal) {
+            gemm_tile_update<32, NB, KC, DIST, false, Hook>(A, np, row0, col0, k0, k1, smem, kh, hook);
+            return;
+        }
+    }
+    if (row0 >= nreal) {
+        // nothing to update; a hook (a wait) still runs
+        if constexpr (Hook::kActive) hook();
+        return;
+    }
+    gemm_tile_update<MB, NB, KC, DIST, false, Hook>(A, np, row0, col0, k0, k1, smem, kh, hook);
 }
 
 // Hook of an update that takes the previous panel's columns last
@@ -344,7 +371,8 @@ __device__ __forceinline__ void chain_role(int b, double *__restrict__ K, int np
         // the role's own 64 rows against the previous panel (D: b0, B / BC:
         // b1; C reads B's)
         if (role <= 1) {
-            if (info[s] == 0) solve_rows_tile<true>(A, np, cp, role == 0 ? c0 : c1, dinv + (size_t)s * nbk * kDinv);
+            if (info[s] == 0)
+                solve_rows_tile<true>(A, np, cp, role == 0 ? c0 : c1, dinv + (size_t)s * nbk * kDinv, n + 2);
             chain_publish_mem(cw + (role == 0 ? kChainS0 : kChainS1), tag);
         }
     }
@@ -448,8 +476,8 @@ __device__ __forceinline__ void chain_role(int b, double *__restrict__ K, int np
 // previous panel, then updates with that panel's columns last, after the
 // flags of the two diagonal row blocks.
 template <int KC, int MB, int NB, bool FUSED, bool AHEAD = false, int DIST = 2>
-__device__ __forceinline__ void chain_rowtile(int lt, double *__restrict__ K, int np, int c0, int k0, int S,
-                                              int *__restrict__ info, double *__restrict__ dinv,
+__device__ __forceinline__ void chain_rowtile(int lt, double *__restrict__ K, int np, int nreal, int c0, int k0,
+                                              int S, int *__restrict__ info, double *__restrict__ dinv,
                                               int *__restrict__ chain, unsigned long long wait_limit,
                                               double *smem, int &ok)
 {
@@ -463,15 +491,15 @@ __device__ __forceinline__ void chain_rowtile(int lt, double *__restrict__ K, in
     const int row0 = c0 + kNBO + (u / CT) * MB;
     if constexpr (AHEAD) {
         static_assert(MB == kNBI && NB == kNBO && !FUSED, "the ahead row solve needs 64 x 128 row tiles");
-        solve_rows_tile(A, np, c0 - kNBO, row0, dinv + (size_t)s * (np / kNBI) * kDinv);
+        solve_rows_tile(A, np, c0 - kNBO, row0, dinv + (size_t)s * (np / kNBI) * kDinv, nreal);
         __syncthreads();   // each wave solved 16 rows; the update reads all 64
         const int *cw = chain + (size_t)s * kChainStride, tag = c0 / kNBO + 1;
-        gemm_tile_update<MB, NB, KC, DIST, false, FlagWait>(
-            A, np, row0, c0, k0, c0, smem, c0 - kNBO,
+        row_tile_update<MB, NB, KC, DIST, FlagWait>(
+            A, np, row0, c0, k0, c0, nreal, smem, c0 - kNBO,
             FlagWait{cw + kChainS0, cw + kChainS1, tag, info + s, &ok, wait_limit});
         return;
     }
-    gemm_tile_update<MB, NB, KC, DIST>(A, np, row0, c0 + (u % CT) * NB, k0, c0, smem);
+    row_tile_update<MB, NB, KC, DIST>(A, np, row0, c0 + (u % CT) * NB, k0, c0, nreal, smem);
     if constexpr (FUSED) {
         static_assert(MB == kNBI && NB == kNBO, "the fused row solve needs 64 x 128 row tiles");
         const int tag = c0 / kNBO + 1;
@@ -480,7 +508,7 @@ __device__ __forceinline__ void chain_rowtile(int lt, double *__restrict__ K, in
             return;
         }
         if (info[s] != 0) return;
-        solve_rows_tile(A, np, c0, row0, dinv + (size_t)s * (np / kNBI) * kDinv);
+        solve_rows_tile(A, np, c0, row0, dinv + (size_t)s * (np / kNBI) * kDinv, nreal);
     }
 }
 
@@ -489,8 +517,8 @@ __device__ __forceinline__ void chain_rowtile(int lt, double *__restrict__ K, in
 // diagonal 64 x 64 block or a 64 x 128 row tile of one of the next `tail`
 // panels.
 template <int KC, int DIST = 2>
-__device__ __forceinline__ void chain_tailtile(int lt, double *__restrict__ K, int np, int c0, int kt, int tail,
-                                               int S, const int *__restrict__ info, double *smem)
+__device__ __forceinline__ void chain_tailtile(int lt, double *__restrict__ K, int np, int nreal, int c0, int kt,
+                                               int tail, int S, const int *__restrict__ info, double *smem)
 {
     const int ct = tail_tiles(np, c0, tail);
     const int t = xcd_tile(lt, S * ct);
@@ -506,7 +534,7 @@ __device__ __forceinline__ void chain_tailtile(int lt, double *__restrict__ K, i
     if (info[s] != 0) return;
     double *A = K + (size_t)s * np * np;
     if (v == 0) gemm_tile_update<kMB, kNBI, KC, DIST>(A, np, cp, cp, kt, c0, smem);
-    else gemm_tile_update<kMB, kNBO, KC, DIST>(A, np, cp + kNBI * v, cp, kt, c0, smem);
+    else row_tile_update<kMB, kNBO, KC, DIST>(A, np, cp + kNBI * v, cp, kt, c0, nreal, smem);
 }
 
 // WPE work-groups per CU (waves per SIMD): 2 (16-deep chunks), 3 or 4
@@ -536,15 +564,15 @@ __attribute__((amdgpu_waves_per_eu(WPE, WPE))) panel_chain_kernel(double *__rest
     const int rt = (np - c0 - kNBO) / MB;    // row tiles below the panel
     const int nreg = 8 * ceil_div(S * rt * CT, 8);
     if (b - nrole < nreg) {
-        chain_rowtile<KC, MB, NB, FUSED, AHEAD, (WPE >= 4 ? 1 : 2)>(b - nrole, K, np, c0, k0, S, info, dinv, chain,
-                                                                   wait_limit, smem, ok);
+        chain_rowtile<KC, MB, NB, FUSED, AHEAD, (WPE >= 4 ? 1 : 2)>(b - nrole, K, np, n + 2, c0, k0, S, info, dinv,
+                                                                   chain, wait_limit, smem, ok);
         return;
     }
-    chain_tailtile<KC, (WPE >= 4 ? 1 : 2)>(b - nrole - nreg, K, np, c0, kt, tail, S, info, smem);
+    chain_tailtile<KC, (WPE >= 4 ? 1 : 2)>(b - nrole - nreg, K, np, n + 2, c0, kt, tail, S, info, smem);
 }
 
 // Row-solve tile lt (< 8 ceil(S rt / 8), XCD-aware order) of panel c0.
-__device__ __forceinline__ void solve_rows_block(int lt, double *__restrict__ K, int np, int c0, int S,
+__device__ __forceinline__ void solve_rows_block(int lt, double *__restrict__ K, int np, int nreal, int c0, int S,
                                                  const int *__restrict__ info, const double *__restrict__ dinv)
 {
     const int rt = (np - c0 - kNBO) / kNBI;
@@ -553,7 +581,7 @@ __device__ __forceinline__ void solve_rows_block(int lt, double *__restrict__ K,
     const int s = t / rt;
     const int row0 = c0 + kNBO + (t % rt) * kNBI;
     if (info[s] != 0) return;
-    solve_rows_tile(K + (size_t)s * np * np, np, c0, row0, dinv + (size_t)s * (np / kNBI) * kDinv);
+    solve_rows_tile(K + (size_t)s * np * np, np, c0, row0, dinv + (size_t)s * (np / kNBI) * kDinv, nreal);
 }
 
 // Launch 2 of panel c0, chained: every row tile below the panel is solved
@@ -561,11 +589,10 @@ __device__ __forceinline__ void solve_rows_block(int lt, double *__restrict__ K,
 // tile, each wave on its own 16 rows; no LDS).
 // (three waves per SIMD: at most 168 VGPRs)
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3)))
-solve_rows_kernel(double *__restrict__ K, int np, int c0, int S,
-                                                              const int *__restrict__ info,
-                                                              const double *__restrict__ dinv)
+solve_rows_kernel(double *__restrict__ K, int np, int nreal, int c0, int S, const int *__restrict__ info,
+                  const double *__restrict__ dinv)
 {
-    solve_rows_block(blockIdx.x, K, np, c0, S, info, dinv);
+    solve_rows_block(blockIdx.x, K, np, nreal, c0, S, info, dinv);
 }
 
 // ---- several swarms in one chained launch ----------------------------------
@@ -606,13 +633,13 @@ __attribute__((amdgpu_waves_per_eu(KC == 8 ? 3 : 2, KC == 8 ? 3 : 2))) panel_cha
         while (b >= m.seg[g].tile_end) ++g;
         const ChainSegment &q = m.seg[g];
         chain_rowtile<KC, MB, NB, FUSED>(b - (g ? m.seg[g - 1].tile_end : m.seg[m.G - 1].role_end), q.K, q.np,
-                                         q.c0, q.k0, q.S, q.info, q.dinv, q.chain, m.wait_limit, smem, ok);
+                                         q.n + 2, q.c0, q.k0, q.S, q.info, q.dinv, q.chain, m.wait_limit, smem, ok);
         return;
     }
     while (b >= m.seg[g].tail_end) ++g;
     const ChainSegment &q = m.seg[g];
-    chain_tailtile<KC>(b - (g ? m.seg[g - 1].tail_end : m.seg[m.G - 1].tile_end), q.K, q.np, q.c0, q.kt, q.tail,
-                       q.S, q.info, smem);
+    chain_tailtile<KC>(b - (g ? m.seg[g - 1].tail_end : m.seg[m.G - 1].tile_end), q.K, q.np, q.n + 2, q.c0, q.kt,
+                       q.tail, q.S, q.info, smem);
 }
 
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3)))
@@ -622,7 +649,7 @@ solve_rows_multi_kernel(ChainMulti m)
     int g = 0;
     while (b >= m.seg[g].solve_end) ++g;
     const ChainSegment &q = m.seg[g];
-    solve_rows_block(b - (g ? m.seg[g - 1].solve_end : 0), q.K, q.np, q.c0, q.S, q.info, q.dinv);
+    solve_rows_block(b - (g ? m.seg[g - 1].solve_end : 0), q.K, q.np, q.n + 2, q.c0, q.S, q.info, q.dinv);
 }
 
 template <int KC, int MB, int NB, bool TWO, bool FUSED = false, bool AHEAD = false, int WPE = (KC == 8 ? 3 : 2)>
@@ -725,7 +752,7 @@ void launch_solve_rows(const SwarmBuffers &b, int c0, hipStream_t st)
     const int rt = (b.np - c0 - kNBO) / kNBI;
     if (rt <= 0) return;
     hipLaunchKernelGGL(solve_rows_kernel, dim3(8 * ceil_div(b.S * rt, 8)), dim3(kThreads), 0, st, b.K, b.np,
-                       c0, b.S, b.info, b.dinv);
+                       b.n + 2, c0, b.S, b.info, b.dinv);
 }
 
 template <int KC, int MB, int NB, bool TWO, bool FUSED>
