@@ -56,9 +56,12 @@ struct TileShape {
 // the accumulators: no LDS epilogue, no barrier after the loop.  K is
 // streamed in chunks of KC through two LDS buffers; the global loads of a
 // chunk are issued two chunks ahead (two register sets), so each has two
-// chunks of matrix-core work to land.  For -= the A chunk is negated as it
-// is staged, so the inner loop passes LDS reads straight to the matrix cores
-// (the products see the same operands either way).
+// chunks of matrix-core work to land.  For -= the accumulators start at -C
+// and take the products added (the chunks are staged as loaded, the inner
+// loop passes LDS reads straight to the matrix cores), and -acc is stored:
+// negation is exact and rounding symmetric, so every partial sum is the
+// exact negative of that of C - A B^T (the same bits, without a negation
+// per staged element).
 // ZERO_C: C starts at zero (not loaded); PLUS: C += A B^T instead of -=.
 // hook (Hook::kActive): called by every thread once, right before the
 // global loads of chunk hook_chunk (0 <= hook_chunk < depth / KC) are
@@ -94,7 +97,8 @@ __device__ inline void tile_gemm_sub(double *__restrict__ Cg, int ldc,
         for (int j = 0; j < T::BN; ++j)
 #pragma unroll
             for (int v = 0; v < 4; ++v)
-                acc[i][j][v] = ZERO_C ? 0.0 : Cg[(size_t)(wcol + 16 * j + 4 * v + lq) * ldc + wrow + 16 * i + lr];
+                acc[i][j][v] = ZERO_C ? 0.0
+                               : (PLUS ? 1.0 : -1.0) * Cg[(size_t)(wcol + 16 * j + 4 * v + lq) * ldc + wrow + 16 * i + lr];
 
     if (depth > 0) {
         // per-thread load coordinates (fixed for every stage)
@@ -138,8 +142,7 @@ __device__ inline void tile_gemm_sub(double *__restrict__ Cg, int ldc,
             double *an = smem + (BUF) * T::ASZ;                                                 \
             double *bn = smem + 2 * T::ASZ + (BUF) * T::BSZ;                                    \
             _Pragma("unroll") for (int u = 0; u < NA; ++u)                                      \
-                *reinterpret_cast<double2 *>(an + a_lds[u]) =                                   \
-                    PLUS ? double2{r##SET##ax[u], r##SET##ay[u]} : double2{-r##SET##ax[u], -r##SET##ay[u]}; \
+                *reinterpret_cast<double2 *>(an + a_lds[u]) = double2{r##SET##ax[u], r##SET##ay[u]};  \
             _Pragma("unroll") for (int u = 0; u < NBL; ++u)                                     \
                 *reinterpret_cast<double2 *>(bn + b_lds[u]) = double2{r##SET##bx[u], r##SET##by[u]}; \
         } while (0)
@@ -209,7 +212,7 @@ __device__ inline void tile_gemm_sub(double *__restrict__ Cg, int ldc,
             for (int j = 0; j < T::BN; ++j)
 #pragma unroll
                 for (int v = 0; v < 4; ++v)
-                    smem[(wcol + 16 * j + 4 * v + lq) * kPLD + wrow + 16 * i + lr] = acc[i][j][v];
+                    smem[(wcol + 16 * j + 4 * v + lq) * kPLD + wrow + 16 * i + lr] = PLUS ? acc[i][j][v] : -acc[i][j][v];
     } else {
 #pragma unroll
         for (int i = 0; i < T::BM; ++i)
@@ -217,7 +220,7 @@ __device__ inline void tile_gemm_sub(double *__restrict__ Cg, int ldc,
             for (int j = 0; j < T::BN; ++j)
 #pragma unroll
                 for (int v = 0; v < 4; ++v)
-                    Cg[(size_t)(wcol + 16 * j + 4 * v + lq) * ldc + wrow + 16 * i + lr] = acc[i][j][v];
+                    Cg[(size_t)(wcol + 16 * j + 4 * v + lq) * ldc + wrow + 16 * i + lr] = PLUS ? acc[i][j][v] : -acc[i][j][v];
     }
 }
 
