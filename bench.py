@@ -29,10 +29,22 @@ sys.path.insert(0, str(REPO))
 
 
 
-# committed rocprofv3 summaries of this workload, per factorisation path
-# (lock-step paths: the three-launch schedule, FB_CHAIN=0; CHAINED_PROFILES:
-# the default two-launch schedule, keyed on (path, stream groups))
+# committed rocprofv3 summaries, per factorisation path (PROFILES: the
+# methanol-like N=2000 swarm-50 workload on the three-launch lock-step
+# schedule, FB_CHAIN=0; CHAINED_PROFILES: the default two-launch schedule,
+# keyed on (workload, path, stream groups) at the default sizes)
 CHAINED_PROFILES = {
+    ("methanol", "lockstep-kc16-per-panel", 1): (REPO / "profiles" / "chained_factor_per_panel_methanol_n2000_s50.md",
+                                                 "lockstep_factor"),
+    ("methanol", "lockstep-kc16-per-panel", 2): (REPO / "profiles" /
+                                                 "chained_factor_per_panel_groups2_methanol_n2000_s50.md",
+                                                 "lockstep_factor"),
+    ("methanol", "lockstep-kc16", 2): (REPO / "profiles" / "chained_factor_groups2_methanol_n2000_s50.md",
+                                       "lockstep_factor"),
+    ("ibuprofen", "lockstep-kc16-per-panel", 2): (REPO / "profiles" / "ibuprofen_build_and_factor_n6000_s64.md",
+                                                  "lockstep_factor"),
+}
+PROFILES = {
     ("lockstep-kc16-per-panel", 1): (REPO / "profiles" / "chained_factor_per_panel_methanol_n2000_s50.md",
                                      "lockstep_factor"),
     ("lockstep-kc16", 2): (REPO / "profiles" / "chained_factor_groups2_methanol_n2000_s50.md",
@@ -48,11 +60,16 @@ PROFILES = {
 }
 
 
-def profiled_traffic(path: str, chained: bool = False, groups: int = 1):
+def profiled_traffic(path: str, chained: bool = False, groups: int = 1, workload: str = "methanol"):
     """(HBM bytes per factorisation, source) from the committed rocprofv3
     counter summary of that path (FETCH_SIZE + WRITE_SIZE, copy-kernel
     calibrated), or (None, None)."""
-    table, key_ = (CHAINED_PROFILES, (path, groups)) if chained else (PROFILES, path)
+    if chained:
+        table, key_ = CHAINED_PROFILES, (workload, path, groups)
+    elif workload == "methanol":
+        table, key_ = PROFILES, path
+    else:
+        return None, None
     if key_ not in table:
         return None, None
     f, key = table[key_]
@@ -370,8 +387,9 @@ def main():
                        "duration = the factorisation's span")
     fact_flops = S * potrf_flops(n)
     factor_path = gp.factor_path(S)
-    traffic, traffic_src = (profiled_traffic(factor_path, chained, groups)
-                            if (n == 2000 and S == 50 and not ibu) else (None, None))
+    default_size = (n, S) == ((6000, 64) if ibu else (2000, 50))
+    traffic, traffic_src = (profiled_traffic(factor_path, chained, groups, args.workload)
+                            if default_size else (None, None))
     build_ms = span["build"][0] / prof_steps
     build_bytes = S * 8.0 * n * (n + 1) / 2.0 + 8.0 * n * d
     achieved = fact_flops / (fact_ms * 1e-3) / 1e12
