@@ -80,6 +80,9 @@ __device__ inline void tile_gemm_sub(double *__restrict__ Cg, int ldc,
                                      double *__restrict__ smem, int hook_chunk = -1, const Hook &hook = Hook())
 {
     using T = TileShape<MB, NB, KC>;
+    // the software-pipelined K loop: two fragment sets cost 12-24 VGPRs, so
+    // only the 16- and 32-deep tiles (two work-groups per CU) take it
+    constexpr bool PIPE = DIST > 1 && KC >= 16;
     constexpr int NA = MB * KC / 2 / kThreads;   // double2 loads per thread (A)
     constexpr int NBL = NB * KC / 2 / kThreads;  // double2 loads per thread (B)
     const int tid = threadIdx.x;
@@ -165,7 +168,71 @@ __device__ inline void tile_gemm_sub(double *__restrict__ Cg, int ldc,
         const int nchunks = depth / KC;
         FB_LOAD(0, 0);
         FB_STORE(0, 0);
-        if constexpr (DIST == 1) {
+        if constexpr (PIPE) {
+            // fragments of k-step kk + 1 read from LDS before the products of
+            // k-step kk are issued (two fragment register sets; sched_barrier
+            // keeps the order, which the scheduler otherwise undoes); the
+            // first step of the next chunk is read right after the barrier,
+            // while the matrix cores still work through the last step
+            double af[2][T::BM], bf[2][T::BN];
+            auto rd = [&](int set, int buf, int kk) {
+                const double *as = smem + buf * T::ASZ;
+                const double *bs = smem + 2 * T::ASZ + buf * T::BSZ;
+#pragma unroll
+                for (int i = 0; i < T::BM; ++i) af[set][i] = as[(kk * 4 + lq) * T::LDA + wrow + i * 16 + lr];
+#pragma unroll
+                for (int j = 0; j < T::BN; ++j) bf[set][j] = bs[(kk * 4 + lq) * T::LDB + wcol + j * 16 + lr];
+            };
+            auto mm = [&](int set) {
+#pragma unroll
+                for (int i = 0; i < T::BM; ++i)
+#pragma unroll
+                    for (int j = 0; j < T::BN; ++j) acc[i][j] = mfma_f64(bf[set][j], af[set][i], acc[i][j]);
+            };
+            // k-steps 1 .. KC/4 - 1 of the chunk in buf (step 0's fragments
+            // in set 0); the last step's fragments are left in set 1
+            auto steps = [&](int buf) {
+#pragma unroll
+                for (int kk = 1; kk < KC / 4; ++kk) {
+                    rd(kk & 1, buf, kk);
+                    __builtin_amdgcn_sched_barrier(0);
+                    mm((kk - 1) & 1);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            };
+            if (nchunks > 1) FB_LOAD(1, 1);
+            __syncthreads();
+            rd(0, 0, 0);
+            int c = 0;
+            for (; c + 1 < nchunks; c += 2) {
+                // chunk c in LDS buffer 0, chunk c + 1 in register set 1
+                if (c + 2 < nchunks) FB_LOAD(0, c + 2);
+                steps(0);
+                FB_STORE(1, 1);
+                __syncthreads();
+                // chunk c + 1 in LDS buffer 1, chunk c + 2 in register set 0
+                // (the loads of c + 3 go first: a branch after the fragment
+                // reads would let the compiler sink them past the products)
+                if (c + 3 < nchunks) FB_LOAD(1, c + 3);
+                rd(0, 1, 0);
+                __builtin_amdgcn_sched_barrier(0);
+                mm(1);
+                __builtin_amdgcn_sched_barrier(0);
+                steps(1);
+                if (c + 2 < nchunks) FB_STORE(0, 0);
+                __syncthreads();
+                rd(0, 0, 0);
+                __builtin_amdgcn_sched_barrier(0);
+                mm(1);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            if (c < nchunks) {
+                // an odd count: the last chunk is in LDS buffer 0
+                steps(0);
+                mm(1);
+                __syncthreads();
+            }
+        } else if constexpr (DIST == 1) {
             // one register set: the loads of chunk c + 1 land during chunk c
             __syncthreads();
             for (int c = 0; c < nchunks; ++c) {
