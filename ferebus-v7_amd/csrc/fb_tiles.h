@@ -22,6 +22,14 @@ __device__ inline d4 mfma_f64(double a, double b, d4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// D = C - A B: the same product with the first operand negated by the
+// instruction's neg modifier (on the f64 products the last argument is
+// neg[a, b, c]; bit for bit the product with -A, checked on gfx950 with a
+// one-wave probe), so no operand is negated in registers or LDS.
+__device__ inline d4 mfma_f64_sub(double a, double b, d4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 1);
+}
+
 constexpr int kPLD = 68;                        // LDS stride (doubles) of a 64x64 block kept on chip
 
 // Work-group tile of MB rows x NB columns computed by 4 waves, K streamed in
@@ -56,12 +64,12 @@ struct TileShape {
 // the accumulators: no LDS epilogue, no barrier after the loop.  K is
 // streamed in chunks of KC through two LDS buffers; the global loads of a
 // chunk are issued two chunks ahead (two register sets), so each has two
-// chunks of matrix-core work to land.  For -= the accumulators start at -C
-// and take the products added (the chunks are staged as loaded, the inner
-// loop passes LDS reads straight to the matrix cores), and -acc is stored:
-// negation is exact and rounding symmetric, so every partial sum is the
-// exact negative of that of C - A B^T (the same bits, without a negation
-// per staged element).
+// chunks of matrix-core work to land.  For -= the products are subtracted by
+// the matrix cores' neg modifier (mfma_f64_sub): C is loaded straight into
+// the accumulators and stored straight back, the chunks are staged as
+// loaded and the inner loop passes LDS reads straight to the matrix cores
+// (no negation anywhere, so the loads of the first chunks are not held up
+// behind the loads of C either).
 // ZERO_C: C starts at zero (not loaded); PLUS: C += A B^T instead of -=.
 // hook (Hook::kActive): called by every thread once, right before the
 // global loads of chunk hook_chunk (0 <= hook_chunk < depth / KC) are
@@ -100,8 +108,7 @@ __device__ inline void tile_gemm_sub(double *__restrict__ Cg, int ldc,
         for (int j = 0; j < T::BN; ++j)
 #pragma unroll
             for (int v = 0; v < 4; ++v)
-                acc[i][j][v] = ZERO_C ? 0.0
-                               : (PLUS ? 1.0 : -1.0) * Cg[(size_t)(wcol + 16 * j + 4 * v + lq) * ldc + wrow + 16 * i + lr];
+                acc[i][j][v] = ZERO_C ? 0.0 : Cg[(size_t)(wcol + 16 * j + 4 * v + lq) * ldc + wrow + 16 * i + lr];
 
     if (depth > 0) {
         // per-thread load coordinates (fixed for every stage)
@@ -162,7 +169,8 @@ __device__ inline void tile_gemm_sub(double *__restrict__ Cg, int ldc,
 #pragma unroll
                 for (int i = 0; i < T::BM; ++i)
 #pragma unroll
-                    for (int j = 0; j < T::BN; ++j) acc[i][j] = mfma_f64(bf[j], af[i], acc[i][j]);
+                    for (int j = 0; j < T::BN; ++j)
+                        acc[i][j] = PLUS ? mfma_f64(bf[j], af[i], acc[i][j]) : mfma_f64_sub(bf[j], af[i], acc[i][j]);
             }
         };
         const int nchunks = depth / KC;
@@ -187,7 +195,9 @@ __device__ inline void tile_gemm_sub(double *__restrict__ Cg, int ldc,
 #pragma unroll
                 for (int i = 0; i < T::BM; ++i)
 #pragma unroll
-                    for (int j = 0; j < T::BN; ++j) acc[i][j] = mfma_f64(bf[set][j], af[set][i], acc[i][j]);
+                    for (int j = 0; j < T::BN; ++j)
+                        acc[i][j] = PLUS ? mfma_f64(bf[set][j], af[set][i], acc[i][j])
+                                         : mfma_f64_sub(bf[set][j], af[set][i], acc[i][j]);
             };
             // k-steps 1 .. KC/4 - 1 of the chunk in buf (step 0's fragments
             // in set 0); the last step's fragments are left in set 1
@@ -279,7 +289,7 @@ __device__ inline void tile_gemm_sub(double *__restrict__ Cg, int ldc,
             for (int j = 0; j < T::BN; ++j)
 #pragma unroll
                 for (int v = 0; v < 4; ++v)
-                    smem[(wcol + 16 * j + 4 * v + lq) * kPLD + wrow + 16 * i + lr] = PLUS ? acc[i][j][v] : -acc[i][j][v];
+                    smem[(wcol + 16 * j + 4 * v + lq) * kPLD + wrow + 16 * i + lr] = acc[i][j][v];
     } else {
 #pragma unroll
         for (int i = 0; i < T::BM; ++i)
@@ -287,7 +297,7 @@ __device__ inline void tile_gemm_sub(double *__restrict__ Cg, int ldc,
             for (int j = 0; j < T::BN; ++j)
 #pragma unroll
                 for (int v = 0; v < 4; ++v)
-                    Cg[(size_t)(wcol + 16 * j + 4 * v + lq) * ldc + wrow + 16 * i + lr] = PLUS ? acc[i][j][v] : -acc[i][j][v];
+                    Cg[(size_t)(wcol + 16 * j + 4 * v + lq) * ldc + wrow + 16 * i + lr] = acc[i][j][v];
     }
 }
 
@@ -554,7 +564,7 @@ __device__ inline int potrf64_block(double *__restrict__ A, int np, int n, int c
                 for (int kk = 0; kk < 4; ++kk) {
                     const double af = Ls[(16 * j + 4 * kk + lq) * kPLD + 16 * bi + lr];
                     const double bf = Ls[(16 * j + 4 * kk + lq) * kPLD + 16 * bj + lr];
-                    acc = mfma_f64(bf, -af, acc);
+                    acc = mfma_f64_sub(bf, af, acc);
                 }
 #pragma unroll
                 for (int v = 0; v < 4; ++v) Ls[(16 * bj + 4 * v + lq) * kPLD + 16 * bi + lr] = acc[v];
@@ -685,7 +695,7 @@ __device__ inline void trsm16_rows(const d4 t[4], const double *__restrict__ Lb,
 #pragma unroll
             for (int kk = 0; kk < 4; ++kk) lf[kk] = Lb[(size_t)(16 * k + 4 * kk + lq) * ldl + 16 * j + lr];
 #pragma unroll
-            for (int kk = 0; kk < 4; ++kk) r = mfma_f64(-lf[kk], y[k][kk], r);
+            for (int kk = 0; kk < 4; ++kk) r = mfma_f64_sub(lf[kk], y[k][kk], r);
         }
         const double *Wj = dinv + j * 256;
         if (flag[j] != 0.0) {
@@ -810,7 +820,7 @@ __device__ inline void syrk64(double *__restrict__ Ls, const double *__restrict_
         for (int kk = 0; kk < 16; ++kk) {
             const double af = X[(size_t)(4 * kk + lq) * ldx + 16 * bi + lr];
             const double bf = X[(size_t)(4 * kk + lq) * ldx + 16 * bj + lr];
-            acc = mfma_f64(bf, -af, acc);
+            acc = mfma_f64_sub(bf, af, acc);
         }
 #pragma unroll
         for (int v = 0; v < 4; ++v) Ls[(16 * bj + 4 * v + lq) * kPLD + 16 * bi + lr] = acc[v];
