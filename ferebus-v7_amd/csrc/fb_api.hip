@@ -1058,15 +1058,19 @@ static int choose_path(fb_handle *h, int s)
             }
         }
     }
-    // one uniform candidate the per-panel mix can miss (it times each panel
-    // beside the other groups' launches): every panel before the tail
+    // two uniform candidates the per-panel mix can miss (it times each
+    // panel beside the other groups' launches): every panel before the tail
     // pre-update solves the previous panel's rows in its update launch, so
-    // no row-solve launch runs until the tail
+    // no row-solve launch runs until the tail; and the same with the panels
+    // from the tail pre-update on (but the last) solving their own rows in
+    // their update launch, with two role work-groups
     const int q_tail = npan - 1 - h->tail;
-    if (h->chain && q_tail >= 2 &&
-        fb::ceil_div(s, std::max(best_g, 1)) <= fb::kChainMaxSwarm) {
+    for (int variant = 0; variant < 2; ++variant) {
+        if (!(h->chain && q_tail >= 2 && fb::ceil_div(s, std::max(best_g, 1)) <= fb::kChainMaxSwarm)) break;
         std::vector<signed char> ahead(npan, 0);
         for (int k = 1; k < q_tail; ++k) ahead[k] = (signed char)fb::kTileAhead;
+        if (variant == 1)
+            for (int k = q_tail; k < npan - 1; ++k) ahead[k] = (signed char)(fb::kTileFused | fb::kTileTwoRoles);
         const bool had = h->panel_tiles.count(s) != 0;
         const std::vector<signed char> keep = had ? h->panel_tiles[s] : std::vector<signed char>();
         h->panel_tiles[s] = ahead;
