@@ -231,7 +231,7 @@ __device__ inline void tile_gemm_sub(double *__restrict__ Cg, int ldc,
                 steps(1);
                 if (c + 2 < nchunks) FB_STORE(0, 0);
                 __syncthreads();
-                rd(0, 0, 0);
+                if (c + 2 < nchunks) rd(0, 0, 0);   // no read past the last chunk
                 __builtin_amdgcn_sched_barrier(0);
                 mm(1);
                 __builtin_amdgcn_sched_barrier(0);
