@@ -199,6 +199,7 @@ def test_chained_schedule_tile_shapes(gpu, monkeypatch, path, tiles):
                                           (900, "8,32,0,32,16,48,2,32", 0),
                                           (1100, "0,32,32,32,0,32,32,32,36", 3),
                                           (1100, "0,32,32,32,32,32,32,32,32", 3),
+                                          (1100, "0,32,32,32,32,12,12,12,0", 3),
                                           (1100, "32,32,4,32,48,0,32,52,32", 2),
                                           (250, "32,32,32", 0),
                                           (900, "64,96,100,68,64,96,4,100", 0),
@@ -209,7 +210,10 @@ def test_ahead_row_solve_bits(gpu, monkeypatch, n, tiles, tail):
     or four work-groups per CU (+ 16, + 64); on the first panel and after a
     fused one (+ 8) the bit is ignored), mixed with the other variants, after tail pre-updates (panels 5-8 of 9 start
     from the pre-updated columns) and in the launch that would carry them
-    (then it carries none): the bits of the three-launch schedule."""
+    (then it carries none), and the autotuner's second uniform candidate
+    (ahead up to the tail pre-update, from there the row solve fused into
+    each update launch with two role work-groups): the bits of the
+    three-launch schedule."""
     _, x, y, prog, mfit = _problem(n, 6, seed=n + tail)
     out = {}
     for chain in (0, 1):
