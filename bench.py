@@ -43,6 +43,8 @@ CHAINED_PROFILES = {
                                        "lockstep_factor"),
     ("ibuprofen", "lockstep-kc16-per-panel", 2): (REPO / "profiles" / "ibuprofen_build_and_factor_n6000_s64.md",
                                                   "lockstep_factor"),
+    ("nma", "lockstep-kc16-per-panel", 2): (REPO / "profiles" / "nma_build_and_factor_n3000_s32.md", "lockstep_factor"),
+    ("water", "lockstep-kc16", 1): (REPO / "profiles" / "water_build_and_factor_n500_s20.md", "lockstep_factor"),
 }
 PROFILES = {
     ("lockstep-kc16-per-panel", 1): (REPO / "profiles" / "chained_factor_per_panel_methanol_n2000_s50.md",
@@ -80,6 +82,19 @@ def profiled_traffic(path: str, chained: bool = False, groups: int = 1, workload
     except OSError:
         pass
     return None, None
+
+
+# --workload: (synthetic generator, default N, default swarm, seed, config text)
+WORKLOADS = {
+    "methanol": ("methanol", 2000, 50, 0, "methanol-like N={n} d={d} swarm={S} rbf(9)+periodic(3) constant mean, "
+                                          "marginal, nugget 1e-10, theta~U[0,1)"),
+    "water": ("water", 500, 20, 0, "water-O1-like N={n} d={d} swarm={S} rbf[1,2]+periodic[3] constant mean, "
+                                   "marginal, nugget 1e-10, theta~U[1e-3,1)"),
+    "nma": ("nma", 3000, 32, 3, "NMA-like N={n} d={d} swarm={S} ARD rbf(30)+periodic(9) constant mean, marginal, "
+                                "optimised nugget~U[1e-12,1e-4), theta~U[0,1)"),
+    "ibuprofen": ("ibuprofen", 6000, 64, 4, "ibuprofen-like N={n} d={d} swarm={S} rbf(63)+periodic(30) linear mean, "
+                                            "concentrated, nugget 1e-10, theta~U[0,1)"),
+}
 
 
 def potrf_flops(n: int) -> float:
@@ -273,11 +288,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=("methanol", "ibuprofen"), default="methanol",
-                    help="methanol: the headline workload (N=2000, d=12, swarm 50); ibuprofen: N=6000, d=93, "
-                         "linear mean, concentrated likelihood, swarm 64")
-    ap.add_argument("--n", type=int, default=None, help="training points (default 2000 / 6000)")
-    ap.add_argument("--swarm", type=int, default=None, help="particles (default 50 / 64)")
+    ap.add_argument("--workload", choices=tuple(WORKLOADS), default="methanol",
+                    help="methanol: the headline workload (N=2000, d=12, swarm 50); water: N=500, d=3, swarm 20; "
+                         "nma: N=3000, d=30, optimised nugget, swarm 32; ibuprofen: N=6000, d=93, linear mean, "
+                         "concentrated likelihood, swarm 64")
+    ap.add_argument("--n", type=int, default=None, help="training points (default: the workload's)")
+    ap.add_argument("--swarm", type=int, default=None, help="particles (default: the workload's)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="bound on the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -309,20 +325,23 @@ def main():
         print(json.dumps({"copy_bytes": args.copy_bytes, "copies": 5}))
         return
 
-    ibu = args.workload == "ibuprofen"
+    make, n_default, s_default, seed, desc = WORKLOADS[args.workload]
     if args.n is None:
-        args.n = 6000 if ibu else 2000
+        args.n = n_default
     if args.swarm is None:
-        args.swarm = 64 if ibu else 50
-    w = synthetic.ibuprofen(n=args.n, seed=4) if ibu else synthetic.methanol(n=args.n, seed=0)
+        args.swarm = s_default
+    w = getattr(synthetic, make)(n=args.n, seed=seed)
     d = w.x.shape[1]
     prog = compile_kernel(w.kernel, w.kernels, d)
     mfit = fit_mean(w.mean, w.x, w.y)
     S = args.swarm
-    gp = SwarmLikelihood(w.x, w.y, prog, mean=mfit, noise=w.noise, likelihood=w.likelihood,
-                         max_swarm=S)
+    gp = SwarmLikelihood(w.x, w.y, prog, mean=mfit, noise=w.noise, optimise_noise=w.optimise_noise,
+                         likelihood=w.likelihood, max_swarm=S)
     rng = np.random.default_rng(1)
     thetas = [rng.uniform(w.search_min, w.search_max, (S, gp.nparams)) for _ in range(4)]
+    if w.optimise_noise:   # the nugget, last, drawn from its own search box
+        for th in thetas:
+            th[:, -1] = rng.uniform(*w.extra.get("noise_search", (1e-12, 1e-6)), S)
 
     torch.cuda.init()
     stream = torch.cuda.Stream()              # default priority
@@ -387,7 +406,7 @@ def main():
                        "duration = the factorisation's span")
     fact_flops = S * potrf_flops(n)
     factor_path = gp.factor_path(S)
-    default_size = (n, S) == ((6000, 64) if ibu else (2000, 50))
+    default_size = (n, S) == (n_default, s_default)
     traffic, traffic_src = (profiled_traffic(factor_path, chained, groups, args.workload)
                             if default_size else (None, None))
     build_ms = span["build"][0] / prof_steps
@@ -406,11 +425,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": f"synthetic (seeded {w.name} ALF features, Gaussian-bump target)",
-        "config": {"workload": (f"ibuprofen-like N={n} d={d} swarm={S} rbf(63)+periodic(30) linear mean, "
-                                f"concentrated, nugget 1e-10, theta~U[0,1)") if ibu else
-                               (f"methanol-like N={n} d={d} swarm={S} rbf(9)+periodic(3) "
-                                f"constant mean, marginal, nugget 1e-10, theta~U[0,1)"),
+        "data": (f"synthetic (seeded {w.name} geometry: r1, r2 and angle, Morse-potential target)"
+                 if args.workload == "water" else f"synthetic (seeded {w.name} ALF features, Gaussian-bump target)"),
+        "config": {"workload": desc.format(n=n, d=d, S=S),
                    "path": "batched (hipGraph replay per swarm evaluation, "
                            + ("persistent factorisation)" if persistent else "lock-step factorisation)"),
                    "factor_path": factor_path,
@@ -459,7 +476,8 @@ def main():
         cores = host["threads_used"]
         spec = oracle.Spec.from_program(prog)
         lapack_path = oracle.scipy_openblas_path()
-        bl = oracle.Baseline(w.x, w.y, mfit.value, spec, noise=w.noise, lapack_path=lapack_path)
+        bl = oracle.Baseline(w.x, w.y, mfit.value, spec, noise=w.noise, optimise_noise=w.optimise_noise,
+                             likelihood=w.likelihood, lapack_path=lapack_path)
         pool = np.concatenate([np.atleast_2d(tt) for tt in thetas])
         bl.swarm(pool[:cores], threads=cores)               # first touch of the tensors, untimed
         done, t = 0, time.perf_counter()
