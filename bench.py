@@ -43,6 +43,7 @@ CHAINED_PROFILES = {
                                        "lockstep_factor"),
     ("ibuprofen", "lockstep-kc16-per-panel", 2): (REPO / "profiles" / "ibuprofen_build_and_factor_n6000_s64.md",
                                                   "lockstep_factor"),
+    ("ibuprofen", "lockstep-kc16", 2): (REPO / "profiles" / "ibuprofen_chained_groups2_n6000_s64.md", "lockstep_factor"),
     ("nma", "lockstep-kc16-per-panel", 2): (REPO / "profiles" / "nma_build_and_factor_n3000_s32.md", "lockstep_factor"),
     ("water", "lockstep-kc16", 1): (REPO / "profiles" / "water_build_and_factor_n500_s20.md", "lockstep_factor"),
 }
