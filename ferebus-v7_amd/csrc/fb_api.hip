@@ -1111,7 +1111,10 @@ static int choose_path(fb_handle *h, int s)
                 apply(*b);
                 tb = std::min(tb, time_eval());
             }
-            const Cand &w = tb < ta ? *b : a;
+            // as in the group loop, more stream groups must win by 1 %
+            const float pa = a.groups > b->groups ? 0.99f : 1.0f;
+            const float pb = b->groups > a.groups ? 0.99f : 1.0f;
+            const Cand &w = tb * pa < ta * pb ? *b : a;
             apply(w);
             best = std::min(ta, tb);
             best_mode = w.mode;
