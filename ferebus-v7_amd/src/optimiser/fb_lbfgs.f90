@@ -2,7 +2,8 @@
 !> -logL, driven by the adjoint gradient of the gfx950 library
 !> (fb_swarm_loglik_grad).  The reference PSO has no gradient step (the
 !> gradient flags of gpr.f90:272-282 are parsed but unused); this is an
-!> optional addition, off by default, so plain PSO stays the judge.
+!> optional addition, off by default: plain PSO stays the default and the
+!> baseline its results are compared with.
 !>
 !> Limited-memory BFGS with the two-loop recursion (Nocedal 1980) and
 !> H0 = (s'y / y'y) I from the newest pair (1 / |g| on the first step), and a
