@@ -25,7 +25,8 @@ module ferebus_hip
   end type fb_kernel_spec
 
   public :: fb_create, fb_destroy, fb_nparams, fb_set_stream, fb_swarm_loglik, &
-            fb_swarm_loglik_async, fb_particle_loglik, fb_get_alpha, fb_swarm_loglik_grad, &
+            fb_swarm_loglik_async, fb_swarm_loglik_static, fb_particle_loglik, fb_get_alpha, &
+            fb_swarm_loglik_grad, fb_debug_matrix, &
             fb_multi_create, fb_multi_swarm_loglik, fb_multi_destroy, fb_multi_schedule, fb_multi_pso_step, &
             fb_pso_create, fb_pso_destroy, fb_pso_init, fb_pso_step, fb_pso_retry, fb_pso_select, &
             fb_pso_set_gbest, fb_pso_positions, fb_pso_set_coefficients, &
@@ -77,6 +78,26 @@ module ferebus_hip
       integer(c_int), value :: s
       type(c_ptr), value :: d_theta, d_ll, d_info
     end function fb_swarm_loglik_async
+
+    ! no allocation: the library's fixed scratch array on the device (for
+    ! callers that cannot allocate); stream = c_null_ptr for the handle's
+    integer(c_int) function fb_swarm_loglik_static(h, s, d_theta, d_ll, d_info, stream) &
+        bind(c, name="fb_swarm_loglik_static")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: s
+      type(c_ptr), value :: d_theta, d_ll, d_info, stream
+    end function fb_swarm_loglik_static
+
+    ! one particle's covariance (stage 0) or Cholesky factor (stage 1), n x n
+    integer(c_int) function fb_debug_matrix(h, theta, stage, out, info) bind(c, name="fb_debug_matrix")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: theta(*)
+      integer(c_int), value :: stage
+      real(c_double), intent(out) :: out(*)
+      integer(c_int), intent(out) :: info
+    end function fb_debug_matrix
 
     integer(c_int) function fb_particle_loglik(h, s, theta, ll, info) bind(c, name="fb_particle_loglik")
       import :: c_int, c_double, c_ptr
