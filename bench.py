@@ -23,6 +23,13 @@ from pathlib import Path
 
 import numpy as np
 
+# Kernel arguments in device memory: the ~40 dependent launches of one
+# evaluation start sooner (+1.2-1.8 % evaluations/s at the default workload,
+# three interleaved runs).  Read when the HIP runtime initialises, so set
+# before torch or the library touch the GPU; a value already in the
+# environment is kept.
+os.environ.setdefault("HIP_FORCE_DEV_KERNARG", "1")
+
 REPO = Path(__file__).resolve().parent
 sys.path.insert(0, str(REPO / "ferebus-v7_amd"))
 sys.path.insert(0, str(REPO))
