@@ -4,10 +4,11 @@ Python host mirror of the C ABI in include/ferebus_hip.h.  The arithmetic
 runs in libferebus_hip.so (hand-written HIP for gfx950); this package builds
 inputs (kernel programs, mean vectors) and calls it.
 
-Importing the package before the process first uses the GPU also asks the
-HIP runtime for kernel arguments in device memory (HIP_FORCE_DEV_KERNARG=1,
-unless the environment already sets it): the factorisation's dependent
-launches then start sooner.  Imported later, this has no effect.
+Imported before torch or anything else that loads the HIP runtime, the
+package also asks the runtime for kernel arguments in device memory
+(HIP_FORCE_DEV_KERNARG=1, unless the environment already sets it): the
+factorisation's dependent launches then start sooner.  Imported later, it
+may have no effect.
 """
 import os as _os
 
