@@ -55,16 +55,8 @@ CHAINED_PROFILES = {
     ("water", "lockstep-kc16", 1): (REPO / "profiles" / "water_build_and_factor_n500_s20.md", "lockstep_factor"),
 }
 PROFILES = {
-    ("lockstep-kc16-per-panel", 1): (REPO / "profiles" / "chained_factor_per_panel_methanol_n2000_s50.md",
-                                     "lockstep_factor"),
-    ("lockstep-kc16", 2): (REPO / "profiles" / "chained_factor_groups2_methanol_n2000_s50.md",
-                           "lockstep_factor"),
-}
-PROFILES = {
     "persistent": (REPO / "profiles" / "dag_factor_kernel_methanol_n2000_s50.md", "dag_factor_kernel"),
     "lockstep-kc16": (REPO / "profiles" / "lockstep_factor_methanol_n2000_s50.md", "lockstep_factor"),
-    "lockstep-kc16-32x128": (REPO / "profiles" / "lockstep_factor_32x128_methanol_n2000_s50.md", "lockstep_factor"),
-    "lockstep-kc16-32x64": (REPO / "profiles" / "lockstep_factor_32x64_methanol_n2000_s50.md", "lockstep_factor"),
     "lockstep-kc16-per-panel": (REPO / "profiles" / "lockstep_factor_per_panel_methanol_n2000_s50.md",
                                 "lockstep_factor"),
 }
@@ -318,7 +310,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the swarm evaluation is not sharded across GPUs "
-              "in this build (run with --gpus 1)", file=sys.stderr)
+              "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
     import torch
