@@ -100,6 +100,10 @@ __device__ inline double cyclic_wrap_abs(double diff) {
     return fmod(fabs(diff) + kPi, kTwoPi) - kPi;
 }
 
+// Right-hand sides bordered below the matrix: y - m, and the ones column of
+// the concentrated likelihood.
+constexpr int kMaxRhs = 2;
+
 struct BuildArgs {
     int n, np, nrhs, S, n_slots;
     const double *slots;   // n_slots x n: feature value of point i at slots[slot*n + i]
@@ -157,12 +161,22 @@ __global__ void __launch_bounds__(kThreads) build_kernel(BuildArgs a)
     const int nsp = min(SP, a.S - s0);
     const KernelProgram &p = a.prog;
     const int hs = hyper_stride(p);
+    // the right-hand sides of the tile's columns (bordered rows n ..
+    // n + nrhs - 1), staged with the features: a global load inside the
+    // column loop would make every column wait until the stores of the
+    // columns before it had completed (loads and stores share one in-order
+    // counter)
+    double *rhs_l = hyp_t + (size_t)SP * hs;
 
     for (int e = threadIdx.x; e < a.n_slots * TB; e += kThreads) {
         const int slot = e / TB, r = e % TB;
         const int gi = ti * TB + r, gj = tj * TB + r;
         row_f[e] = gi < a.n ? a.slots[(size_t)slot * a.n + gi] : 0.0;
         col_f[e] = gj < a.n ? a.slots[(size_t)slot * a.n + gj] : 0.0;
+    }
+    for (int e = threadIdx.x; e < a.nrhs * TB; e += kThreads) {
+        const int gj = tj * TB + e % TB;
+        rhs_l[e] = gj < a.n ? a.rhs[(size_t)(e / TB) * a.n + gj] : 0.0;
     }
     for (int e = threadIdx.x; e < SP * hs; e += kThreads) {
         const int k = e / SP, sp = min(s0 + e % SP, a.S - 1);
@@ -260,7 +274,7 @@ __global__ void __launch_bounds__(kThreads) build_kernel(BuildArgs a)
             // bordered rows (right-hand sides) and the identity padding block
             double v = 0.0;
             if (j < a.n) {
-                if (i < a.n + a.nrhs) v = a.rhs[(size_t)(i - a.n) * a.n + j];
+                if (i < a.n + a.nrhs) v = rhs_l[(i - a.n) * TB + jc];
             } else if (i == j) {
                 v = 1.0;
             }
@@ -307,10 +321,16 @@ __global__ void __launch_bounds__(kThreads) build_wide_kernel(BuildArgs a)
     const int nsp = min(SP, a.S - s0);
     const KernelProgram &p = a.prog;
 
+    // the right-hand sides of the tile's columns, staged as in build_kernel
+    double *rhs_l = row_f + (size_t)a.n_slots * TB;
     for (int e = threadIdx.x; e < a.n_slots * TB; e += kThreads) {
         const int slot = e / TB, r = e % TB;
         const int gi = ti * TB + r;
         row_f[e] = gi < a.n ? a.slots[(size_t)slot * a.n + gi] : 0.0;
+    }
+    for (int e = threadIdx.x; e < a.nrhs * TB; e += kThreads) {
+        const int gj = tj * TB + e % TB;
+        rhs_l[e] = gj < a.n ? a.rhs[(size_t)(e / TB) * a.n + gj] : 0.0;
     }
     if (b == 0 && threadIdx.x < nsp) {
         a.info[s0 + threadIdx.x] = 0;
@@ -440,7 +460,7 @@ __global__ void __launch_bounds__(kThreads) build_wide_kernel(BuildArgs a)
             } else {
                 double v = 0.0;
                 if (j < a.n) {
-                    if (i < a.n + a.nrhs) v = a.rhs[(size_t)(i - a.n) * a.n + j];
+                    if (i < a.n + a.nrhs) v = rhs_l[(i - a.n) * TB + (j - tj * TB)];
                 } else if (i == j) {
                     v = 1.0;
                 }
@@ -456,7 +476,14 @@ __global__ void __launch_bounds__(kThreads) build_wide_kernel(BuildArgs a)
 // the hyperparameter records of its SP particles.
 static size_t build_lds_bytes(int TB, int n_slots, int SP, const KernelProgram &p)
 {
-    return ((size_t)2 * n_slots * TB + (size_t)SP * hyper_stride(p)) * sizeof(double);
+    return ((size_t)2 * n_slots * TB + (size_t)SP * hyper_stride(p) + (size_t)kMaxRhs * TB) * sizeof(double);
+}
+
+// Dynamic LDS of a build_wide_kernel work-group: the row features of its
+// 64-row tile and the right-hand sides of its columns.
+static size_t wide_lds_bytes(int n_slots)
+{
+    return ((size_t)n_slots * 64 + (size_t)kMaxRhs * 64) * sizeof(double);
 }
 
 template <int TB, int MAXL, int SP>
@@ -493,7 +520,7 @@ static void launch_wide_cb(const BuildArgs &a, hipStream_t st)
 {
     const int nt = a.np / 64;
     dim3 grid(nt * (nt + 1) / 2, ceil_div(a.S, SP));
-    const size_t shm = (size_t)a.n_slots * 64 * sizeof(double);
+    const size_t shm = wide_lds_bytes(a.n_slots);
     if (shm > 65536)
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&build_wide_kernel<MAXL, SP, CB>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
@@ -522,8 +549,8 @@ static bool use_wide_build(const BuildArgs &a)
         if (!env) return -1;
         return std::string(env) == "wide" ? 1 : (std::string(env) == "narrow" ? 0 : -1);
     }();
-    if (forced >= 0) return forced == 1 && (size_t)a.n_slots * 64 * sizeof(double) <= 160 * 1024;
-    return build_lds_bytes(64, a.n_slots, 10, a.prog) > 65536 && (size_t)a.n_slots * 64 * sizeof(double) <= 160 * 1024;
+    if (forced >= 0) return forced == 1 && wide_lds_bytes(a.n_slots) <= 160 * 1024;
+    return build_lds_bytes(64, a.n_slots, 10, a.prog) > 65536 && wide_lds_bytes(a.n_slots) <= 160 * 1024;
 }
 
 void launch_build(const SwarmBuffers &b, hipStream_t st)
