@@ -33,6 +33,24 @@ def test_header_lists_known_entry_points():
     assert declared_entry_points() == sorted(_abi.EXPORTS)
 
 
+def test_fortran_module_binds_every_entry_point():
+    """The host's binding module (the reference-side binding a maintainer
+    adds) has a bind(c) interface for every declared entry point, and the
+    binding excerpt in INTEGRATION.md names each one."""
+    module = (REPO / "ferebus-v7_amd" / "src" / "ferebus_hip.f90").read_text()
+    bound = set(re.findall(r'bind\(c,\s*name="(fb_[a-z_0-9]+)"\)', module))
+    assert set(declared_entry_points()) <= bound, sorted(set(declared_entry_points()) - bound)
+    integration = (REPO / "INTEGRATION.md").read_text()
+    integration = re.sub(r"/\s*\n\s*!\s*", "/ ", integration)   # Fortran comment continuation
+    named = set(re.findall(r"\b(fb_[a-z_0-9]+)", integration))
+    # the excerpt abbreviates families ("fb_pso_create / init / retry"):
+    # expand "fb_<family>_<a> / <b>" into fb_<family>_<b>
+    for fam, rest in re.findall(r"\b(fb_[a-z]+)_[a-z_]+((?:\s*/\s*[a-z_]+)+)", integration):
+        named |= {f"{fam}_{w.strip()}" for w in rest.split("/") if w.strip()}
+    missing = set(declared_entry_points()) - named
+    assert not missing, sorted(missing)
+
+
 def test_library_exports_every_declared_symbol():
     lib = _abi.load()
     for name in declared_entry_points():
